@@ -1,0 +1,579 @@
+"""Approximate-EP sparse GP models on the GPU: SGPR, SDGPR, SGPLVM, SGPSSM.
+
+Drop-in for geepee/aep_models.py:589-1437 (with the model bases of
+geepee/base_models.py:15-165, 661-1752): same constructors, ``objective_function(params,
+mb_size, alpha, prop_mode) -> (energy, grad_dict)``, ``predict_f`` / ``predict_y``,
+``optimise``, ``get_hypers`` / ``update_hypers``.  Parameters may be numpy arrays (the
+reference's contract: uploaded in one copy, gradients returned as numpy in one copy)
+or device tensors (``objective_function_device`` keeps everything resident).
+Minibatches are drawn on the host with numpy's legacy global RNG exactly as the
+reference does, so the same seed selects the same points.
+"""
+import pickle
+
+import numpy as np
+import torch
+
+from . import _lib, dops
+from .config import PROP_MM, PROP_MC
+from .dops import mm, lincomb, colsum
+from .layers import SGP_Layer
+from .utils import ObjectiveWrapper, flatten_dict, unflatten_dict, adam
+
+F64 = torch.float64
+
+
+class DeviceParams(object):
+    """A params dict laid out as one flat device buffer in sorted-key order (the order of
+    geepee/utils.py:68-81), with one view per key."""
+
+    def __init__(self, shapes, device='cuda'):
+        self.keys = sorted(shapes)
+        self.shapes = {k: tuple(shapes[k]) for k in self.keys}
+        self.sizes = [int(np.prod(self.shapes[k])) for k in self.keys]
+        self.total = int(sum(self.sizes))
+        self.buf = torch.empty(self.total, dtype=F64, device=device)
+        self.views = {}
+        o = 0
+        for k, n in zip(self.keys, self.sizes):
+            self.views[k] = self.buf[o:o + n].view(self.shapes[k])
+            o += n
+
+    def upload(self, params):
+        vec = np.concatenate([np.asarray(params[k], dtype=np.float64).ravel() for k in self.keys])
+        self.buf.copy_(torch.from_numpy(vec))
+        return self.views
+
+    def __getitem__(self, k):
+        return self.views[k]
+
+
+def _shapes(params):
+    return {k: np.shape(v) for k, v in params.items()}
+
+
+class Gauss_Layer(object):
+    """Gaussian likelihood (geepee/lik_layers.py:87-282) on the GPU."""
+
+    def __init__(self, N, D):
+        self.N = N
+        self.D = D
+
+    def compute_log_Z(self, mout, vout, y, alpha=1.0, scale=1.0, want_dmall=False):
+        """Returns (dm, dv, dm_all, scal) with dm, dv scaled by ``scale`` and
+        scal = [logZ, dsn * scale].  Unlike lik_layers.py:122 the caller's vout is not
+        modified in place (nothing on the path reads the mutated value)."""
+        return dops.gauss_tilted(mout, vout, y, self.sn, alpha, scale, want_dmall)
+
+    def output_probabilistic(self, mf, vf, alpha=1.0):
+        sn2 = float(np.exp(2.0 * np.asarray(self.sn_host).ravel()[0]))
+        out = torch.empty_like(vf)
+        ones = torch.ones(1, dtype=F64, device=vf.device)
+        return mf, lincomb(out, 1.0, vf, sn2 / alpha, ones)
+
+    def update_hypers(self, params, key_suffix=''):
+        self.sn = params['sn' + key_suffix]
+
+
+class Base_Model(object):
+    """Common driver (geepee/base_models.py:15-165)."""
+
+    def __init__(self, y_train):
+        self.y_train = np.asarray(y_train, dtype=np.float64)
+        self.N = self.y_train.shape[0]
+        self.fixed_params = []
+        self.updated = False
+        self._dp = None
+        _lib.require_device()
+        self.y_dev = torch.as_tensor(self.y_train, device='cuda')
+
+    # parameter transfer -------------------------------------------------
+    def _device_params(self, params):
+        if isinstance(next(iter(params.values())), torch.Tensor):
+            return params
+        shapes = _shapes(params)
+        if self._dp is None or self._dp.shapes != {k: tuple(v) for k, v in shapes.items()}:
+            self._dp = DeviceParams(shapes)
+        self._dp.upload(params)
+        self._params_host = params
+        return self._dp.views
+
+    def _finish(self, energy_dev, grads_dev, params, divide_by):
+        """Zero fixed params, divide by N, return (float, numpy dict) with one D2H copy."""
+        keys = sorted(grads_dev)
+        for p in self.fixed_params:
+            grads_dev[p] = torch.zeros_like(grads_dev[p])
+        flat = torch.cat([energy_dev.reshape(-1)] + [grads_dev[k].reshape(-1) for k in keys])
+        if divide_by is not None:
+            lincomb(flat, 1.0 / divide_by, flat)
+        out = flat.cpu().numpy()
+        energy = float(out[0])
+        grads = {}
+        o = 1
+        for k in keys:
+            shape = np.shape(params[k])
+            n = int(np.prod(shape))
+            grads[k] = out[o:o + n].reshape(shape)
+            o += n
+        return energy, grads
+
+    def _check_info(self, layers):
+        bad = []
+        for L in layers:
+            for name in ('info_Kuu', 'info_Su', 'info_Suhat'):
+                t = getattr(L, name, None)
+                if t is not None and int(t.max().item()) > 0:
+                    bad.append(name)
+        if bad:
+            raise np.linalg.LinAlgError('matrix not positive definite: %s' % ', '.join(bad))
+
+    # optimisation (base_models.py:59-134) --------------------------------
+    def optimise(self, method='L-BFGS-B', tol=None, reinit_hypers=True, callback=None,
+                 maxfun=100000, maxiter=1000, alpha=0.5, mb_size=None, adam_lr=0.001,
+                 prop_mode=PROP_MM, disp=True, return_cost=False, **kargs):
+        self.updated = False
+        init_params_dict = self.init_hypers(self.y_train) if reinit_hypers else self.get_hypers()
+        init_params_vec, params_args = flatten_dict(init_params_dict)
+        objective_wrapper = ObjectiveWrapper()
+        if mb_size is None:
+            mb_size = self.N
+        costs = []
+        try:
+            if method.lower() == 'adam':
+                results = adam(objective_wrapper, init_params_vec, step_size=adam_lr,
+                               maxiter=maxiter, args=(params_args, self, mb_size, alpha, prop_mode),
+                               disp=disp, callback=callback, return_cost=return_cost)
+                if return_cost:
+                    final_params, costs = results
+                else:
+                    final_params = results
+            else:
+                from scipy.optimize import minimize
+                options = {'maxfun': maxfun, 'maxiter': maxiter, 'disp': disp, 'gtol': 1e-6,
+                           'ftol': 1e-6}
+                results = minimize(fun=objective_wrapper, x0=init_params_vec,
+                                   args=(params_args, self, self.N, alpha, prop_mode),
+                                   method=method, jac=True, tol=tol, callback=callback,
+                                   options=options)
+                final_params = results.x
+        except KeyboardInterrupt:
+            print('Caught KeyboardInterrupt ...')
+            final_params = objective_wrapper.previous_x
+            costs = []
+        final_params = unflatten_dict(final_params, params_args)
+        self.update_hypers(final_params)
+        if return_cost and method.lower() == 'adam':
+            return final_params, costs
+        return final_params
+
+    def set_fixed_params(self, params):
+        if isinstance(params, (list, tuple)):
+            for p in params:
+                if p not in self.fixed_params:
+                    self.fixed_params.append(p)
+        else:
+            self.fixed_params.append(params)
+
+    def save_model(self, fname='/tmp/model.pickle'):
+        with open(fname, 'wb') as f:
+            pickle.dump({k: np.asarray(v) for k, v in self.get_hypers().items()}, f)
+
+    def load_model(self, fname='/tmp/model.pickle'):
+        with open(fname, 'rb') as f:
+            params = pickle.load(f)
+        self.update_hypers(params)
+
+    def get_hypers(self):
+        return {k: np.array(v, copy=True) for k, v in self._params_host.items()}
+
+
+def _check_mode(prop_mode):
+    if prop_mode == PROP_MC:
+        raise NotImplementedError('Monte Carlo propagation is not on the GPU path')
+    if prop_mode != PROP_MM:
+        raise NotImplementedError('propagation mode not implemented')
+
+
+def _check_lik(lik):
+    if lik.lower() != 'gaussian':
+        raise NotImplementedError('likelihood %s is not on the GPU path' % lik)
+
+
+class SGPR(Base_Model):
+    """aep_models.py:589-667 + base_models.py:932-1070."""
+
+    def __init__(self, x_train, y_train, no_pseudo, lik='Gaussian', nat_param=True):
+        super(SGPR, self).__init__(y_train)
+        _check_lik(lik)
+        self.x_train = np.asarray(x_train, dtype=np.float64)
+        self.Din = self.x_train.shape[1]
+        self.Dout = self.y_train.shape[1]
+        self.M = no_pseudo
+        self.nat_param = nat_param
+        self.x_dev = torch.as_tensor(self.x_train, device='cuda')
+        self.lik_layer = Gauss_Layer(self.N, self.Dout)
+        self.sgp_layer = SGP_Layer(self.N, self.Din, self.Dout, self.M, nat_param)
+
+    def update_hypers(self, params):
+        dp = self._device_params(params)
+        self.sgp_layer.update_hypers(dp)
+        self.lik_layer.update_hypers(dp)
+        self.lik_layer.sn_host = np.asarray(params['sn']) if not isinstance(params['sn'], torch.Tensor) \
+            else params['sn'].detach().cpu().numpy()
+        return dp
+
+    def objective_function(self, params, mb_size, alpha=1.0, prop_mode=PROP_MM):
+        _check_mode(prop_mode)
+        N = self.N
+        if mb_size >= N:
+            xb, yb = self.x_dev, self.y_dev
+        else:
+            idxs = torch.as_tensor(np.random.choice(N, mb_size, replace=False), device='cuda')
+            xb, yb = self.x_dev.index_select(0, idxs), self.y_dev.index_select(0, idxs)
+        B = yb.shape[0]
+        scale_logZ = -N * 1.0 / B / alpha
+        self.update_hypers(params)
+        layer = self.sgp_layer
+        layer.compute_cavity(alpha)
+        mout, vout, kfu = layer.forward_prop_thru_cav(xb)
+        dm, dv, _, scal = self.lik_layer.compute_log_Z(mout, vout, yb, alpha, scale_logZ)
+        grads = layer.backprop_grads_reg(mout, vout, dm, dv, kfu, xb, alpha)
+        grads['sn'] = scal[1:2]
+        phi = layer.compute_phi(alpha)
+        energy = lincomb(torch.empty(1, dtype=F64, device='cuda'), scale_logZ, scal[0:1], 1.0, phi)
+        self._check_info([layer])
+        return self._finish(energy, grads, params, N)
+
+    def predict_f(self, inputs):
+        if not self.updated:
+            self.updated = True
+        x = torch.as_tensor(np.asarray(inputs, dtype=np.float64), device='cuda')
+        mf, vf = self.sgp_layer.forward_prop_thru_post(x)
+        return mf.cpu().numpy(), vf.cpu().numpy()
+
+    def predict_y(self, inputs):
+        x = torch.as_tensor(np.asarray(inputs, dtype=np.float64), device='cuda')
+        mf, vf = self.sgp_layer.forward_prop_thru_post(x)
+        my, vy = self.lik_layer.output_probabilistic(mf, vf)
+        return my.cpu().numpy(), vy.cpu().numpy()
+
+    def init_hypers(self, y_train=None):
+        from .workloads import layer_params
+        rs = np.random.RandomState(0)
+        p = layer_params(rs, self.Din, self.Dout, self.M, x=self.x_train)
+        p['sn'] = np.log(0.01) * np.ones(1)
+        return p
+
+
+class SDGPR(Base_Model):
+    """aep_models.py:870-988 + base_models.py:1073-1336; layer i keys end in '_i'."""
+
+    def __init__(self, x_train, y_train, no_pseudos, hidden_sizes, lik='Gaussian'):
+        super(SDGPR, self).__init__(y_train)
+        _check_lik(lik)
+        self.x_train = np.asarray(x_train, dtype=np.float64)
+        self.Din = self.x_train.shape[1]
+        self.Dout = self.y_train.shape[1]
+        self.size = [self.Din] + list(hidden_sizes) + [self.Dout]
+        self.L = len(self.size) - 1
+        self.Ms = list(no_pseudos) if isinstance(no_pseudos, (list, tuple)) else [no_pseudos] * self.L
+        self.x_dev = torch.as_tensor(self.x_train, device='cuda')
+        self.lik_layer = Gauss_Layer(self.N, self.Dout)
+        self.sgp_layers = [SGP_Layer(self.N, self.size[i], self.size[i + 1], self.Ms[i])
+                           for i in range(self.L)]
+
+    def update_hypers(self, params):
+        dp = self._device_params(params)
+        for i, layer in enumerate(self.sgp_layers):
+            layer.update_hypers(dp, key_suffix='_%d' % i)
+        self.lik_layer.update_hypers(dp)
+        self.lik_layer.sn_host = np.asarray(params['sn']) if not isinstance(params['sn'], torch.Tensor) \
+            else params['sn'].detach().cpu().numpy()
+        return dp
+
+    def objective_function(self, params, mb_size, alpha=1.0, prop_mode=PROP_MM):
+        _check_mode(prop_mode)
+        N = self.N
+        if mb_size >= N:
+            xb, yb = self.x_dev, self.y_dev
+        else:
+            idxs = torch.as_tensor(np.random.choice(N, mb_size, replace=False), device='cuda')
+            xb, yb = self.x_dev.index_select(0, idxs), self.y_dev.index_select(0, idxs)
+        B = yb.shape[0]
+        scale_logZ = -N * 1.0 / B / alpha
+        self.update_hypers(params)
+        for layer in self.sgp_layers:
+            layer.compute_cavity(alpha)
+        ms, vs, p1, p2 = [], [], [], []
+        for i, layer in enumerate(self.sgp_layers):
+            if i == 0:
+                m, v, kfu = layer.forward_prop_thru_cav(xb)
+                ms.append(m); vs.append(v); p1.append(kfu); p2.append(None)
+            else:
+                m, v, a, b = layer.forward_prop_thru_cav(ms[i - 1], vs[i - 1])
+                ms.append(m); vs.append(v); p1.append(a); p2.append(b)
+        last_mm = self.L > 1
+        dmi, dvi, dma, scal = self.lik_layer.compute_log_Z(ms[-1], vs[-1], yb, alpha, scale_logZ,
+                                                           want_dmall=last_mm)
+        grads = {}
+        for i in range(self.L - 1, -1, -1):
+            layer = self.sgp_layers[i]
+            if i == 0:
+                g = layer.backprop_grads_reg(ms[0], vs[0], dmi, dvi, p1[0], xb, alpha)
+            else:
+                g, gin = layer.backprop_grads_lvm_mm(ms[i], vs[i], dmi, dvi, p1[i], p2[i],
+                                                     ms[i - 1], vs[i - 1], alpha, dm_all=dma)
+                dmi, dvi, dma = gin['mx'], gin['vx'], None
+                p2[i] = None
+            for k, val in g.items():
+                grads[k + '_%d' % i] = val
+        grads['sn'] = scal[1:2]
+        energy = lincomb(torch.empty(1, dtype=F64, device='cuda'), scale_logZ, scal[0:1])
+        for layer in self.sgp_layers:
+            lincomb(energy, 1.0, energy, 1.0, layer.compute_phi(alpha))
+        self._check_info(self.sgp_layers)
+        return self._finish(energy, grads, params, N)
+
+    def predict_f(self, inputs):
+        x = torch.as_tensor(np.asarray(inputs, dtype=np.float64), device='cuda')
+        mf, vf = self._predict_dev(x)
+        return mf.cpu().numpy(), vf.cpu().numpy()
+
+    def _predict_dev(self, x):
+        mf = vf = None
+        for i, layer in enumerate(self.sgp_layers):
+            if i == 0:
+                mf, vf = layer.forward_prop_thru_post(x)
+            else:
+                mf, vf = layer.forward_prop_thru_post(mf, vf)
+        return mf, vf
+
+    def predict_y(self, inputs):
+        x = torch.as_tensor(np.asarray(inputs, dtype=np.float64), device='cuda')
+        mf, vf = self._predict_dev(x)
+        my, vy = self.lik_layer.output_probabilistic(mf, vf)
+        return my.cpu().numpy(), vy.cpu().numpy()
+
+
+class SGPLVM(Base_Model):
+    """aep_models.py:670-867 + base_models.py:661-929.  The energy is not divided by N
+    (aep_models.py:808-815)."""
+
+    def __init__(self, y_train, hidden_size, no_pseudo, lik='Gaussian', prior_mean=0, prior_var=1,
+                 nat_param=True):
+        super(SGPLVM, self).__init__(y_train)
+        _check_lik(lik)
+        self.Dout = self.y_train.shape[1]
+        self.Din = hidden_size
+        self.M = no_pseudo
+        self.nat_param = nat_param
+        self.prior_mean = prior_mean
+        self.prior_var = prior_var
+        self.lik_layer = Gauss_Layer(self.N, self.Dout)
+        self.sgp_layer = SGP_Layer(self.N, self.Din, self.Dout, self.M, nat_param)
+        from . import xfactors
+        self._xf = xfactors
+
+    def update_hypers(self, params):
+        dp = self._device_params(params)
+        self.sgp_layer.update_hypers(dp)
+        self.lik_layer.update_hypers(dp)
+        self.lik_layer.sn_host = np.asarray(params['sn']) if not isinstance(params['sn'], torch.Tensor) \
+            else params['sn'].detach().cpu().numpy()
+        return dp
+
+    def objective_function(self, params, mb_size, alpha=1.0, prop_mode=PROP_MM):
+        _check_mode(prop_mode)
+        N = self.N
+        if mb_size == N:
+            idx_np = None
+            yb = self.y_dev
+        else:
+            idx_np = np.random.choice(N, mb_size, replace=False)
+            yb = self.y_dev.index_select(0, torch.as_tensor(idx_np, device='cuda'))
+        B = yb.shape[0]
+        scale_logZ = -N * 1.0 / B / alpha
+        dp = self.update_hypers(params)
+        layer = self.sgp_layer
+        layer.compute_cavity(alpha)
+        xs = self._xf.lvm_x_state(dp['x1'], dp['x2'], idx_np, alpha, self.prior_mean,
+                                  self.prior_var, self.nat_param)
+        mout, vout, psi1, psi2 = layer.forward_prop_thru_cav(xs['mcav'], xs['vcav'])
+        dm, dv, dma, scal = self.lik_layer.compute_log_Z(mout, vout, yb, alpha, scale_logZ,
+                                                          want_dmall=True)
+        grads, gin = layer.backprop_grads_lvm_mm(mout, vout, dm, dv, psi1, psi2, xs['mcav'],
+                                                 xs['vcav'], alpha, dm_all=dma)
+        del psi2
+        grads['sn'] = scal[1:2]
+        gx1, gx2, xcontrib = self._xf.lvm_x_grads(xs, gin['mx'], gin['vx'], N, B, alpha,
+                                                 self.prior_mean, self.prior_var, self.Din,
+                                                 self.nat_param)
+        grads['x1'] = gx1
+        grads['x2'] = gx2
+        energy = lincomb(torch.empty(1, dtype=F64, device='cuda'), scale_logZ, scal[0:1], 1.0,
+                         xcontrib, 1.0, layer.compute_phi(alpha))
+        self._check_info([layer])
+        return self._finish(energy, grads, params, None)
+
+    def predict_f(self, inputs):
+        x = torch.as_tensor(np.asarray(inputs, dtype=np.float64), device='cuda')
+        mf, vf = self.sgp_layer.forward_prop_thru_post(x)
+        return mf.cpu().numpy(), vf.cpu().numpy()
+
+    def predict_y(self, inputs):
+        x = torch.as_tensor(np.asarray(inputs, dtype=np.float64), device='cuda')
+        mf, vf = self.sgp_layer.forward_prop_thru_post(x)
+        my, vy = self.lik_layer.output_probabilistic(mf, vf)
+        return my.cpu().numpy(), vy.cpu().numpy()
+
+
+class Gauss_Emis(object):
+    """Linear-Gaussian emission y_t = C x_t + N(0, diag R), R = exp(2 R_param)
+    (geepee/lik_layers.py:474-676) on the GPU."""
+
+    def __init__(self, y, Dout, Din):
+        self.y = y
+        self.N = y.shape[0]
+        self.Dout = Dout
+        self.Din = Din
+
+    def update_hypers(self, params, key_suffix=''):
+        self.C = params['C' + key_suffix]
+        self.Rp = params['R' + key_suffix]
+
+    def compute_emission_tilted(self, mx, vx, alpha, scale, rows):
+        """(dmx, dvx, dC, dR, logZ*scale) for the emission rows [rows)."""
+        return dops.emission(self.C, self.Rp, self.y[rows], mx, vx, alpha, scale)
+
+    def output_probabilistic(self, mf, vf):
+        """my = C mf, diagonal of C diag(vf) C^T + diag(R) (lik_layers.py:558-571)."""
+        return dops.emission_predict(self.C, self.Rp, mf, vf)
+
+
+class SGPSSM(Base_Model):
+    """aep_models.py:991-1437 + base_models.py:1339-1752.  The dynamics layer sees N-1
+    training points (aep_models.py:1018); energy and gradients are divided by N."""
+
+    def __init__(self, y_train, hidden_size, no_pseudo, lik='Gaussian', prior_mean=0, prior_var=1,
+                 x_control=None, gp_emi=False, control_to_emi=True):
+        super(SGPSSM, self).__init__(y_train)
+        _check_lik(lik)
+        self.Dout = self.y_train.shape[1]
+        self.Din = hidden_size
+        self.M = no_pseudo
+        self.gp_emi = gp_emi
+        self.prior_mean = prior_mean
+        self.prior_var = prior_var
+        if x_control is not None:
+            self.x_control = np.asarray(x_control, dtype=np.float64)
+            self.Dcon_dyn = self.x_control.shape[1]
+            self.Dcon_emi = self.Dcon_dyn if control_to_emi else 0
+            self.u_dev = torch.as_tensor(self.x_control, device='cuda')
+        else:
+            self.x_control = None
+            self.Dcon_dyn = self.Dcon_emi = 0
+        self.dyn_layer = SGP_Layer(self.N - 1, self.Din + self.Dcon_dyn, self.Din, self.M)
+        if gp_emi:
+            self.emi_layer = SGP_Layer(self.N, self.Din + self.Dcon_emi, self.Dout, self.M)
+            self.lik_layer = Gauss_Layer(self.N, self.Dout)
+        else:
+            self.emi_layer = Gauss_Emis(self.y_dev, self.Dout, self.Din + self.Dcon_emi)
+
+    def update_hypers(self, params):
+        dp = self._device_params(params)
+        self.dyn_layer.update_hypers(dp, key_suffix='_dynamic')
+        self.emi_layer.update_hypers(dp, key_suffix='_emission')
+        if self.gp_emi:
+            self.lik_layer.update_hypers(dp, key_suffix='_emission')
+        self.sn = dp['sn']
+        return dp
+
+    def _with_control(self, m, v, s, e, D):
+        if D == 0:
+            return m, v
+        u = self.u_dev[s:e]
+        return (torch.cat([m, u], dim=1), torch.cat([v, torch.zeros_like(u)], dim=1))
+
+    def objective_function(self, params, mb_size, alpha=1.0, prop_mode=PROP_MM):
+        _check_mode(prop_mode)
+        N, Din = self.N, self.Din
+        if mb_size >= N:
+            s, e = 0, N
+        else:
+            s = int(np.random.randint(0, N - mb_size))
+            e = s + mb_size
+        Bd, Be = e - 1 - s, e - s
+        scale_dyn = -(N - 1) * 1.0 / Bd / alpha
+        scale_emi = -N * 1.0 / Be / alpha
+        dp = self.update_hypers(params)
+        dyn = self.dyn_layer
+        dyn.compute_cavity(alpha)
+        if self.gp_emi:
+            self.emi_layer.compute_cavity(alpha)
+        cm, cv = self._xf().ssm_cavity(dp['x_factor_1'], dp['x_factor_2'], alpha, self.prior_mean,
+                                       self.prior_var)
+        tm1_m, tm1_v = self._with_control(cm[s:e - 1], cv[s:e - 1], s, e - 1, self.Dcon_dyn)
+        up_m, up_v = self._with_control(cm[s:e], cv[s:e], s, e, self.Dcon_emi)
+        if self.Dcon_dyn == 0:
+            tm1_m, tm1_v = tm1_m.contiguous(), tm1_v.contiguous()
+        if self.Dcon_emi == 0:
+            up_m, up_v = up_m.contiguous(), up_v.contiguous()
+        mprop, vprop, psi1, psi2 = dyn.forward_prop_thru_cav(tm1_m, tm1_v)
+        dmall, dvprop, dmt, dvt, scal_t = dops.transition(mprop, vprop, cm[s + 1:e].contiguous(),
+                                                          cv[s + 1:e].contiguous(), self.sn, alpha,
+                                                          scale_dyn)
+        g_dyn, gin_dyn = dyn.backprop_grads_lvm_mm(mprop, vprop, None, dvprop, psi1, psi2, tm1_m, tm1_v,
+                                                   alpha, dm_all=dmall)
+        del psi2
+        grads = {'sn': scal_t[1:2]}
+        for k, v in g_dyn.items():
+            grads[k + '_dynamic'] = v
+        energy = lincomb(torch.empty(1, dtype=F64, device='cuda'), 1.0, scal_t[0:1], 1.0,
+                         dyn.compute_phi(alpha))
+        if self.gp_emi:
+            emi = self.emi_layer
+            mout, vout, e1, e2 = emi.forward_prop_thru_cav(up_m, up_v)
+            dm, dv, dma, scal_e = self.lik_layer.compute_log_Z(mout, vout, self.y_dev[s:e].contiguous(),
+                                                               alpha, scale_emi, want_dmall=True)
+            g_emi, gin_emi = emi.backprop_grads_lvm_mm(mout, vout, dm, dv, e1, e2, up_m, up_v, alpha,
+                                                       dm_all=dma)
+            del e2
+            for k, v in g_emi.items():
+                grads[k + '_emission'] = v
+            grads['sn_emission'] = scal_e[1:2]
+            lincomb(energy, 1.0, energy, scale_emi, scal_e[0:1], 1.0, emi.compute_phi(alpha))
+            dup, vup = gin_emi['mx'], gin_emi['vx']
+        else:
+            dup, vup, dC, dR, lz = dops.emission(self.emi_layer.C, self.emi_layer.Rp,
+                                                 self.y_dev[s:e], up_m, up_v, alpha, scale_emi)
+            grads['C_emission'] = dC
+            grads['R_emission'] = dR
+            lincomb(energy, 1.0, energy, 1.0, lz)
+        g1, g2, xc = self._xf().ssm_grads(dp['x_factor_1'], dp['x_factor_2'], alpha, self.prior_mean,
+                                          self.prior_var, s, e, dup, vup, dmt, dvt,
+                                          gin_dyn['mx'], gin_dyn['vx'])
+        grads['x_factor_1'] = g1
+        grads['x_factor_2'] = g2
+        lincomb(energy, 1.0, energy, 1.0, xc)
+        self._check_info([dyn] + ([self.emi_layer] if self.gp_emi else []))
+        return self._finish(energy, grads, params, N)
+
+    @staticmethod
+    def _xf():
+        from . import xfactors
+        return xfactors
+
+    def predict_f(self, inputs):
+        x = torch.as_tensor(np.asarray(inputs, dtype=np.float64), device='cuda')
+        mf, vf = self.dyn_layer.forward_prop_thru_post(x)
+        return mf.cpu().numpy(), vf.cpu().numpy()
+
+    def predict_y(self, inputs):
+        x = torch.as_tensor(np.asarray(inputs, dtype=np.float64), device='cuda')
+        mf, vf = self.dyn_layer.forward_prop_thru_post(x)
+        if self.gp_emi:
+            mg, vg = self.emi_layer.forward_prop_thru_post(mf, vf)
+            my, vy = self.lik_layer.output_probabilistic(mg, vg)
+        else:
+            my, vy = self.emi_layer.output_probabilistic(mf, vf)
+        return my.cpu().numpy(), vy.cpu().numpy()

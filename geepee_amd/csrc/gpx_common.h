@@ -1,0 +1,45 @@
+// Shared helpers for the gfx950 FP64 kernels of libgpx.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define GPX_OK 0
+#define GPX_EINVAL (-1)
+#define GPX_EHIP (-2)
+
+#define GPX_CHECK_LAUNCH()                                   \
+  do {                                                       \
+    hipError_t e_ = hipGetLastError();                       \
+    if (e_ != hipSuccess) return GPX_EHIP;                   \
+  } while (0)
+
+static inline unsigned gpx_cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
+
+// Wave64 sum reduction with cross-lane shuffles.
+__device__ __forceinline__ double gpx_wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block-wide sum for blockDim.x a multiple of 64 (<= 1024).  `red` is LDS scratch of
+// at least 16 doubles.  Every thread gets the total.
+__device__ __forceinline__ double gpx_block_sum(double v, double *red) {
+  v = gpx_wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int i = 0; i < nw; ++i) t += red[i];
+  return t;
+}
+
+typedef double gpx_d4 __attribute__((ext_vector_type(4)));
+
+// D(16x16) += A(16x4) * B(4x16) in FP64 on the matrix cores.  Operand layout:
+// lane l supplies A[l%16][l/16] and B[l/16][l%16]; accumulator register r of lane l
+// holds D[l/16 + 4*r][l%16] (measured on gfx950 with an in-bounds probe).
+__device__ __forceinline__ gpx_d4 gpx_mfma16(double a, double b, gpx_d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}

@@ -1,0 +1,801 @@
+// SE kernel matrices, their adjoints, and the small per-layer / likelihood kernels of
+// the AEP objective.  All arrays are row-major FP64 device memory; log
+// hyper-parameters (ls, sf, sn, R) are read from the device so no host sync is needed.
+#include "gpx_common.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// SE kernel: compute_kernel (kernels.py:10-22), K = sf2 exp(-sum_q (x-z)^2 / (2 l_q))
+// ---------------------------------------------------------------------------
+__global__ void se_kernel(int N, int M, int Q, const double *__restrict__ x, int ldx,
+                          const double *__restrict__ z, const double *__restrict__ ls,
+                          const double *__restrict__ sf, double jitter, double *__restrict__ K) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)N * M) return;
+  const int n = (int)(e / M), m = (int)(e % M);
+  double r2 = 0.0;
+  for (int q = 0; q < Q; ++q) {
+    const double d = x[(long)n * ldx + q] - z[m * Q + q];
+    r2 += d * d / exp(2.0 * ls[q]);
+  }
+  double k = exp(2.0 * sf[0]) * exp(-0.5 * r2);
+  if (n == m) k += jitter;
+  K[e] = k;
+}
+
+// kfucompDer (kernels.py:381-399), pass 1 over chunks of 64 rows:
+// partial[blk][m] = [colsum_m, sum_n L x_nq (Q), sum_n L (z_mq - x_nq)^2 (Q)],
+// dx_nq = sum_m L_nm (z_mq - x_nq) / l_q.
+constexpr int KFU_NB = 64;
+__global__ __launch_bounds__(256) void kfu_bwd_partial(int N, int M, int Q, const double *__restrict__ dK,
+                                                       const double *__restrict__ K,
+                                                       const double *__restrict__ x,
+                                                       const double *__restrict__ z,
+                                                       const double *__restrict__ ls,
+                                                       double *__restrict__ part,
+                                                       double *__restrict__ dx) {
+  __shared__ double xs[KFU_NB][17];
+  const int n0 = blockIdx.x * KFU_NB, nb = min(KFU_NB, N - n0), tid = threadIdx.x;
+  for (int e = tid; e < nb * Q; e += 256) xs[e / Q][e % Q] = x[(long)(n0 + e / Q) * Q + e % Q];
+  __syncthreads();
+  const int W = 2 * Q + 1;
+  for (int m = tid; m < M; m += 256) {
+    double cs = 0.0, lx[16], ld2[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) lx[q] = ld2[q] = 0.0;
+    for (int i = 0; i < nb; ++i) {
+      const long o = (long)(n0 + i) * M + m;
+      const double L = dK[o] * K[o];
+      cs += L;
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (q < Q) {
+          const double xv = xs[i][q], d = z[m * Q + q] - xv;
+          lx[q] += L * xv;
+          ld2[q] += L * d * d;
+        }
+    }
+    double *P = part + ((long)blockIdx.x * M + m) * W;
+    P[0] = cs;
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      if (q < Q) {
+        P[1 + q] = lx[q];
+        P[1 + Q + q] = ld2[q];
+      }
+  }
+  if (!dx) return;
+  for (int e = tid; e < nb * Q; e += 256) {
+    const int i = e / Q, q = e % Q;
+    const long row = (long)(n0 + i) * M;
+    double s = 0.0;
+    for (int m = 0; m < M; ++m) s += dK[row + m] * K[row + m] * (z[m * Q + q] - xs[i][q]);
+    dx[(long)(n0 + i) * Q + q] = s / exp(2.0 * ls[q]);
+  }
+}
+
+// pass 2: dZ_mq = -(z_mq colsum_m - Lx_mq)/l_q, dell_q = sum_m Ld2_mq / ell^3,
+// dsf2 = sum colsum / sf2.
+__global__ __launch_bounds__(256) void kfu_bwd_final(int nblk, int M, int Q, const double *__restrict__ part,
+                                                     const double *__restrict__ z,
+                                                     const double *__restrict__ ls,
+                                                     const double *__restrict__ sf,
+                                                     double *__restrict__ dsf2, double *__restrict__ dell,
+                                                     double *__restrict__ dZ) {
+  __shared__ double red[16];
+  const int W = 2 * Q + 1, tid = threadIdx.x;
+  for (int e = tid; e < M * Q; e += 256) {
+    const int m = e / Q, q = e % Q;
+    double cs = 0.0, lx = 0.0;
+    for (int b = 0; b < nblk; ++b) {
+      const double *P = part + ((long)b * M + m) * W;
+      cs += P[0];
+      lx += P[1 + q];
+    }
+    dZ[e] = -(z[e] * cs - lx) / exp(2.0 * ls[q]);
+  }
+  for (int q = 0; q < Q; ++q) {
+    double s = 0.0;
+    for (int m = tid; m < M; m += 256)
+      for (int b = 0; b < nblk; ++b) s += part[((long)b * M + m) * W + 1 + Q + q];
+    s = gpx_block_sum(s, red);
+    if (tid == 0) dell[q] = s / exp(3.0 * ls[q]);
+  }
+  double c = 0.0;
+  for (int m = tid; m < M; m += 256)
+    for (int b = 0; b < nblk; ++b) c += part[((long)b * M + m) * W];
+  c = gpx_block_sum(c, red);
+  if (tid == 0) dsf2[0] = c / exp(2.0 * sf[0]);
+}
+
+// d_trace_MKzz_dhypers (kernels.py:447-475) with Kzz = SE(z, z) (no jitter):
+//   gsf = sum_ab M_ab K_ab, gls_q = sum_ab M_ab K_ab (z_aq - z_bq)^2 / (2 l_q),
+//   gz_aq = -sum_b (M_ab + M_ba) K_ab (z_aq - z_bq) / l_q.
+// One block per row a; part[a] = [gsf_a, gls_a (Q)].
+__global__ __launch_bounds__(256) void dtrace_kernel(int M, int Q, const double *__restrict__ Mm,
+                                                     const double *__restrict__ z,
+                                                     const double *__restrict__ ls,
+                                                     const double *__restrict__ sf,
+                                                     double *__restrict__ gz, double *__restrict__ part) {
+  __shared__ double red[16];
+  const int a = blockIdx.x, tid = threadIdx.x;
+  const double sf2 = exp(2.0 * sf[0]);
+  double il[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) il[q] = (q < Q) ? 1.0 / exp(2.0 * ls[q]) : 0.0;
+  double g0 = 0.0, gl[16], gzq[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) gl[q] = gzq[q] = 0.0;
+  for (int b = tid; b < M; b += 256) {
+    double r2 = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      if (q < Q) {
+        const double d = z[a * Q + q] - z[b * Q + q];
+        r2 += d * d * il[q];
+      }
+    const double K = sf2 * exp(-0.5 * r2);
+    const double mab = Mm[(long)a * M + b], mba = Mm[(long)b * M + a];
+    g0 += mab * K;
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      if (q < Q) {
+        const double d = z[a * Q + q] - z[b * Q + q];
+        gl[q] += 0.5 * mab * K * d * d * il[q];
+        gzq[q] += (mab + mba) * K * d;
+      }
+  }
+  g0 = gpx_block_sum(g0, red);
+  if (tid == 0) part[(long)a * (Q + 1)] = g0;
+  for (int q = 0; q < Q; ++q) {
+    double s = 0.0, t = 0.0;
+#pragma unroll
+    for (int qq = 0; qq < 16; ++qq)
+      if (qq == q) { s = gl[qq]; t = gzq[qq]; }
+    s = gpx_block_sum(s, red);
+    t = gpx_block_sum(t, red);
+    if (tid == 0) {
+      part[(long)a * (Q + 1) + 1 + q] = s;
+      gz[a * Q + q] = -t * il[q];
+    }
+  }
+}
+
+__global__ void dtrace_final(int M, int Q, const double *__restrict__ part, double *__restrict__ gsf,
+                             double *__restrict__ gls) {
+  const int j = threadIdx.x;
+  if (j > Q) return;
+  double s = 0.0;
+  for (int a = 0; a < M; ++a) s += part[(long)a * (Q + 1) + j];
+  if (j == 0) gsf[0] = s; else gls[j - 1] = s;
+}
+
+__global__ void colsum_kernel(int rows, int cols, const double *__restrict__ P, double alpha,
+                              double beta, double *__restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cols) return;
+  double s = 0.0;
+  for (int i = 0; i < rows; ++i) s += P[(long)i * cols + j];
+  out[j] = (beta == 0.0) ? alpha * s : alpha * s + beta * out[j];
+}
+
+// ---------------------------------------------------------------------------
+// q(u) parameter plumbing (base_models.py:645-653; aep_models.py:267-280)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ long triu_index(int M, int i, int j) {
+  return (long)i * M - (long)i * (i - 1) / 2 + (j - i);
+}
+
+__global__ void eta_to_R_kernel(int D, int M, const double *__restrict__ eta, double *__restrict__ R) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long MM = (long)M * M;
+  if (e >= D * MM) return;
+  const int d = (int)(e / MM), i = (int)((e % MM) / M), j = (int)(e % M);
+  double v = 0.0;
+  if (j >= i) {
+    const double t = eta[(long)d * (M * (M + 1) / 2) + triu_index(M, i, j)];
+    v = (i == j) ? exp(t) : t;
+  }
+  R[e] = v;
+}
+
+// out[d, triu(i,j)] = (sum_k R_ik (G_kj + G_jk)) * (i == j ? R_ii : 1)
+__global__ void pack_dR_kernel(int D, int M, const double *__restrict__ R, const double *__restrict__ G,
+                               double *__restrict__ out) {
+  const long K = (long)M * (M + 1) / 2;
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= D * K) return;
+  const int d = (int)(e / K);
+  long p = e % K;
+  int i = 0;
+  while (p >= M - i) { p -= M - i; ++i; }
+  const int j = i + (int)p;
+  const double *Rd = R + (long)d * M * M, *Gd = G + (long)d * M * M;
+  double s = 0.0;
+  for (int k = i; k < M; ++k) s += Rd[(long)i * M + k] * (Gd[(long)k * M + j] + Gd[(long)j * M + k]);
+  if (i == j) s *= Rd[(long)i * M + i];
+  out[e] = s;
+}
+
+__global__ void lincomb_kernel(long n, double a, const double *__restrict__ x, long nx, double b,
+                               const double *__restrict__ y, long ny, double c,
+                               const double *__restrict__ w, long nw, double *__restrict__ out) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double v = a * x[i % nx];
+    if (y) v += b * y[i % ny];
+    if (w) v += c * w[i % nw];
+    out[i] = v;
+  }
+}
+
+__global__ void sym_kernel(int batch, int M, double a, const double *__restrict__ X, double *__restrict__ out) {
+  const long MM = (long)M * M;
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= batch * MM) return;
+  const long base = (e / MM) * MM;
+  const int i = (int)((e % MM) / M), j = (int)(e % M);
+  out[e] = a * (X[base + (long)i * M + j] + X[base + (long)j * M + i]);
+}
+
+// v[n,d] = sf2 + sum_a T[d][n][a] kfu[n][a]   (aep_models.py:155-157)
+__global__ void det_var_kernel(int N, int D, int M, const double *__restrict__ T,
+                               const double *__restrict__ kfu, const double *__restrict__ sf,
+                               double *__restrict__ v) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)N * D) return;
+  const int n = (int)(e / D), d = (int)(e % D);
+  const double *t = T + ((long)d * N + n) * M, *k = kfu + (long)n * M;
+  double s = 0.0;
+  for (int a = 0; a < M; ++a) s += t[a] * k[a];
+  v[e] = exp(2.0 * sf[0]) + s;
+}
+
+// v = sf2 + Bpsi2 - m^2   (aep_models.py:196-198)
+__global__ void mm_var_kernel(long n, const double *__restrict__ m, const double *__restrict__ Bp,
+                              const double *__restrict__ sf, double *__restrict__ v) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  v[e] = exp(2.0 * sf[0]) + Bp[e] - m[e] * m[e];
+}
+
+// out[b] = sum_i x[b][i] y[b][i]
+__global__ void batched_dot_kernel(int batch, int n, const double *__restrict__ x,
+                                   const double *__restrict__ y, double *__restrict__ out) {
+  __shared__ double red[16];
+  const int b = blockIdx.x;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += x[(long)b * n + i] * y[(long)b * n + i];
+  s = gpx_block_sum(s, red);
+  if (threadIdx.x == 0) out[b] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Gauss_Layer.compute_log_Z + backprop_grads (lik_layers.py:104-181), 2-D branch.
+// scal[0] = logZ (with the alpha constant), scal[1] = dsn (scaled).
+// dm/dv outputs are scaled by `scale`; dmall = dm - 2 dv m when requested.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void gauss_tilted_kernel(int B, int D, const double *__restrict__ m,
+                                                            const double *__restrict__ v,
+                                                            const double *__restrict__ y,
+                                                            const double *__restrict__ sn, double alpha,
+                                                            double scale, double *__restrict__ dm,
+                                                            double *__restrict__ dv,
+                                                            double *__restrict__ dmall,
+                                                            double *__restrict__ scal) {
+  __shared__ double red[16];
+  const double sn2 = exp(2.0 * sn[0]);
+  const double two_pi = 6.283185307179586;
+  double lz = 0.0, sdv = 0.0;
+  const long n = (long)B * D;
+  for (long e = threadIdx.x; e < n; e += blockDim.x) {
+    const double vt = v[e] + sn2 / alpha;
+    const double r = y[e] - m[e];
+    lz += -0.5 * (log(two_pi * vt) + r * r / vt);
+    const double gm = r / vt, gv = -0.5 / vt + 0.5 * r * r / (vt * vt);
+    sdv += gv;
+    dm[e] = scale * gm;
+    dv[e] = scale * gv;
+    if (dmall) dmall[e] = scale * gm - 2.0 * scale * gv * m[e];
+  }
+  lz = gpx_block_sum(lz, red);
+  sdv = gpx_block_sum(sdv, red);
+  if (threadIdx.x == 0) {
+    lz += (double)B * D * (0.5 * log(two_pi * sn2 / alpha) - 0.5 * alpha * log(two_pi * sn2));
+    scal[0] = lz;
+    scal[1] = (sdv * 2.0 * sn2 / alpha + (double)B * D * (1.0 - alpha)) * scale;
+  }
+}
+
+// SGPSSM.compute_transition_tilted, 2-D branch (aep_models.py:1317-1343).
+// Outputs are already scaled; dmall = dmprop - 2 dvprop mprop for the MM adjoint.
+__global__ __launch_bounds__(1024) void transition_kernel(int B, int D, const double *__restrict__ mp,
+                                                          const double *__restrict__ vp,
+                                                          const double *__restrict__ mt,
+                                                          const double *__restrict__ vt,
+                                                          const double *__restrict__ sn, double alpha,
+                                                          double scale, double *__restrict__ dmall,
+                                                          double *__restrict__ dvp,
+                                                          double *__restrict__ dmt,
+                                                          double *__restrict__ dvt,
+                                                          double *__restrict__ scal) {
+  __shared__ double red[16];
+  const double sn2 = exp(2.0 * sn[0]);
+  const double two_pi = 6.283185307179586;
+  double lz = 0.0, sdv = 0.0;
+  const long n = (long)B * D;
+  for (long e = threadIdx.x; e < n; e += blockDim.x) {
+    const double vs = vt[e] + vp[e] + sn2 / alpha;
+    const double md = mt[e] - mp[e];
+    lz += -0.5 * md * md / vs - 0.5 * log(1.0 + alpha * (vt[e] + vp[e]) / sn2) -
+          0.5 * alpha * log(two_pi * sn2);
+    const double gv = scale * (-0.5 / vs + 0.5 * md * md / (vs * vs));
+    const double gmt = scale * (-md / vs);
+    sdv += gv;
+    dvt[e] = gv;
+    dvp[e] = gv;
+    dmt[e] = gmt;
+    dmall[e] = -gmt - 2.0 * gv * mp[e];
+  }
+  lz = gpx_block_sum(lz, red);
+  sdv = gpx_block_sum(sdv, red);
+  if (threadIdx.x == 0) {
+    scal[0] = scale * lz;
+    scal[1] = sdv * 2.0 * sn2 / alpha + scale * (double)B * D * (1.0 - alpha);
+  }
+}
+
+// Gauss_Emis.compute_emission_tilted (lik_layers.py:573-627) for Dout <= 16, one thread
+// per time step, Vy factorised by Cholesky in a padded LDS slice.  part[blk] collects
+// [logZ, dC (Dout x Din), dR (Dout)] partial sums of the block (unscaled).
+constexpr int EM_T = 64;
+constexpr int EM_DMAX = 8;
+__global__ __launch_bounds__(EM_T) void emission_kernel(int Nb, int Dout, int Din, const double *__restrict__ C,
+                                                        const double *__restrict__ Rp,
+                                                        const double *__restrict__ y, int ldy,
+                                                        const double *__restrict__ mx, int ldm,
+                                                        const double *__restrict__ vx, int ldv,
+                                                        double alpha, double scale,
+                                                        double *__restrict__ dmx, double *__restrict__ dvx,
+                                                        double *__restrict__ part) {
+  __shared__ double Ls[EM_T][EM_DMAX * EM_DMAX + 1];
+  __shared__ double Cs[EM_DMAX * EM_DMAX];
+  __shared__ double Rs[EM_DMAX];
+  __shared__ double acc[EM_T][EM_DMAX * EM_DMAX + EM_DMAX + 2];
+  const int tid = threadIdx.x, t = blockIdx.x * EM_T + tid;
+  for (int e = tid; e < Dout * Din; e += EM_T) Cs[e] = C[e];
+  for (int e = tid; e < Dout; e += EM_T) Rs[e] = exp(2.0 * Rp[e]);
+  __syncthreads();
+  const int W = Dout * Din + Dout + 1;
+  double *my = acc[tid];
+  for (int i = 0; i < W; ++i) my[i] = 0.0;
+  if (t < Nb) {
+    double *L = Ls[tid];
+    const double *vt = vx + (long)t * ldv, *mt = mx + (long)t * ldm;
+    // Vy = diag(R/alpha) + C diag(vx) C^T
+    for (int a = 0; a < Dout; ++a)
+      for (int b = 0; b <= a; ++b) {
+        double s = 0.0;
+        for (int c = 0; c < Din; ++c) s += Cs[a * Din + c] * vt[c] * Cs[b * Din + c];
+        if (a == b) s += Rs[a] / alpha;
+        L[a * Dout + b] = s;
+      }
+    // Cholesky (lower) in place
+    double logdetV = 0.0;
+    for (int k = 0; k < Dout; ++k) {
+      double s = L[k * Dout + k];
+      for (int j = 0; j < k; ++j) s -= L[k * Dout + j] * L[k * Dout + j];
+      const double lkk = sqrt(s);
+      L[k * Dout + k] = lkk;
+      logdetV += 2.0 * log(lkk);
+      for (int i = k + 1; i < Dout; ++i) {
+        double u = L[i * Dout + k];
+        for (int j = 0; j < k; ++j) u -= L[i * Dout + j] * L[k * Dout + j];
+        L[i * Dout + k] = u / lkk;
+      }
+    }
+    // r = y - C mx;  u = Vy^{-1} r
+    double r[EM_DMAX], u[EM_DMAX];
+    for (int a = 0; a < Dout; ++a) {
+      double s = y[(long)t * ldy + a];
+      for (int c = 0; c < Din; ++c) s -= Cs[a * Din + c] * mt[c];
+      r[a] = s;
+    }
+    for (int a = 0; a < Dout; ++a) {  // forward
+      double s = r[a];
+      for (int j = 0; j < a; ++j) s -= L[a * Dout + j] * u[j];
+      u[a] = s / L[a * Dout + a];
+    }
+    for (int a = Dout - 1; a >= 0; --a) {  // backward
+      double s = u[a];
+      for (int j = a + 1; j < Dout; ++j) s -= L[j * Dout + a] * u[j];
+      u[a] = s / L[a * Dout + a];
+    }
+    // Linv (lower) into the strictly-upper part is awkward; invert L into a local array.
+    double Li[EM_DMAX * EM_DMAX];
+    for (int j = 0; j < Dout; ++j)
+      for (int i = 0; i < Dout; ++i) {
+        if (i < j) { Li[i * Dout + j] = 0.0; continue; }
+        double s = (i == j) ? 1.0 : 0.0;
+        for (int k = j; k < i; ++k) s -= L[i * Dout + k] * Li[k * Dout + j];
+        Li[i * Dout + j] = s / L[i * Dout + i];
+      }
+    // Vinv = Li^T Li, overwrite L storage with it
+    for (int a = 0; a < Dout; ++a)
+      for (int b = 0; b < Dout; ++b) {
+        double s = 0.0;
+        for (int k = (a > b ? a : b); k < Dout; ++k) s += Li[k * Dout + a] * Li[k * Dout + b];
+        L[a * Dout + b] = s;
+      }
+    double quad = 0.0, sumlogR = 0.0;
+    for (int a = 0; a < Dout; ++a) {
+      quad += r[a] * u[a];
+      sumlogR += log(Rs[a]);
+    }
+    // logdet(I + alpha C V C^T / R) = logdet(Vy) + Dout log(alpha) - sum log R
+    const double vlog = logdetV + Dout * log(alpha) - sumlogR;
+    my[0] = -0.5 * quad - 0.5 * vlog - 0.5 * Dout * alpha * log(6.283185307179586) -
+            0.5 * alpha * sumlogR;
+    // dSigma = -Vinv/2 + u u^T / 2
+    for (int a = 0; a < Dout; ++a) {
+      my[1 + Dout * Din + a] = (-0.5 * L[a * Dout + a] + 0.5 * u[a] * u[a]);
+      for (int c = 0; c < Din; ++c) {
+        double s = 0.0;
+        for (int b = 0; b < Dout; ++b)
+          s += (-0.5 * L[a * Dout + b] + 0.5 * u[a] * u[b]) * Cs[b * Din + c];
+        my[1 + a * Din + c] = u[a] * mt[c] + 2.0 * s * vt[c];
+      }
+    }
+    for (int c = 0; c < Din; ++c) {
+      double gm = 0.0, gv = 0.0;
+      for (int a = 0; a < Dout; ++a) {
+        gm += u[a] * Cs[a * Din + c];
+        for (int b = 0; b < Dout; ++b)
+          gv += (-0.5 * L[a * Dout + b] + 0.5 * u[a] * u[b]) * Cs[a * Din + c] * Cs[b * Din + c];
+      }
+      dmx[(long)t * Din + c] = gm * scale;
+      dvx[(long)t * Din + c] = gv * scale;
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < W; i += EM_T) {
+    double s = 0.0;
+    for (int k = 0; k < EM_T; ++k) s += acc[k][i];
+    part[(long)blockIdx.x * W + i] = s;
+  }
+}
+
+// logZ, dC, dR from the block partials (the dR chain of lik_layers.py:610-613).
+__global__ void emission_final(int nblk, int Nb, int Dout, int Din, const double *__restrict__ part,
+                               const double *__restrict__ Rp, double alpha, double scale,
+                               double *__restrict__ dC, double *__restrict__ dR, double *__restrict__ scal) {
+  const int W = Dout * Din + Dout + 1;
+  for (int i = threadIdx.x; i < W; i += blockDim.x) {
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += part[(long)b * W + i];
+    if (i == 0) {
+      scal[0] = s * scale;
+    } else if (i <= Dout * Din) {
+      dC[i - 1] = s * scale;
+    } else {
+      const int a = i - 1 - Dout * Din;
+      const double R = exp(2.0 * Rp[a]);
+      const double g = s / alpha + 0.5 * Nb * (1.0 - alpha) / R;
+      dR[a] = g * 2.0 * R * scale;
+    }
+  }
+}
+
+
+// out[d][n][m] = w[n, d] * X[n][m]   (the dv-weighted rows of aep_models.py:464, 490)
+__global__ void rowscale_kernel(int N, int D, int M, const double *__restrict__ w,
+                                const double *__restrict__ X, double *__restrict__ out) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long NM = (long)N * M;
+  if (e >= D * NM) return;
+  const int d = (int)(e / NM);
+  const long r = e % NM;
+  const int n = (int)(r / M);
+  out[e] = w[(long)n * D + d] * X[r];
+}
+
+// out[n][m] = alpha sum_d w[n, d] T[d][n][m] + beta out[n][m]   (aep_models.py:455)
+__global__ void wsum_kernel(int N, int D, int M, double alpha, const double *__restrict__ w,
+                            const double *__restrict__ T, double beta, double *__restrict__ out) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long NM = (long)N * M;
+  if (e >= NM) return;
+  const int n = (int)(e / M);
+  double s = 0.0;
+  for (int d = 0; d < D; ++d) s += w[(long)n * D + d] * T[(long)d * NM + e];
+  out[e] = (beta == 0.0) ? alpha * s : alpha * s + beta * out[e];
+}
+
+// out = dm - 2 dv m   (aep_models.py:235)
+__global__ void dmall_kernel(long n, const double *__restrict__ dm, const double *__restrict__ dv,
+                             const double *__restrict__ m, double *__restrict__ out) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e < n) out[e] = dm[e] - 2.0 * dv[e] * m[e];
+}
+
+// dls = dell * exp(ls) + 2 gls;  dsf = 2 exp(2 sf) (dsf2 + sumdv) + 2 gsf
+// (aep_models.py:244-247, 296-298, 456-460, 502-504)
+__global__ void hyper_kernel(int Q, const double *__restrict__ ls, const double *__restrict__ sf,
+                             const double *__restrict__ dell, const double *__restrict__ gls,
+                             const double *__restrict__ dsf2, const double *__restrict__ sumdv,
+                             const double *__restrict__ gsf, double *__restrict__ dls,
+                             double *__restrict__ dsf) {
+  const int q = threadIdx.x;
+  if (q < Q) dls[q] = dell[q] * exp(ls[q]) + 2.0 * gls[q];
+  if (q == 0) dsf[0] = 2.0 * exp(2.0 * sf[0]) * (dsf2[0] + sumdv[0]) + 2.0 * gsf[0];
+}
+
+// phi = (D/2) ldK + sp/2 sum_d (sp_sign ldP_d + dotP_d) + sc/2 sum_d (sc_sign ldC_d + dotC_d)
+// (aep_models.py:62-114)
+__global__ void phi_kernel(int D, const double *__restrict__ ldK, const double *__restrict__ ldP,
+                           double sgnP, const double *__restrict__ dotP, const double *__restrict__ ldC,
+                           double sgnC, const double *__restrict__ dotC, double sp, double sc,
+                           double *__restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double p = 0.0, c = 0.0;
+  for (int d = 0; d < D; ++d) {
+    p += sgnP * ldP[d] + dotP[d];
+    c += sgnC * ldC[d] + dotC[d];
+  }
+  out[0] = D * 0.5 * ldK[0] + sp * 0.5 * p + sc * 0.5 * c;
+}
+
+// my = C mf, vy_d = sum_c C_dc^2 vf_c + R_d (diagonal of lik_layers.py:558-571)
+__global__ void emission_predict_kernel(int Nb, int Dout, int Din, const double *__restrict__ C,
+                                        const double *__restrict__ Rp, const double *__restrict__ mf,
+                                        const double *__restrict__ vf, double *__restrict__ my,
+                                        double *__restrict__ vy) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)Nb * Dout) return;
+  const int n = (int)(e / Dout), d = (int)(e % Dout);
+  double m = 0.0, v = exp(2.0 * Rp[d]);
+  for (int c = 0; c < Din; ++c) {
+    const double cc = C[d * Din + c];
+    m += cc * mf[(long)n * Din + c];
+    v += cc * cc * vf[(long)n * Din + c];
+  }
+  my[e] = m;
+  vy[e] = v;
+}
+
+}  // namespace
+
+extern "C" {
+
+// K[n,m] = SE(x_n, z_m) (+ jitter on n == m); x is N rows with leading dim ldx.
+int gpx_se_kernel(int N, int M, int Q, const double *x, int ldx, const double *z, const double *ls,
+                  const double *sf, double jitter, double *K, void *stream) {
+  if (N < 0 || M < 0 || Q < 1 || ldx < Q) return GPX_EINVAL;
+  if (N == 0 || M == 0) return GPX_OK;
+  if (!x || !z || !ls || !sf || !K) return GPX_EINVAL;
+  hipLaunchKernelGGL(se_kernel, dim3(gpx_cdiv((long)N * M, 256)), dim3(256), 0, (hipStream_t)stream, N,
+                     M, Q, x, ldx, z, ls, sf, jitter, K);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+long gpx_kfu_bwd_workspace(int N, int M, int Q) {
+  return (long)gpx_cdiv(N, KFU_NB) * M * (2 * Q + 1);
+}
+
+// Adjoint of K(x, Z): dsf2 (w.r.t. sf2), dell (w.r.t. ell), dZ (M x Q), dx (N x Q, optional).
+int gpx_kfu_bwd(int N, int M, int Q, const double *dK, const double *K, const double *x,
+                const double *z, const double *ls, const double *sf, double *dsf2, double *dell,
+                double *dZ, double *dx, double *work, void *stream) {
+  if (N < 1 || M < 1 || Q < 1 || Q > 16) return GPX_EINVAL;
+  if (!dK || !K || !x || !z || !ls || !sf || !dsf2 || !dell || !dZ || !work) return GPX_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned nblk = gpx_cdiv(N, KFU_NB);
+  hipLaunchKernelGGL(kfu_bwd_partial, dim3(nblk), dim3(256), 0, st, N, M, Q, dK, K, x, z, ls, work, dx);
+  GPX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(kfu_bwd_final, dim3(1), dim3(256), 0, st, (int)nblk, M, Q, work, z, ls, sf, dsf2,
+                     dell, dZ);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+long gpx_dtrace_workspace(int M, int Q) { return (long)M * (Q + 1); }
+
+// d tr(Mm Kzz) / d(lsf, lls, z) for Kzz = SE(z, z): gsf (1), gls (Q), gz (M x Q).
+int gpx_dtrace_kzz(int M, int Q, const double *Mm, const double *z, const double *ls,
+                   const double *sf, double *gsf, double *gls, double *gz, double *work,
+                   void *stream) {
+  if (M < 1 || Q < 1 || Q > 16 || !Mm || !z || !ls || !sf || !gsf || !gls || !gz || !work)
+    return GPX_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(dtrace_kernel, dim3(M), dim3(256), 0, st, M, Q, Mm, z, ls, sf, gz, work);
+  GPX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(dtrace_final, dim3(1), dim3(64), 0, st, M, Q, (const double *)work, gsf, gls);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+// out[j] = alpha * sum_i P[i, j] + beta * out[j]
+int gpx_colsum(int rows, int cols, const double *P, double alpha, double beta, double *out, void *stream) {
+  if (rows < 0 || cols < 0 || !P || !out) return GPX_EINVAL;
+  if (cols == 0) return GPX_OK;
+  hipLaunchKernelGGL(colsum_kernel, dim3(gpx_cdiv(cols, 256)), dim3(256), 0, (hipStream_t)stream, rows, cols,
+                     P, alpha, beta, out);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+int gpx_eta_to_R(int D, int M, const double *eta, double *R, void *stream) {
+  if (D < 0 || M < 1 || !eta || !R) return GPX_EINVAL;
+  long n = (long)D * M * M;
+  if (n == 0) return GPX_OK;
+  hipLaunchKernelGGL(eta_to_R_kernel, dim3(gpx_cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream, D, M, eta, R);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+int gpx_pack_dR(int D, int M, const double *R, const double *G, double *out, void *stream) {
+  if (D < 0 || M < 1 || !R || !G || !out) return GPX_EINVAL;
+  long n = (long)D * M * (M + 1) / 2;
+  if (n == 0) return GPX_OK;
+  hipLaunchKernelGGL(pack_dR_kernel, dim3(gpx_cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream, D, M, R, G, out);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+// out[i] = a x[i % nx] + b y[i % ny] + c w[i % nw]  (y, w optional); out may alias inputs.
+int gpx_lincomb(long n, double a, const double *x, long nx, double b, const double *y, long ny,
+                double c, const double *w, long nw, double *out, void *stream) {
+  if (n < 0 || !x || nx < 1 || !out || (y && ny < 1) || (w && nw < 1)) return GPX_EINVAL;
+  if (n == 0) return GPX_OK;
+  unsigned g = gpx_cdiv(n, 256);
+  if (g > 8192) g = 8192;
+  hipLaunchKernelGGL(lincomb_kernel, dim3(g), dim3(256), 0, (hipStream_t)stream, n, a, x, nx, b, y, ny, c, w,
+                     nw, out);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+// out[b] = a (X[b] + X[b]^T); out must not alias X.
+int gpx_sym(int batch, int M, double a, const double *X, double *out, void *stream) {
+  if (batch < 0 || M < 0 || !X || !out || X == out) return GPX_EINVAL;
+  long n = (long)batch * M * M;
+  if (n == 0) return GPX_OK;
+  hipLaunchKernelGGL(sym_kernel, dim3(gpx_cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream, batch, M, a, X, out);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+int gpx_det_var(int N, int D, int M, const double *T, const double *kfu, const double *sf, double *v,
+                void *stream) {
+  if (N < 0 || D < 0 || M < 1 || !T || !kfu || !sf || !v) return GPX_EINVAL;
+  long n = (long)N * D;
+  if (n == 0) return GPX_OK;
+  hipLaunchKernelGGL(det_var_kernel, dim3(gpx_cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream, N, D, M, T, kfu,
+                     sf, v);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+int gpx_mm_var(long n, const double *m, const double *Bp, const double *sf, double *v, void *stream) {
+  if (n < 0 || !m || !Bp || !sf || !v) return GPX_EINVAL;
+  if (n == 0) return GPX_OK;
+  hipLaunchKernelGGL(mm_var_kernel, dim3(gpx_cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream, n, m, Bp, sf, v);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+int gpx_batched_dot(int batch, int n, const double *x, const double *y, double *out, void *stream) {
+  if (batch < 0 || n < 0 || !x || !y || !out) return GPX_EINVAL;
+  if (batch == 0) return GPX_OK;
+  hipLaunchKernelGGL(batched_dot_kernel, dim3(batch), dim3(256), 0, (hipStream_t)stream, batch, n, x, y, out);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+int gpx_gauss_tilted(int B, int D, const double *m, const double *v, const double *y, const double *sn,
+                     double alpha, double scale, double *dm, double *dv, double *dmall, double *scal,
+                     void *stream) {
+  if (B < 1 || D < 1 || !(alpha > 0) || !m || !v || !y || !sn || !dm || !dv || !scal) return GPX_EINVAL;
+  hipLaunchKernelGGL(gauss_tilted_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, B, D, m, v, y, sn, alpha,
+                     scale, dm, dv, dmall, scal);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+int gpx_transition_tilted(int B, int D, const double *mp, const double *vp, const double *mt,
+                          const double *vt, const double *sn, double alpha, double scale,
+                          double *dmall, double *dvp, double *dmt, double *dvt, double *scal,
+                          void *stream) {
+  if (B < 1 || D < 1 || !(alpha > 0) || !mp || !vp || !mt || !vt || !sn || !dmall || !dvp || !dmt ||
+      !dvt || !scal)
+    return GPX_EINVAL;
+  hipLaunchKernelGGL(transition_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, B, D, mp, vp, mt, vt, sn,
+                     alpha, scale, dmall, dvp, dmt, dvt, scal);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+long gpx_emission_workspace(int Nb, int Dout, int Din) {
+  return (long)gpx_cdiv(Nb, EM_T) * (Dout * Din + Dout + 1);
+}
+
+int gpx_emission_tilted(int Nb, int Dout, int Din, const double *C, const double *Rp, const double *y,
+                        int ldy, const double *mx, int ldm, const double *vx, int ldv, double alpha,
+                        double scale, double *dmx, double *dvx, double *dC, double *dR, double *scal,
+                        double *work, void *stream) {
+  if (Nb < 1 || Dout < 1 || Dout > EM_DMAX || Din < 1 || !(alpha > 0)) return GPX_EINVAL;
+  if (!C || !Rp || !y || !mx || !vx || !dmx || !dvx || !dC || !dR || !scal || !work) return GPX_EINVAL;
+  if (Dout * Din > EM_DMAX * EM_DMAX || Dout * Din + Dout + 2 > EM_DMAX * EM_DMAX + EM_DMAX + 2)
+    return GPX_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned nblk = gpx_cdiv(Nb, EM_T);
+  hipLaunchKernelGGL(emission_kernel, dim3(nblk), dim3(EM_T), 0, st, Nb, Dout, Din, C, Rp, y, ldy, mx, ldm, vx,
+                     ldv, alpha, scale, dmx, dvx, work);
+  GPX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(emission_final, dim3(1), dim3(256), 0, st, (int)nblk, Nb, Dout, Din, (const double *)work,
+                     Rp, alpha, scale, dC, dR, scal);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+int gpx_rowscale(int N, int D, int M, const double *w, const double *X, double *out, void *stream) {
+  if (N < 0 || D < 0 || M < 0 || !w || !X || !out) return GPX_EINVAL;
+  long n = (long)N * D * M;
+  if (n == 0) return GPX_OK;
+  hipLaunchKernelGGL(rowscale_kernel, dim3(gpx_cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream, N, D, M, w, X, out);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+int gpx_wsum(int N, int D, int M, double alpha, const double *w, const double *T, double beta,
+             double *out, void *stream) {
+  if (N < 0 || D < 0 || M < 0 || !w || !T || !out) return GPX_EINVAL;
+  long n = (long)N * M;
+  if (n == 0) return GPX_OK;
+  hipLaunchKernelGGL(wsum_kernel, dim3(gpx_cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream, N, D, M, alpha, w, T,
+                     beta, out);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+int gpx_dmall(long n, const double *dm, const double *dv, const double *m, double *out, void *stream) {
+  if (n < 0 || !dm || !dv || !m || !out) return GPX_EINVAL;
+  if (n == 0) return GPX_OK;
+  hipLaunchKernelGGL(dmall_kernel, dim3(gpx_cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream, n, dm, dv, m, out);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+int gpx_hyper_grads(int Q, const double *ls, const double *sf, const double *dell, const double *gls,
+                    const double *dsf2, const double *sumdv, const double *gsf, double *dls, double *dsf,
+                    void *stream) {
+  if (Q < 1 || Q > 1024 || !ls || !sf || !dell || !gls || !dsf2 || !sumdv || !gsf || !dls || !dsf)
+    return GPX_EINVAL;
+  hipLaunchKernelGGL(hyper_kernel, dim3(1), dim3(Q < 64 ? 64 : ((Q + 63) / 64) * 64), 0, (hipStream_t)stream, Q,
+                     ls, sf, dell, gls, dsf2, sumdv, gsf, dls, dsf);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+int gpx_phi(int D, const double *ldK, const double *ldP, double sgnP, const double *dotP,
+            const double *ldC, double sgnC, const double *dotC, double sp, double sc, double *out,
+            void *stream) {
+  if (D < 1 || !ldK || !ldP || !dotP || !ldC || !dotC || !out) return GPX_EINVAL;
+  hipLaunchKernelGGL(phi_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, D, ldK, ldP, sgnP, dotP, ldC, sgnC,
+                     dotC, sp, sc, out);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+int gpx_emission_predict(int Nb, int Dout, int Din, const double *C, const double *Rp, const double *mf,
+                         const double *vf, double *my, double *vy, void *stream) {
+  if (Nb < 0 || Dout < 1 || Din < 1 || !C || !Rp || !mf || !vf || !my || !vy) return GPX_EINVAL;
+  if (Nb == 0) return GPX_OK;
+  hipLaunchKernelGGL(emission_predict_kernel, dim3(gpx_cdiv((long)Nb * Dout, 256)), dim3(256), 0,
+                     (hipStream_t)stream, Nb, Dout, Din, C, Rp, mf, vf, my, vy);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+}  // extern "C"

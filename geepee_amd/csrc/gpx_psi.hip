@@ -1,0 +1,381 @@
+// psi statistics of the ARD SE kernel under Gaussian inputs and their adjoint.
+//
+// Forward: compute_psi_weave (kernels.py:181-240, loop :201-234)
+//   psi1[n,m]   = sf2 prod_q (l/(l+v))^1/2 exp(-(mu-z_m)^2 / (2(l+v)))
+//   psi2[n,a,b] = sf2^2 prod_q (l/(l+2v))^1/2
+//                 exp(-(z_a-z_b)^2/(4l) - (mu-(z_a+z_b)/2)^2/(2v+l))
+// with l = ell^2 = exp(2 ls) and sf2 = exp(2 sf).  psi2 is written exactly
+// symmetric: every value is computed once and stored at (a,b) and (b,a).
+//
+// Adjoint: compute_psi_derivatives -> psi1compDer / psi2compDer (kernels.py:302-309,
+// 355-444), restated with per-point sufficient statistics.  With
+// L[n,a,b] = dpsi2[n,a,b] psi2[n,a,b] (dpsi2 symmetric), T_na = sum_b L_nab,
+// V_nq = sum_ab L_nab z_aq z_bq, S0 = sum_a T_a, S1_q = sum_a T_a z_aq,
+// S2_q = (sum_a T_a z_aq^2 + V_q)/2 and d = 1/(2v+l):
+//   dmu = -2 d (mu S0 - S1),  dv = 2 d^2 (mu^2 S0 - 2 mu S1 + S2) - d S0,
+//   dell (per n) = 2 ell [(v/l) d S0 + d^2 (mu^2 S0 - 2 mu S1 + S2)
+//                         + (sum_a T_a z_aq^2 - V_q) / (2 l^2)],
+//   dZ_aq = -(z_aq R_a - (Lsum z)_aq)/l + 2 P_aq - z_aq R'_aq - sum_b W_abq z_bq,
+// where Lsum = sum_n L_n, R = rowsum(Lsum), W_abq = sum_n L_nab d_nq,
+// P_aq = sum_n T_na d_nq mu_nq, R'_aq = sum_n T_na d_nq (the n-reductions run as
+// MFMA GEMMs on the host side of this file's entry points).
+#include "gpx_common.h"
+
+#define GPX_QMAX 16
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// forward
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void psi1_kernel(int N, int M, int Q, const double *__restrict__ mx,
+                                                    const double *__restrict__ vx,
+                                                    const double *__restrict__ z,
+                                                    const double *__restrict__ ls,
+                                                    const double *__restrict__ sf,
+                                                    double *__restrict__ psi1) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)N * M) return;
+  const int n = (int)(e / M), m = (int)(e % M);
+  const double sf2 = exp(2.0 * sf[0]);
+  double acc = 0.0, lc = 0.0;
+  for (int q = 0; q < Q; ++q) {
+    const double l = exp(2.0 * ls[q]);
+    const double v = vx[(long)n * Q + q];
+    const double d = mx[(long)n * Q + q] - z[m * Q + q];
+    acc += -d * d / (2.0 * (v + l));
+    lc += 0.5 * log(l / (l + v));
+  }
+  psi1[e] = sf2 * exp(acc + lc);
+}
+
+// One block per (point n, tile pair ta >= tb) of 32 x 32 entries.
+__global__ __launch_bounds__(256) void psi2_kernel(int N, int M, int Q, const double *__restrict__ mx,
+                                                    const double *__restrict__ vx,
+                                                    const double *__restrict__ z,
+                                                    const double *__restrict__ ls,
+                                                    const double *__restrict__ sf,
+                                                    double *__restrict__ psi2) {
+  __shared__ double za[32][GPX_QMAX], zb[32][GPX_QMAX];
+  __shared__ double mu[GPX_QMAX], dd[GPX_QMAX], i4l[GPX_QMAX];
+  __shared__ double tile[32][33];
+  __shared__ double lc_s;
+  const int n = blockIdx.x;
+  // decode the tile pair index p -> (ta, tb), ta >= tb
+  int p = blockIdx.y, ta = 0;
+  while (p > ta) { p -= ta + 1; ++ta; }
+  const int tb = p;
+  const int a0 = ta * 32, b0 = tb * 32, tid = threadIdx.x;
+  for (int e = tid; e < 32 * Q; e += 256) {
+    const int i = e / Q, q = e % Q;
+    za[i][q] = (a0 + i < M) ? z[(a0 + i) * Q + q] : 0.0;
+    zb[i][q] = (b0 + i < M) ? z[(b0 + i) * Q + q] : 0.0;
+  }
+  if (tid < Q) {
+    const double l = exp(2.0 * ls[tid]);
+    const double v = vx[(long)n * Q + tid];
+    mu[tid] = mx[(long)n * Q + tid];
+    dd[tid] = 1.0 / (2.0 * v + l);
+    i4l[tid] = 1.0 / (4.0 * l);
+  }
+  if (tid == 0) {
+    double lc = 0.0;
+    for (int q = 0; q < Q; ++q) {
+      const double l = exp(2.0 * ls[q]);
+      lc += 0.5 * log(l / (l + 2.0 * vx[(long)n * Q + q]));
+    }
+    lc_s = lc;
+  }
+  __syncthreads();
+  const int j = tid & 31;
+  const double sf2 = exp(2.0 * sf[0]), sf4 = sf2 * sf2;
+  double *out = psi2 + (long)n * M * M;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int i = (tid >> 5) + 8 * s;
+    double e = lc_s;
+    for (int q = 0; q < Q; ++q) {
+      const double x = za[i][q], y = zb[j][q];
+      const double dz = x - y, h = mu[q] - 0.5 * (x + y);
+      e -= dz * dz * i4l[q] + h * h * dd[q];
+    }
+    const double val = sf4 * exp(e);
+    tile[i][j] = val;
+    if (a0 + i < M && b0 + j < M) out[(long)(a0 + i) * M + b0 + j] = val;
+  }
+  if (ta == tb) return;
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int i = (tid >> 5) + 8 * s;  // row within the mirrored tile (b index)
+    if (b0 + i < M && a0 + j < M) out[(long)(b0 + i) * M + a0 + j] = tile[j][i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// adjoint, per-point pass (one block per n, thread b owns column b of L_n)
+// ---------------------------------------------------------------------------
+
+constexpr int NRED = 5 * GPX_QMAX + 2;
+
+__global__ __launch_bounds__(256) void psi_bwd_rows_kernel(
+    int N, int M, int Q, const double *__restrict__ mx, const double *__restrict__ vx,
+    const double *__restrict__ z, const double *__restrict__ ls,
+    const double *__restrict__ dpsi1, const double *__restrict__ psi1,
+    const double *__restrict__ dpsi2, const double *__restrict__ psi2,
+    double *__restrict__ dmx, double *__restrict__ dvx,
+    double *__restrict__ Lout,    // N x M x M
+    double *__restrict__ Tout,    // N x M
+    double *__restrict__ Lpout,   // N x M
+    double *__restrict__ Hout,    // N x Q   per-point d/d ell contribution
+    double *__restrict__ s0out,   // N x 2   (S0, sum Lp)
+    double *__restrict__ Dn,      // N x (Q+1): [1 | d]
+    double *__restrict__ Pn,      // N x 2Q : [d*mu | d]
+    double *__restrict__ En) {    // N x 2Q : [e | e*mu]
+  __shared__ double zs[256][GPX_QMAX];
+  __shared__ double red[4][NRED];
+  __shared__ double tot[NRED];
+  const int n = blockIdx.x, b = threadIdx.x, lane = b & 63, w = b >> 6;
+  for (int e = b; e < M * Q; e += 256) zs[e / Q][e % Q] = z[e];
+  __syncthreads();
+  double mu[GPX_QMAX];
+#pragma unroll
+  for (int q = 0; q < GPX_QMAX; ++q) mu[q] = (q < Q) ? mx[(long)n * Q + q] : 0.0;
+  double T = 0.0, U[GPX_QMAX];
+#pragma unroll
+  for (int q = 0; q < GPX_QMAX; ++q) U[q] = 0.0;
+  double Lp = 0.0;
+  if (b < M) {
+    const double *P2 = psi2 + (long)n * M * M + b;
+    const double *G2 = dpsi2 + (long)n * M * M + b;
+    double *Lo = Lout ? Lout + (long)n * M * M + b : nullptr;
+    for (int a = 0; a < M; ++a) {
+      const double l = G2[(long)a * M] * P2[(long)a * M];
+      if (Lo) Lo[(long)a * M] = l;
+      T += l;
+#pragma unroll
+      for (int q = 0; q < GPX_QMAX; ++q)
+        if (q < Q) U[q] = fma(l, zs[a][q], U[q]);
+    }
+    Lp = dpsi1[(long)n * M + b] * psi1[(long)n * M + b];
+    Tout[(long)n * M + b] = T;
+    Lpout[(long)n * M + b] = Lp;
+  }
+  // per-thread contributions, reduced over b:
+  // [0] S0, [1] sum Lp, [2+q] S1, [2+Q+q] sum T z^2, [2+2Q+q] V,
+  // [2+3Q+q] B1 = sum Lp (z-mu), [2+4Q+q] C1 = sum Lp (z-mu)^2
+  double c[NRED];
+#pragma unroll
+  for (int i = 0; i < NRED; ++i) c[i] = 0.0;
+  c[0] = T;
+  c[1] = Lp;
+  if (b < M) {
+#pragma unroll
+    for (int q = 0; q < GPX_QMAX; ++q)
+      if (q < Q) {
+        const double zq = zs[b][q], zm = zq - mu[q];
+        c[2 + q] = T * zq;
+        c[2 + GPX_QMAX + q] = T * zq * zq;
+        c[2 + 2 * GPX_QMAX + q] = U[q] * zq;
+        c[2 + 3 * GPX_QMAX + q] = Lp * zm;
+        c[2 + 4 * GPX_QMAX + q] = Lp * zm * zm;
+      }
+  }
+#pragma unroll
+  for (int i = 0; i < NRED; ++i) {
+    if (i >= 2 && (i - 2) % GPX_QMAX >= Q) continue;  // unused q slots
+    const double s = gpx_wave_sum(c[i]);
+    if (lane == 0) red[w][i] = s;
+  }
+  __syncthreads();
+  for (int i = b; i < NRED; i += 256) tot[i] = red[0][i] + red[1][i] + red[2][i] + red[3][i];
+  __syncthreads();
+  if (b < Q) {
+    const int q = b;
+    const double S0 = tot[0], A1 = tot[1];
+    const double S1 = tot[2 + q], TZ2 = tot[2 + GPX_QMAX + q], V = tot[2 + 2 * GPX_QMAX + q];
+    const double B1 = tot[2 + 3 * GPX_QMAX + q], C1 = tot[2 + 4 * GPX_QMAX + q];
+    const double l = exp(2.0 * ls[q]);
+    const double m = mx[(long)n * Q + q], vv = vx[(long)n * Q + q];
+    const double ell = exp(ls[q]);
+    const double d = 1.0 / (2.0 * vv + l), e = 1.0 / (vv + l);
+    const double S2 = 0.5 * (TZ2 + V);
+    const double quad = m * m * S0 - 2.0 * m * S1 + S2;
+    const double dmu = -2.0 * d * (m * S0 - S1) + e * B1;
+    const double dvv = 2.0 * d * d * quad - d * S0 + 0.5 * e * (e * C1 - A1);
+    const double h2 = 2.0 * ell * ((vv / l) * d * S0 + d * d * quad + (TZ2 - V) / (2.0 * l * l));
+    const double h1 = e * ell * (e * C1 + (vv / l) * A1);
+    dmx[(long)n * Q + q] = dmu;
+    dvx[(long)n * Q + q] = dvv;
+    Hout[(long)n * Q + q] = h1 + h2;
+    Dn[(long)n * (Q + 1) + 1 + q] = d;
+    Pn[(long)n * 2 * Q + q] = d * m;
+    Pn[(long)n * 2 * Q + Q + q] = d;
+    En[(long)n * 2 * Q + q] = e;
+    En[(long)n * 2 * Q + Q + q] = e * m;
+  }
+  if (b == 0) {
+    Dn[(long)n * (Q + 1)] = 1.0;
+    s0out[2 * n] = tot[0];
+    s0out[2 * n + 1] = tot[1];
+  }
+}
+
+// Final assembly for the hyper-parameter adjoints.  One block per inducing point a
+// (threads over b), plus block M which finishes the scalar / per-q sums.
+//   LW:  M^2 x (Q+1)  [Lsum | W]          (sum_n L_nab [1 | d_nq])
+//   TP:  M x 2Q       [P | R']            (sum_n T_na [d mu | d])
+//   LE:  M x 2Q       [sum Lp e | sum Lp e mu]
+//   colH: Q (sum_n H), cols0: 2 (sum_n S0, sum_n Lp)
+__global__ __launch_bounds__(256) void psi_bwd_final_kernel(
+    int M, int Q, const double *__restrict__ z, const double *__restrict__ ls,
+    const double *__restrict__ sf, const double *__restrict__ LW,
+    const double *__restrict__ TP, const double *__restrict__ LE,
+    const double *__restrict__ colH, const double *__restrict__ cols0,
+    double *__restrict__ dsf2, double *__restrict__ dell, double *__restrict__ dZ) {
+  const int a = blockIdx.x, b = threadIdx.x, lane = b & 63, w = b >> 6;
+  if (a == M) {
+    if (b == 0) {
+      const double sf2 = exp(2.0 * sf[0]);
+      dsf2[0] = (cols0[1] + 2.0 * cols0[0]) / sf2;
+    }
+    return;
+  }
+  // per-a sums over b: R_a, (Lsum z)_aq, sum_b W_abq z_bq
+  double c[2 * GPX_QMAX + 1];
+#pragma unroll
+  for (int i = 0; i < 2 * GPX_QMAX + 1; ++i) c[i] = 0.0;
+  for (int bb = b; bb < M; bb += 256) {
+    const double *row = LW + ((long)a * M + bb) * (Q + 1);
+    const double ls_ab = row[0];
+    c[0] += ls_ab;
+#pragma unroll
+    for (int q = 0; q < GPX_QMAX; ++q)
+      if (q < Q) {
+        const double zb = z[bb * Q + q];
+        c[1 + q] += ls_ab * zb;
+        c[1 + GPX_QMAX + q] += row[1 + q] * zb;
+      }
+  }
+  __shared__ double tot[2 * GPX_QMAX + 1];
+  __shared__ double red2[4][2 * GPX_QMAX + 1];
+#pragma unroll
+  for (int i = 0; i < 2 * GPX_QMAX + 1; ++i) {
+    const double s = gpx_wave_sum(c[i]);
+    if (lane == 0) red2[w][i] = s;
+  }
+  __syncthreads();
+  for (int i = b; i < 2 * GPX_QMAX + 1; i += 256)
+    tot[i] = red2[0][i] + red2[1][i] + red2[2][i] + red2[3][i];
+  __syncthreads();
+  if (b < Q) {
+    const int q = b;
+    const double l = exp(2.0 * ls[q]);
+    const double zaq = z[a * Q + q];
+    const double Ra = tot[0], LZ = tot[1 + q], WZ = tot[1 + GPX_QMAX + q];
+    const double P = TP[a * 2 * Q + q], Rp = TP[a * 2 * Q + Q + q];
+    const double g2 = -(zaq * Ra - LZ) / l + 2.0 * P - zaq * Rp - WZ;
+    const double g1 = -(zaq * LE[a * 2 * Q + q] - LE[a * 2 * Q + Q + q]);
+    dZ[a * Q + q] = g1 + g2;
+  }
+}
+
+// out[j] = sum_i P[i*cols + j] over `rows`, in a fixed order.
+__global__ void colsum_kernel(int rows, int cols, const double *__restrict__ P, double *__restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cols) return;
+  double s = 0.0;
+  for (int i = 0; i < rows; ++i) s += P[(long)i * cols + j];
+  out[j] = s;
+}
+
+}  // namespace
+
+extern "C" int gpx_gemm_batched(int transA, int transB, int m, int n, int k, double alpha,
+                                const double *A, int lda, long strideA, const double *B, int ldb,
+                                long strideB, double beta, double *C, int ldc, long strideC,
+                                int batch, int splits, double *work, int tile, void *stream);
+
+extern "C" {
+
+int gpx_psi_qmax(void) { return GPX_QMAX; }
+
+// psi1 (N x M) and, if psi2 != NULL, psi2 (N x M x M) for inputs x_n ~ N(mx_n, diag vx_n).
+// ls (Q) and sf (1) are the raw log hyper-parameters on the device.
+int gpx_psi_fwd(int N, int M, int Q, const double *mx, const double *vx, const double *z,
+                const double *ls, const double *sf, double *psi1, double *psi2, void *stream) {
+  if (N < 0 || M < 0 || Q < 1 || Q > GPX_QMAX) return GPX_EINVAL;
+  if (N == 0 || M == 0) return GPX_OK;
+  if (!mx || !vx || !z || !ls || !sf || !psi1) return GPX_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(psi1_kernel, dim3(gpx_cdiv((long)N * M, 256)), dim3(256), 0, st, N, M, Q, mx, vx,
+                     z, ls, sf, psi1);
+  GPX_CHECK_LAUNCH();
+  if (psi2) {
+    const int T = (M + 31) / 32;
+    hipLaunchKernelGGL(psi2_kernel, dim3(N, T * (T + 1) / 2), dim3(256), 0, st, N, M, Q, mx, vx, z,
+                       ls, sf, psi2);
+    GPX_CHECK_LAUNCH();
+  }
+  return GPX_OK;
+}
+
+// Workspace (doubles) for gpx_psi_bwd.
+long gpx_psi_bwd_workspace(int N, int M, int Q) {
+  long s = (long)N * M * M;                 // L
+  s += 2L * N * M;                          // T, Lp
+  s += (long)N * Q + 2L * N;                // H, s0
+  s += (long)N * (Q + 1) + 4L * N * Q;      // Dn, Pn, En
+  s += (long)M * M * (Q + 1) + 4L * M * Q;  // LW, TP, LE
+  s += Q + 2;                               // colH, cols0
+  return s;
+}
+
+// Adjoint of gpx_psi_fwd given dL/dpsi1 (N x M) and a symmetric dL/dpsi2 (N x M x M):
+// dsf2 (1, w.r.t. sf2), dell (Q, w.r.t. ell = exp(ls)), dZ (M x Q), dmx, dvx (N x Q).
+int gpx_psi_bwd(int N, int M, int Q, const double *mx, const double *vx, const double *z,
+                const double *ls, const double *sf, const double *dpsi1, const double *psi1,
+                const double *dpsi2, const double *psi2, double *dsf2, double *dell, double *dZ,
+                double *dmx, double *dvx, double *work, void *stream) {
+  if (N < 1 || M < 1 || M > 256 || Q < 1 || Q > GPX_QMAX) return GPX_EINVAL;
+  if (!mx || !vx || !z || !ls || !sf || !dpsi1 || !psi1 || !dpsi2 || !psi2 || !work) return GPX_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  double *L = work;
+  double *T = L + (long)N * M * M;
+  double *Lp = T + (long)N * M;
+  double *H = Lp + (long)N * M;
+  double *s0 = H + (long)N * Q;
+  double *Dn = s0 + 2L * N;
+  double *Pn = Dn + (long)N * (Q + 1);
+  double *En = Pn + 2L * N * Q;
+  double *LW = En + 2L * N * Q;
+  double *TP = LW + (long)M * M * (Q + 1);
+  double *LE = TP + 2L * M * Q;
+  double *colH = LE + 2L * M * Q;
+  double *cols0 = colH + Q;
+  hipLaunchKernelGGL(psi_bwd_rows_kernel, dim3(N), dim3(256), 0, st, N, M, Q, mx, vx, z, ls, dpsi1,
+                     psi1, dpsi2, psi2, dmx, dvx, L, T, Lp, H, s0, Dn, Pn, En);
+  GPX_CHECK_LAUNCH();
+  int rc;
+  // LW = L^T [1 | d]  (M^2 x (Q+1)), reduction over n
+  rc = gpx_gemm_batched(1, 0, M * M, Q + 1, N, 1.0, L, M * M, 0, Dn, Q + 1, 0, 0.0, LW, Q + 1, 0, 1,
+                        1, nullptr, 0, stream);
+  if (rc) return rc;
+  rc = gpx_gemm_batched(1, 0, M, 2 * Q, N, 1.0, T, M, 0, Pn, 2 * Q, 0, 0.0, TP, 2 * Q, 0, 1, 1,
+                        nullptr, 0, stream);
+  if (rc) return rc;
+  rc = gpx_gemm_batched(1, 0, M, 2 * Q, N, 1.0, Lp, M, 0, En, 2 * Q, 0, 0.0, LE, 2 * Q, 0, 1, 1,
+                        nullptr, 0, stream);
+  if (rc) return rc;
+  hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(64), 0, st, N, Q, H, dell);
+  GPX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(64), 0, st, N, 2, s0, cols0);
+  GPX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(psi_bwd_final_kernel, dim3(M + 1), dim3(256), 0, st, M, Q, z, ls, sf, LW, TP,
+                     LE, colH, cols0, dsf2, dell, dZ);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+}  // extern "C"

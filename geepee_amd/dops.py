@@ -1,0 +1,224 @@
+"""Device operations: typed wrappers that allocate outputs (torch, as plain device
+memory) and launch the libgpx kernels on the current stream.  Nothing here computes
+on the host or falls back to torch arithmetic."""
+import torch
+
+from . import _lib
+from ._lib import P, S, call, query
+
+F64 = torch.float64
+
+
+def empty(*shape, like=None, device='cuda'):
+    return torch.empty(*shape, dtype=F64, device=like.device if like is not None else device)
+
+
+def mm(A, B, C=None, tA=False, tB=False, alpha=1.0, beta=0.0, splits=None):
+    """C = alpha op(A) op(B) + beta C (2-D or batched 3-D; 2-D operands broadcast).
+    Picks a split-K factor when the output has few tiles and K is long."""
+    if C is None:
+        batch = max(t.shape[0] if t.dim() == 3 else 1 for t in (A, B))
+        m = A.shape[-1] if tA else A.shape[-2]
+        n = B.shape[-2] if tB else B.shape[-1]
+        C = empty(batch, m, n, like=A) if (A.dim() == 3 or B.dim() == 3) else empty(m, n, like=A)
+        beta = 0.0
+    if splits is None:
+        m, n = C.shape[-2:]
+        k = A.shape[-2] if tA else A.shape[-1]
+        batch = C.shape[0] if C.dim() == 3 else 1
+        tiles = -(-m // 64) * -(-n // 64) * batch
+        splits = 1
+        if tiles < 256 and k >= 1024:
+            splits = max(1, min(-(-512 // tiles), k // 256, 64))
+    return _lib.mm(A, B, C, tA=tA, tB=tB, alpha=alpha, beta=beta, splits=splits)
+
+
+def spd_inv(A, out=None):
+    """(inverse, logdet[b], info[b]) of a batch (b, M, M) or one (M, M) SPD matrix."""
+    batch = A.shape[0] if A.dim() == 3 else 1
+    if out is None:
+        out = torch.empty_like(A)
+    ld = empty(batch, like=A)
+    info = torch.empty(batch, dtype=torch.int32, device=A.device)
+    _lib.spd_inv(A, out, ld, info)
+    return out, ld, info
+
+
+def lincomb(out, a, x, b=0.0, y=None, c=0.0, w=None):
+    call('gpx_lincomb', out.numel(), float(a), P(x), x.numel(), float(b), P(y),
+         y.numel() if y is not None else 1, float(c), P(w), w.numel() if w is not None else 1,
+         P(out), S())
+    return out
+
+
+def colsum(Pm, rows, cols, out=None, alpha=1.0, beta=0.0):
+    if out is None:
+        out = empty(cols, like=Pm)
+        beta = 0.0
+    call('gpx_colsum', rows, cols, P(Pm), float(alpha), float(beta), P(out), S())
+    return out
+
+
+def se(x, z, ls, sf, jitter=0.0, out=None):
+    N, Q = x.shape
+    M = z.shape[0]
+    if out is None:
+        out = empty(N, M, like=x)
+    call('gpx_se_kernel', N, M, Q, P(x), x.stride(0), P(z), P(ls), P(sf), float(jitter), P(out), S())
+    return out
+
+
+def eta_to_R(eta, M):
+    D = eta.shape[0]
+    R = empty(D, M, M, like=eta)
+    call('gpx_eta_to_R', D, M, P(eta), P(R), S())
+    return R
+
+
+def pack_dR(R, G, out=None):
+    D, M, _ = R.shape
+    if out is None:
+        out = empty(D, M * (M + 1) // 2, like=R)
+    call('gpx_pack_dR', D, M, P(R), P(G), P(out), S())
+    return out
+
+
+def sym(X, a=0.5):
+    out = torch.empty_like(X)
+    batch = X.shape[0] if X.dim() == 3 else 1
+    call('gpx_sym', batch, X.shape[-1], float(a), P(X), P(out), S())
+    return out
+
+
+def rowscale(w, X):
+    N, D = w.shape
+    M = X.shape[1]
+    out = empty(D, N, M, like=X)
+    call('gpx_rowscale', N, D, M, P(w), P(X), P(out), S())
+    return out
+
+
+def wsum(w, T, out, alpha=1.0, beta=0.0):
+    N, D = w.shape
+    M = T.shape[2]
+    call('gpx_wsum', N, D, M, float(alpha), P(w), P(T), float(beta), P(out), S())
+    return out
+
+
+def dmall(dm, dv, m):
+    out = torch.empty_like(dm)
+    call('gpx_dmall', dm.numel(), P(dm), P(dv), P(m), P(out), S())
+    return out
+
+
+def hyper(ls, sf, dell, gls, dsf2, sumdv, gsf):
+    dls = torch.empty_like(ls)
+    dsf = torch.empty_like(sf)
+    call('gpx_hyper_grads', ls.numel(), P(ls), P(sf), P(dell), P(gls), P(dsf2), P(sumdv), P(gsf),
+         P(dls), P(dsf), S())
+    return dls, dsf
+
+
+def phi(D, ldK, ldP, sgnP, dotP, ldC, sgnC, dotC, sp, sc):
+    out = empty(1, like=ldK)
+    call('gpx_phi', D, P(ldK), P(ldP), float(sgnP), P(dotP), P(ldC), float(sgnC), P(dotC),
+         float(sp), float(sc), P(out), S())
+    return out
+
+
+def batched_dot(x, y):
+    b = x.shape[0]
+    out = empty(b, like=x)
+    call('gpx_batched_dot', b, x[0].numel(), P(x), P(y), P(out), S())
+    return out
+
+
+def det_var(T, kfu, sf):
+    D, N, M = T.shape
+    v = empty(N, D, like=T)
+    call('gpx_det_var', N, D, M, P(T), P(kfu), P(sf), P(v), S())
+    return v
+
+
+def mm_var(m, Bp, sf):
+    v = torch.empty_like(m)
+    call('gpx_mm_var', m.numel(), P(m), P(Bp), P(sf), P(v), S())
+    return v
+
+
+def psi_fwd(mx, vx, z, ls, sf, with_psi2=True):
+    N, Q = mx.shape
+    M = z.shape[0]
+    psi1 = empty(N, M, like=mx)
+    psi2 = empty(N, M, M, like=mx) if with_psi2 else None
+    call('gpx_psi_fwd', N, M, Q, P(mx), P(vx), P(z), P(ls), P(sf), P(psi1), P(psi2), S())
+    return psi1, psi2
+
+
+def psi_bwd(mx, vx, z, ls, sf, dpsi1, psi1, dpsi2, psi2):
+    N, Q = mx.shape
+    M = z.shape[0]
+    outs = [empty(1, like=mx), empty(Q, like=mx), empty(M, Q, like=mx), empty(N, Q, like=mx),
+            empty(N, Q, like=mx)]
+    work = empty(query('gpx_psi_bwd_workspace', N, M, Q), like=mx)
+    call('gpx_psi_bwd', N, M, Q, P(mx), P(vx), P(z), P(ls), P(sf), P(dpsi1), P(psi1), P(dpsi2),
+         P(psi2), *[P(o) for o in outs], P(work), S())
+    return outs
+
+
+def kfu_bwd(dK, K, x, z, ls, sf, want_dx=False):
+    N, Q = x.shape
+    M = z.shape[0]
+    outs = [empty(1, like=x), empty(Q, like=x), empty(M, Q, like=x),
+            empty(N, Q, like=x) if want_dx else None]
+    work = empty(query('gpx_kfu_bwd_workspace', N, M, Q), like=x)
+    call('gpx_kfu_bwd', N, M, Q, P(dK), P(K), P(x), P(z), P(ls), P(sf), *[P(o) for o in outs],
+         P(work), S())
+    return outs
+
+
+def dtrace(Mm, z, ls, sf):
+    M, Q = z.shape
+    outs = [empty(1, like=z), empty(Q, like=z), empty(M, Q, like=z)]
+    work = empty(query('gpx_dtrace_workspace', M, Q), like=z)
+    call('gpx_dtrace_kzz', M, Q, P(Mm), P(z), P(ls), P(sf), *[P(o) for o in outs], P(work), S())
+    return outs
+
+
+def gauss_tilted(m, v, y, sn, alpha, scale, want_dmall=False):
+    B, D = m.shape
+    dm, dv = torch.empty_like(m), torch.empty_like(m)
+    dma = torch.empty_like(m) if want_dmall else None
+    scal = empty(2, like=m)
+    call('gpx_gauss_tilted', B, D, P(m), P(v), P(y), P(sn), float(alpha), float(scale), P(dm), P(dv),
+         P(dma), P(scal), S())
+    return dm, dv, dma, scal
+
+
+def transition(mp, vp, mt, vt, sn, alpha, scale):
+    B, D = mp.shape
+    outs = [torch.empty_like(mp) for _ in range(4)]
+    scal = empty(2, like=mp)
+    call('gpx_transition_tilted', B, D, P(mp), P(vp), P(mt), P(vt), P(sn), float(alpha),
+         float(scale), *[P(o) for o in outs], P(scal), S())
+    return outs + [scal]
+
+
+def emission(C, Rp, y, mx, vx, alpha, scale):
+    Nb, Din = mx.shape
+    Dout = C.shape[0]
+    outs = [empty(Nb, Din, like=mx), empty(Nb, Din, like=mx), empty(Dout, Din, like=mx),
+            empty(Dout, like=mx), empty(1, like=mx)]
+    work = empty(query('gpx_emission_workspace', Nb, Dout, Din), like=mx)
+    call('gpx_emission_tilted', Nb, Dout, Din, P(C), P(Rp), P(y), y.stride(0), P(mx), mx.stride(0),
+         P(vx), vx.stride(0), float(alpha), float(scale), *[P(o) for o in outs], P(work), S())
+    return outs
+
+
+def emission_predict(C, Rp, mf, vf):
+    Nb, Din = mf.shape
+    Dout = C.shape[0]
+    my = empty(Nb, Dout, like=mf)
+    vy = empty(Nb, Dout, like=mf)
+    call('gpx_emission_predict', Nb, Dout, Din, P(C), P(Rp), P(mf), P(vf), P(my), P(vy), S())
+    return my, vy

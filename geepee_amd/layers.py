@@ -1,0 +1,301 @@
+"""Sparse-GP layer on the GPU: q(u), cavity, moment propagation and adjoints.
+
+API of the reference's ``SGP_Layer`` (geepee/aep_models.py:26-586 over
+geepee/base_models.py:168-658).  State lives in device tensors; every arithmetic
+step is a libgpx kernel launch (``dops``).  Shapes: D = output dims, M = inducing
+points, Q = input dims; matrices are (D, M, M) stacks, vectors (D, M).
+"""
+import torch
+
+from . import dops
+from .config import JITTER, PROP_MM, PROP_MC, PROP_LIN
+from .dops import mm, lincomb, colsum
+
+
+def _col(v):
+    """(D, M) -> (D, M, 1) view for batched mat-vecs."""
+    return v.reshape(v.shape[0], v.shape[1], 1)
+
+
+def _add_outer(C, u, v, alpha):
+    """C[d] += alpha u_d v_d^T (batched rank-1)."""
+    mm(_col(u), _col(v), C, tB=True, alpha=alpha, beta=1.0)
+    return C
+
+
+def _sandwich(L, X, R, alpha=1.0, out=None):
+    """alpha L X R with L, R (M, M) or (D, M, M) and X (D, M, M)."""
+    T = mm(X, R)
+    if out is None:
+        return mm(L, T, alpha=alpha)
+    return mm(L, T, out, alpha=alpha, beta=1.0)
+
+
+class SGP_Layer(object):
+
+    def __init__(self, no_train, input_size, output_size, no_pseudo, nat_param=True):
+        self.N = no_train
+        self.Din = input_size
+        self.Dout = output_size
+        self.M = no_pseudo
+        self.nat_param = nat_param
+        self._pred_ready = False
+
+    # ------------------------------------------------------------------
+    # q(u) and Kuu (base_models.py:454-488, 630-658)
+    # ------------------------------------------------------------------
+    def update_hypers(self, params, key_suffix=''):
+        """params: dict of device tensors (see models._DeviceParams)."""
+        self.ls = params['ls' + key_suffix]
+        self.sf = params['sf' + key_suffix]
+        self.zu = params['zu' + key_suffix]
+        self.theta_2 = params['eta2' + key_suffix]
+        self.theta_1_R = dops.eta_to_R(params['eta1_R' + key_suffix], self.M)
+        self.theta_1 = mm(self.theta_1_R, self.theta_1_R, tA=True)
+        self.compute_kuu()
+        self.update_posterior()
+
+    def compute_kuu(self):
+        self.Kuu = dops.se(self.zu, self.zu, self.ls, self.sf, jitter=JITTER)
+        self.Kuuinv, self.ld_Kuu, self.info_Kuu = dops.spd_inv(self.Kuu)
+
+    def update_posterior(self):
+        D, M = self.Dout, self.M
+        if self.nat_param:
+            self.Suinv = lincomb(dops.empty(D, M, M, like=self.Kuu), 1.0, self.Kuuinv, 1.0, self.theta_1)
+            self.Su, self.ld_Suinv, self.info_Su = dops.spd_inv(self.Suinv)
+            self.mu = mm(self.Su, _col(self.theta_2)).reshape(D, M)
+            self._ld_post, self._sgn_post = self.ld_Suinv, -1.0
+        else:
+            self.Su = self.theta_1
+            self.Suinv, self.ld_Su, self.info_Su = dops.spd_inv(self.Su)
+            self.mu = self.theta_2
+            self._ld_post, self._sgn_post = self.ld_Su, 1.0
+        self.Splusmm = self.Su.clone()
+        _add_outer(self.Splusmm, self.mu, self.mu, 1.0)
+        self._pred_ready = False
+
+    def _posterior_predictive(self):
+        """A, B_sto, B_det of update_posterior (base_models.py:477-488)."""
+        if self._pred_ready:
+            return
+        Ki = self.Kuuinv
+        self.A = mm(self.mu, Ki, tB=True)
+        self.B_sto = lincomb(torch.empty_like(self.Su), -1.0, Ki)
+        _sandwich(Ki, self.Splusmm, Ki, out=self.B_sto)
+        self.B_det = lincomb(torch.empty_like(self.Su), -1.0, Ki)
+        _sandwich(Ki, self.Su, Ki, out=self.B_det)
+        self._pred_ready = True
+
+    # ------------------------------------------------------------------
+    # cavity and phi (aep_models.py:62-114, 513-546)
+    # ------------------------------------------------------------------
+    def compute_cavity(self, alpha):
+        D, M = self.Dout, self.M
+        beta = (self.N - alpha) * 1.0 / self.N
+        Ki = self.Kuuinv
+        if self.nat_param:
+            self.Suhatinv = lincomb(dops.empty(D, M, M, like=Ki), 1.0, Ki, beta, self.theta_1)
+            self.Suhat, self.ld_Suhatinv, self.info_Suhat = dops.spd_inv(self.Suhatinv)
+            self.muhat = mm(self.Suhat, _col(self.theta_2), alpha=beta).reshape(D, M)
+        else:
+            # Kuuinv + beta (Suinv - Kuuinv);  muhat = Suhat beta Suinv mu
+            self.Suhatinv = lincomb(dops.empty(D, M, M, like=Ki), 1.0 - beta, Ki, beta, self.Suinv)
+            self.Suhat, self.ld_Suhatinv, self.info_Suhat = dops.spd_inv(self.Suhatinv)
+            f2 = mm(self.Suinv, _col(self.mu))
+            self.muhat = mm(self.Suhat, f2, alpha=beta).reshape(D, M)
+        self.Ahat = mm(self.muhat, Ki, tB=True)
+        self.Splusmmhat = self.Suhat.clone()
+        _add_outer(self.Splusmmhat, self.muhat, self.muhat, 1.0)
+        self.Bhat_sto = lincomb(torch.empty_like(self.Suhat), -1.0, Ki)
+        _sandwich(Ki, self.Splusmmhat, Ki, out=self.Bhat_sto)
+        self.Bhat_det = lincomb(torch.empty_like(self.Suhat), -1.0, Ki)
+        _sandwich(Ki, self.Suhat, Ki, out=self.Bhat_det)
+
+    def compute_phi(self, alpha=1.0):
+        """Device scalar (1,) of the weighted log-partitions (aep_models.py:97-114)."""
+        N = self.N
+        dotP = dops.batched_dot(self.mu, mm(self.Suinv, _col(self.mu)).reshape(self.Dout, self.M))
+        dotC = dops.batched_dot(self.muhat,
+                                mm(self.Suhatinv, _col(self.muhat)).reshape(self.Dout, self.M))
+        return dops.phi(self.Dout, self.ld_Kuu, self._ld_post, self._sgn_post, dotP,
+                        self.ld_Suhatinv, -1.0, dotC, N * 1.0 / alpha - 1.0, -N * 1.0 / alpha)
+
+    # ------------------------------------------------------------------
+    # forward propagation (aep_models.py:116-199; base_models.py:235-307)
+    # ------------------------------------------------------------------
+    def forward_prop_thru_cav(self, mx, vx=None, mode=PROP_MM):
+        if vx is None:
+            return self._forward_prop_deterministic_thru_cav(mx)
+        if mode == PROP_MM:
+            return self._forward_prop_random_thru_cav_mm(mx, vx)
+        if mode in (PROP_MC, PROP_LIN):
+            raise NotImplementedError('propagation mode %s is not on the GPU path' % mode)
+        raise NotImplementedError('unknown propagation mode')
+
+    def _forward_prop_deterministic_thru_cav(self, x):
+        kfu = dops.se(x, self.zu, self.ls, self.sf)
+        mout = mm(kfu, self.Ahat, tB=True)
+        self._T_det = mm(kfu, self.Bhat_det)          # (D, N, M): kfu Bhat_det[d]
+        vout = dops.det_var(self._T_det, kfu, self.sf)
+        return mout, vout, kfu
+
+    def _forward_prop_random_thru_cav_mm(self, mx, vx):
+        N = mx.shape[0]
+        M, D = self.M, self.Dout
+        psi1, psi2 = dops.psi_fwd(mx, vx, self.zu, self.ls, self.sf)
+        mout = mm(psi1, self.Ahat, tB=True)
+        Bp = mm(psi2.reshape(N, M * M), self.Bhat_sto.reshape(D, M * M), tB=True)
+        vout = dops.mm_var(mout, Bp, self.sf)
+        return mout, vout, psi1, psi2
+
+    def forward_prop_thru_post(self, mx, vx=None, mode=PROP_MM, return_info=False):
+        self._posterior_predictive()
+        if vx is None:
+            kfu = dops.se(mx, self.zu, self.ls, self.sf)
+            mout = mm(kfu, self.A, tB=True)
+            T = mm(kfu, self.B_det)
+            vout = dops.det_var(T, kfu, self.sf)
+            return (mout, vout, kfu) if return_info else (mout, vout)
+        if mode != PROP_MM:
+            raise NotImplementedError('propagation mode %s is not on the GPU path' % mode)
+        N = mx.shape[0]
+        M, D = self.M, self.Dout
+        psi1, psi2 = dops.psi_fwd(mx, vx, self.zu, self.ls, self.sf)
+        mout = mm(psi1, self.A, tB=True)
+        Bp = mm(psi2.reshape(N, M * M), self.B_sto.reshape(D, M * M), tB=True)
+        vout = dops.mm_var(mout, Bp, self.sf)
+        return (mout, vout, psi1, psi2) if return_info else (mout, vout)
+
+    # ------------------------------------------------------------------
+    # adjoints
+    # ------------------------------------------------------------------
+    def _cav_grad_u(self, dmu, dSu, alpha):
+        """compute_cav_grad_u (aep_models.py:548-586) -> deta1_R, deta2, dKuuinv."""
+        D, M = self.Dout, self.M
+        beta = (self.N - alpha) * 1.0 / self.N
+        if self.nat_param:
+            _add_outer(dSu, dmu, self.theta_2, beta)
+            dSuinv = _sandwich(self.Suhat, dSu, self.Suhat, alpha=-1.0)
+            dKuuinv = colsum(dSuinv, D, M * M).reshape(M, M)
+            deta1 = dops.pack_dR(self.theta_1_R, lincomb(torch.empty_like(dSuinv), beta, dSuinv))
+            deta2 = mm(self.Suhat, _col(dmu), alpha=beta).reshape(D, M)
+        else:
+            f2 = mm(self.Suinv, _col(self.mu)).reshape(D, M)
+            _add_outer(dSu, dmu, f2, beta)
+            dSuhatinv = _sandwich(self.Suhat, dSu, self.Suhat, alpha=-1.0)
+            Suhatdmu = mm(self.Suhat, _col(dmu)).reshape(D, M)
+            dSuinv = lincomb(torch.empty_like(dSuhatinv), beta, dSuhatinv)
+            _add_outer(dSuinv, Suhatdmu, self.mu, beta)
+            dtheta1 = _sandwich(self.Suinv, dSuinv, self.Suinv, alpha=-1.0)
+            deta1 = dops.pack_dR(self.theta_1_R, dtheta1)
+            deta2 = mm(self.Suinv, _col(Suhatdmu), alpha=beta).reshape(D, M)
+            dKuuinv = colsum(dSuhatinv, D, M * M, alpha=(1 - beta)).reshape(M, M)
+        return deta1, deta2, dKuuinv
+
+    def _post_grad_u(self, dmu, dSu):
+        """compute_posterior_grad_u (base_models.py:490-516)."""
+        D, M = self.Dout, self.M
+        if self.nat_param:
+            _add_outer(dSu, dmu, self.theta_2, 1.0)
+            dSuinv = _sandwich(self.Su, dSu, self.Su, alpha=-1.0)
+            dKuuinv = colsum(dSuinv, D, M * M).reshape(M, M)
+            deta1 = dops.pack_dR(self.theta_1_R, dSuinv)
+            deta2 = mm(self.Su, _col(dmu)).reshape(D, M)
+            return deta1, deta2, dKuuinv
+        deta1 = dops.pack_dR(self.theta_1_R, dSu)
+        return deta1, dmu, None
+
+    def backprop_grads_reg(self, m, v, dm, dv, kfu, x, alpha=1.0):
+        """backprop_grads_reg (aep_models.py:412-511): dm, dv already scaled."""
+        N = self.N
+        D, M = self.Dout, self.M
+        B = kfu.shape[0]
+        Ki = self.Kuuinv
+        scale_post = N * 1.0 / alpha - 1.0
+        scale_cav = -N * 1.0 / alpha
+        kfuKi = mm(kfu, Ki)
+        # grads wrt kfu: dm Ahat + 2 sum_d dv_nd (kfu Bhat_det[d])_n
+        dkfu = mm(dm, self.Ahat)
+        dops.wsum(dv, self._T_det, dkfu, alpha=2.0, beta=1.0)
+        dsf2, dell, dzu, _ = dops.kfu_bwd(dkfu, kfu, x, self.zu, self.ls, self.sf)
+        sumdv = colsum(dv, B * D, 1)
+        # cavity mean / covariance
+        dmucav = mm(dm, kfuKi, tA=True)
+        dSucav = mm(dops.rowscale(dv, kfuKi), kfuKi, tA=True)
+        Suinvmuhat = mm(self.Suhatinv, _col(self.muhat)).reshape(D, M)
+        lincomb(dmucav, 1.0, dmucav, scale_cav, Suinvmuhat)
+        lincomb(dSucav, 1.0, dSucav, 0.5 * scale_cav, self.Suhatinv)
+        _add_outer(dSucav, Suinvmuhat, Suinvmuhat, -0.5 * scale_cav)
+        deta1_cav, deta2_cav, dKi_cav = self._cav_grad_u(dmucav, dSucav, alpha)
+        # posterior
+        Suinvmu = mm(self.Suinv, _col(self.mu)).reshape(D, M)
+        dmu = lincomb(torch.empty_like(Suinvmu), scale_post, Suinvmu)
+        dSu = lincomb(torch.empty_like(self.Suinv), 0.5 * scale_post, self.Suinv)
+        _add_outer(dSu, Suinvmu, Suinvmu, -0.5 * scale_post)
+        deta1_post, deta2_post, dKi_post = self._post_grad_u(dmu, dSu)
+        # Kuuinv: phi prior, Ahat, Bhat_det, cavity, posterior
+        dAhat = mm(dm, kfu, tA=True)
+        dKi = mm(dAhat, self.muhat, tA=True)
+        lincomb(dKi, 1.0, dKi, -0.5 * D, self.Kuu)
+        dBhat = mm(dops.rowscale(dv, kfu), kfu, tA=True)
+        KiS = mm(Ki, self.Suhat)
+        mm(KiS.reshape(D * M, M), dBhat.reshape(D * M, M), dKi, tA=True, alpha=2.0, beta=1.0)
+        colsum(dBhat, D, M * M, out=dKi.reshape(-1), alpha=-1.0, beta=1.0)
+        lincomb(dKi, 1.0, dKi, 1.0, dKi_cav)
+        if dKi_post is not None:
+            lincomb(dKi, 1.0, dKi, 1.0, dKi_post)
+        M_inner = _sandwich(Ki, dKi, Ki, alpha=-1.0)
+        gsf, gls, gz = dops.dtrace(M_inner, self.zu, self.ls, self.sf)
+        dls, dsf = dops.hyper(self.ls, self.sf, dell, gls, dsf2, sumdv, gsf)
+        lincomb(dzu, 1.0, dzu, 1.0, gz)
+        deta1 = lincomb(deta1_cav, 1.0, deta1_cav, 1.0, deta1_post)
+        deta2 = lincomb(deta2_cav, 1.0, deta2_cav, 1.0, deta2_post)
+        return {'sf': dsf, 'ls': dls, 'zu': dzu, 'eta1_R': deta1, 'eta2': deta2}
+
+    def backprop_grads_lvm_mm(self, m, v, dm, dv, psi1, psi2, mx, vx, alpha=1.0, dm_all=None):
+        """backprop_grads_lvm_mm (aep_models.py:201-304): dm, dv already scaled.
+        Returns (hyper grads, {'mx', 'vx'})."""
+        N = self.N
+        D, M = self.Dout, self.M
+        B = psi1.shape[0]
+        Ki = self.Kuuinv
+        beta = (N - alpha) * 1.0 / N
+        scale_post = N * 1.0 / alpha - 1.0
+        scale_cav = -N * 1.0 / alpha
+        if dm_all is None:
+            dm_all = dops.dmall(dm, dv, m)
+        dAhat = mm(dm_all, psi1, tA=True)
+        dBhat = mm(dv, psi2.reshape(B, M * M), tA=True).reshape(D, M, M)
+        dpsi1 = mm(dm_all, self.Ahat)
+        Bsym = dops.sym(self.Bhat_sto, 0.5)
+        dpsi2 = mm(dv, Bsym.reshape(D, M * M)).reshape(B, M, M)
+        dsf2, dell, dzu, dmx, dvx = dops.psi_bwd(mx, vx, self.zu, self.ls, self.sf,
+                                                 dpsi1, psi1, dpsi2, psi2)
+        del dpsi2
+        sumdv = colsum(dv, B * D, 1)
+        dvcav = _sandwich(Ki, dBhat, Ki)
+        dmcav = mm(dvcav, _col(self.muhat), alpha=2.0).reshape(D, M)
+        mm(dAhat, Ki, dmcav, tB=True, beta=1.0)
+        _add_outer(dvcav, dmcav, self.theta_2, beta)
+        dvcavinv = _sandwich(self.Suhat, dvcav, self.Suhat, alpha=-1.0)
+        dtheta2 = mm(self.Suhat, _col(dmcav), alpha=beta).reshape(D, M)
+        dKi = colsum(dvcavinv, D, M * M).reshape(M, M)
+        mm(dAhat, self.muhat, dKi, tA=True, beta=1.0)
+        KiS = mm(Ki, self.Splusmmhat)
+        mm(KiS.reshape(D * M, M), dBhat.reshape(D * M, M), dKi, tA=True, alpha=2.0, beta=1.0)
+        colsum(dBhat, D, M * M, out=dKi.reshape(-1), alpha=-1.0, beta=1.0)
+        Minner = colsum(self.Splusmm, D, M * M, alpha=scale_post).reshape(M, M)
+        colsum(self.Splusmmhat, D, M * M, out=Minner.reshape(-1), alpha=scale_cav, beta=1.0)
+        lincomb(Minner, 1.0, Minner, -2.0, dKi)
+        dtheta1 = lincomb(torch.empty_like(dvcavinv), beta, dvcavinv, -0.5 * scale_post, self.Splusmm,
+                          -0.5 * scale_cav * beta, self.Splusmmhat)
+        lincomb(dtheta2, 1.0, dtheta2, scale_post, self.mu, scale_cav * beta, self.muhat)
+        deta1 = dops.pack_dR(self.theta_1_R, dtheta1)
+        M_all = lincomb(torch.empty_like(Ki), 0.5 * D, Ki)
+        _sandwich(Ki, Minner, Ki, alpha=0.5, out=M_all)
+        gsf, gls, gz = dops.dtrace(M_all, self.zu, self.ls, self.sf)
+        dls, dsf = dops.hyper(self.ls, self.sf, dell, gls, dsf2, sumdv, gsf)
+        lincomb(dzu, 1.0, dzu, 1.0, gz)
+        grad = {'sf': dsf, 'ls': dls, 'zu': dzu, 'eta1_R': deta1, 'eta2': dtheta2}
+        return grad, {'mx': dmx, 'vx': dvx}

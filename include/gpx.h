@@ -1,0 +1,213 @@
+/* libgpx: MI355X (gfx950) FP64 kernels for the sparse-GP approximate-EP hot path.
+ *
+ * C-ABI over caller-owned device memory.  Conventions for every entry point:
+ *   - arrays are float64, row-major, on the current HIP device;
+ *   - log hyper-parameters (ls, sf, sn, R) are read from device pointers, so no
+ *     host synchronisation is needed and calls can be captured in a hipGraph;
+ *   - `stream` is a hipStream_t (NULL = default stream);
+ *   - the return value is 0 on success, GPX_EINVAL (-1) for an invalid argument
+ *     and GPX_EHIP (-2) when a launch fails; no exceptions cross the ABI;
+ *   - functions with a `work` argument need a device workspace of the size the
+ *     matching *_workspace() query returns (in doubles);
+ *   - nothing allocates device memory and nothing keeps state between calls.
+ *
+ * Each declaration names the reference code (thangbui/geepee, file:line) whose
+ * arithmetic it replaces.  The reference's only native boundary is the weave
+ * inline C++ of compute_psi_weave (geepee/kernels.py:181-240); the other entry
+ * points replace numpy / scipy calls on the same path.
+ */
+#ifndef GPX_H
+#define GPX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPX_OK 0
+#define GPX_EINVAL (-1)
+#define GPX_EHIP (-2)
+
+/* ---- dense algebra -------------------------------------------------------- */
+
+/* C[b] = alpha op(A[b]) op(B[b]) + beta C[b] on the FP64 matrix cores; strides of 0
+ * broadcast an operand over the batch; splits > 1 splits K into partial slabs summed in a
+ * fixed order (needs gpx_gemm_workspace doubles); tile = 0 (auto), 32 or 64.
+ * Replaces the np.dot / np.einsum contractions of geepee/aep_models.py:156, 197, 239-258,
+ * 453-493, 525-546 and geepee/base_models.py:470-488, 497. */
+long gpx_gemm_workspace(int m, int n, int batch, int splits);
+int gpx_gemm_batched(int transA, int transB, int m, int n, int k, double alpha,
+                     const double *A, int lda, long strideA, const double *B, int ldb,
+                     long strideB, double beta, double *C, int ldc, long strideC, int batch,
+                     int splits, double *work, int tile, void *stream);
+
+/* Batched SPD inverse and log-determinant, one workgroup per matrix (M <= 256):
+ * out[b] = inv(A[b]) (out may alias A or be NULL), logdet[b] = log det A[b],
+ * info[b] = 0 or k+1 when pivot k is not positive (outputs NaN).
+ * Replaces np.linalg.inv / slogdet / solve at geepee/base_models.py:462-476 and
+ * geepee/aep_models.py:62-95, 523-533. */
+int gpx_spd_max_m(void);
+int gpx_spd_inv_batched(int M, int batch, const double *A, int lda, long strideA, double *out,
+                        int ldo, long strideOut, double *logdet, int *info, void *stream);
+
+/* ---- SE kernel and psi statistics ------------------------------------------ */
+
+/* K[n,m] = exp(2 sf) exp(-sum_q (x_nq - z_mq)^2 / (2 exp(2 ls_q))) (+ jitter if n == m).
+ * Replaces compute_kernel (geepee/kernels.py:10-22) as called at base_models.py:277, 462
+ * and aep_models.py:154. */
+int gpx_se_kernel(int N, int M, int Q, const double *x, int ldx, const double *z,
+                  const double *ls, const double *sf, double jitter, double *K, void *stream);
+
+/* psi1 (N x M) and psi2 (N x M x M, NULL to skip) for x_n ~ N(mx_n, diag vx_n); psi2 is
+ * exactly symmetric.  Q <= gpx_psi_qmax().
+ * Replaces compute_psi_weave (geepee/kernels.py:181-240), the reference's weave FFI. */
+int gpx_psi_qmax(void);
+int gpx_psi_fwd(int N, int M, int Q, const double *mx, const double *vx, const double *z,
+                const double *ls, const double *sf, double *psi1, double *psi2, void *stream);
+
+/* Adjoint of gpx_psi_fwd given dL/dpsi1 and a symmetric dL/dpsi2: dsf2 (w.r.t. exp(2 sf)),
+ * dell (Q, w.r.t. exp(ls)), dZ (M x Q), dmx, dvx (N x Q).
+ * Replaces compute_psi_derivatives -> psi1compDer / psi2compDer
+ * (geepee/kernels.py:302-309, 355-444). */
+long gpx_psi_bwd_workspace(int N, int M, int Q);
+int gpx_psi_bwd(int N, int M, int Q, const double *mx, const double *vx, const double *z,
+                const double *ls, const double *sf, const double *dpsi1, const double *psi1,
+                const double *dpsi2, const double *psi2, double *dsf2, double *dell, double *dZ,
+                double *dmx, double *dvx, double *work, void *stream);
+
+/* Adjoint of K(x, Z): dsf2, dell, dZ and optionally dx (N x Q).
+ * Replaces compute_kfu_derivatives -> kfucompDer (geepee/kernels.py:312-313, 381-399). */
+long gpx_kfu_bwd_workspace(int N, int M, int Q);
+int gpx_kfu_bwd(int N, int M, int Q, const double *dK, const double *K, const double *x,
+                const double *z, const double *ls, const double *sf, double *dsf2,
+                double *dell, double *dZ, double *dx, double *work, void *stream);
+
+/* d tr(Mm Kzz) / d(2 sf, 2 ls, z) for Kzz = SE(z, z) without jitter: gsf, gls (Q), gz (M x Q).
+ * Replaces d_trace_MKzz_dhypers (geepee/kernels.py:447-475). */
+long gpx_dtrace_workspace(int M, int Q);
+int gpx_dtrace_kzz(int M, int Q, const double *Mm, const double *z, const double *ls,
+                   const double *sf, double *gsf, double *gls, double *gz, double *work,
+                   void *stream);
+
+/* ---- q(u) plumbing and layer helpers --------------------------------------- */
+
+/* R[d] = upper-triangular from packed row-major triu values with exp'd diagonal.
+ * Replaces the loop of Base_SGP_Layer.update_hypers (geepee/base_models.py:645-653). */
+int gpx_eta_to_R(int D, int M, const double *eta, double *R, void *stream);
+
+/* out[d, triu(i,j)] = (R (G + G^T))[i,j] * (i == j ? R_ii : 1).
+ * Replaces the deta1_R packing (geepee/aep_models.py:267-280, 573-584;
+ * base_models.py:504-514). */
+int gpx_pack_dR(int D, int M, const double *R, const double *G, double *out, void *stream);
+
+/* out[i] = a x[i % nx] + b y[i % ny] + c w[i % nw] (y, w may be NULL): the broadcast
+ * additions of the layer algebra (e.g. Kuuinv + beta theta1, aep_models.py:524). */
+int gpx_lincomb(long n, double a, const double *x, long nx, double b, const double *y,
+                long ny, double c, const double *w, long nw, double *out, void *stream);
+
+/* out[b] = a (X[b] + X[b]^T) (the dL/dpsi2 symmetrisation of geepee/kernels.py:415-418). */
+int gpx_sym(int batch, int M, double a, const double *X, double *out, void *stream);
+
+/* out[j] = alpha sum_i P[i, j] + beta out[j] (np.sum(..., axis=0) on the path). */
+int gpx_colsum(int rows, int cols, const double *P, double alpha, double beta, double *out,
+               void *stream);
+
+/* v[n,d] = exp(2 sf) + sum_a T[d,n,a] kfu[n,a] with T[d] = kfu Bhat_det[d]
+ * (geepee/aep_models.py:153-157, base_models.py:280-282). */
+int gpx_det_var(int N, int D, int M, const double *T, const double *kfu, const double *sf,
+                double *v, void *stream);
+
+/* v = exp(2 sf) + Bpsi2 - m^2 (geepee/aep_models.py:196-198, base_models.py:302-304). */
+int gpx_mm_var(long n, const double *m, const double *Bp, const double *sf, double *v,
+               void *stream);
+
+/* out[b] = sum_i x[b,i] y[b,i] (the mu^T Sigma^-1 mu terms of aep_models.py:80, 93). */
+int gpx_batched_dot(int batch, int n, const double *x, const double *y, double *out,
+                    void *stream);
+
+/* out[d][n][m] = w[n,d] X[n][m]: the dv-weighted rows behind the syrk-like einsums
+ * 'nd,na,nb->dab' (geepee/aep_models.py:464, 490). */
+int gpx_rowscale(int N, int D, int M, const double *w, const double *X, double *out, void *stream);
+
+/* out[n][m] = alpha sum_d w[n,d] T[d][n][m] + beta out[n][m]: the einsum
+ * 'nd,dab,na->nb' (geepee/aep_models.py:455) with T[d] = kfu Bhat_det[d]. */
+int gpx_wsum(int N, int D, int M, double alpha, const double *w, const double *T, double beta,
+             double *out, void *stream);
+
+/* out = dm - 2 dv m (geepee/aep_models.py:235). */
+int gpx_dmall(long n, const double *dm, const double *dv, const double *m, double *out, void *stream);
+
+/* dls = dell exp(ls) + 2 gls, dsf = 2 exp(2 sf) (dsf2 + sumdv) + 2 gsf: the hyper-parameter
+ * chain of geepee/aep_models.py:244-247, 296-298, 456-460, 502-504. */
+int gpx_hyper_grads(int Q, const double *ls, const double *sf, const double *dell, const double *gls,
+                    const double *dsf2, const double *sumdv, const double *gsf, double *dls, double *dsf,
+                    void *stream);
+
+/* phi = (D/2) ldK + sp/2 sum_d (sgnP ldP_d + dotP_d) + sc/2 sum_d (sgnC ldC_d + dotC_d):
+ * SGP_Layer.compute_phi (geepee/aep_models.py:62-114). */
+int gpx_phi(int D, const double *ldK, const double *ldP, double sgnP, const double *dotP,
+            const double *ldC, double sgnC, const double *dotC, double sp, double sc, double *out,
+            void *stream);
+
+/* ---- likelihoods ----------------------------------------------------------- */
+
+/* Gaussian tilted log-normaliser and its adjoint: scal[0] = logZ (incl. the alpha
+ * constant), scal[1] = dsn * scale; dm, dv = scale * dlogZ/dm, dlogZ/dv; dmall (optional)
+ * = dm - 2 dv m.  Replaces Gauss_Layer.compute_log_Z + backprop_grads
+ * (geepee/lik_layers.py:104-181). */
+int gpx_gauss_tilted(int B, int D, const double *m, const double *v, const double *y,
+                     const double *sn, double alpha, double scale, double *dm, double *dv,
+                     double *dmall, double *scal, void *stream);
+
+/* GPSSM transition tilted normaliser (scaled): scal[0] = scale * logZ, scal[1] = dsn.
+ * Replaces SGPSSM.compute_transition_tilted (geepee/aep_models.py:1317-1343). */
+int gpx_transition_tilted(int B, int D, const double *mp, const double *vp, const double *mt,
+                          const double *vt, const double *sn, double alpha, double scale,
+                          double *dmall, double *dvp, double *dmt, double *dvt, double *scal,
+                          void *stream);
+
+/* Linear-Gaussian emission tilted normaliser (Dout <= 8): scal[0] = scale * logZ, dC, dR,
+ * dmx, dvx (all scaled).  Replaces Gauss_Emis.compute_emission_tilted
+ * (geepee/lik_layers.py:573-627). */
+long gpx_emission_workspace(int Nb, int Dout, int Din);
+int gpx_emission_tilted(int Nb, int Dout, int Din, const double *C, const double *Rp,
+                        const double *y, int ldy, const double *mx, int ldm, const double *vx,
+                        int ldv, double alpha, double scale, double *dmx, double *dvx, double *dC,
+                        double *dR, double *scal, double *work, void *stream);
+
+/* Emission predictive mean C mf and variance diag(C diag(vf) C^T) + R.
+ * Replaces Gauss_Emis.output_probabilistic (geepee/lik_layers.py:558-571), diagonal only as
+ * returned by Base_SGPSSM.predict_y (geepee/base_models.py:1544-1560). */
+int gpx_emission_predict(int Nb, int Dout, int Din, const double *C, const double *Rp, const double *mf,
+                         const double *vf, double *my, double *vy, void *stream);
+
+/* ---- latent-variable (x-factor) terms --------------------------------------- */
+
+/* SGPLVM cavity inputs of rows idx (NULL = all rows): get_cavity_x
+ * (geepee/aep_models.py:840-861) with factor_x2 = exp(2 x2) (base_models.py:895-911). */
+int gpx_lvm_x_cavity(int B, int Q, const long *idx, const double *x1, const double *x2, double alpha,
+                     double prior_mean, double prior_var, int nat_param, double *mcav, double *vcav,
+                     void *stream);
+
+/* SGPLVM x gradients (rows idx of the N x Q outputs) and scal[0] = the x log-partition
+ * energy: aep_models.py:784-805, 817-838, 863-867 and base_models.py:913-929. */
+int gpx_lvm_x_grads(int N, int B, int Q, const long *idx, const double *x1, const double *x2,
+                    const double *gmx, const double *gvx, double alpha, double prior_mean,
+                    double prior_var, int nat_param, double *g1, double *g2, double *scal,
+                    void *stream);
+
+/* SGPSSM cavity means / variances of every x_t with the 3/2/prior factor ties:
+ * compute_cavity_x (geepee/aep_models.py:1376-1387) over base_models.py:1719-1725. */
+int gpx_ssm_x_cavity(int N, int Q, const double *xf1, const double *x2, double alpha, double prior_mean,
+                     double prior_var, double *cm, double *cv, void *stream);
+
+/* SGPSSM x-factor gradients and scal[0] = phi_prior_x + phi_post_x + phi_cav_x for the
+ * window [s, e): aep_models.py:1170-1196, 1208-1315, 1389-1437. */
+int gpx_ssm_x_grads(int N, int Q, int s, int e, const double *xf1, const double *x2, double alpha,
+                    double prior_mean, double prior_var, const double *dup, const double *vup, int ldu,
+                    const double *dmt, const double *dvt, const double *dmn, const double *dvn, int ldn,
+                    double *g1, double *g2, double *scal, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPX_H */

@@ -1,0 +1,49 @@
+"""Host-only checks of the C-ABI: the library loads without a GPU, exports every
+symbol include/gpx.h declares, and rejects invalid arguments with GPX_EINVAL
+before touching the device."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from geepee_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, 'include', 'gpx.h')).read()
+    return sorted(set(re.findall(r'^\s*(?:long|int)\s+(gpx_\w+)\s*\(', text, flags=re.M)))
+
+
+def test_header_symbols_exported_and_bound():
+    L = _lib.load()
+    names = _declared()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), n
+        assert n in _lib.SIGNATURES, n
+
+
+def test_invalid_arguments_return_einval():
+    L = _lib.load()
+    assert L.gpx_gemm_batched(0, 0, -1, 4, 4, 1.0, None, 4, 0, None, 4, 0, 0.0, None, 4, 0, 1, 1,
+                              None, 0, None) == -1
+    assert L.gpx_spd_inv_batched(300, 1, None, 300, 0, None, 300, 0, None, None, None) == -1
+    assert L.gpx_psi_fwd(4, 4, 99, None, None, None, None, None, None, None, None) == -1
+    assert L.gpx_gauss_tilted(4, 1, None, None, None, None, 0.5, 1.0, None, None, None, None, None) == -1
+    assert L.gpx_emission_tilted(4, 99, 2, None, None, None, 99, None, 2, None, 2, 0.5, 1.0, None,
+                                 None, None, None, None, None, None) == -1
+    # empty problems are a no-op success
+    assert L.gpx_gemm_batched(0, 0, 0, 4, 4, 1.0, None, 4, 0, None, 4, 0, 0.0, None, 4, 0, 1, 1,
+                              None, 0, None) == 0
+
+
+def test_workspace_queries():
+    L = _lib.load()
+    assert L.gpx_gemm_workspace(10, 20, 3, 1) == 0
+    assert L.gpx_gemm_workspace(10, 20, 3, 4) == 4 * 3 * 10 * 20
+    assert L.gpx_psi_bwd_workspace(100, 10, 2) > 100 * 10 * 10
+    assert L.gpx_spd_max_m() == 256
+    assert L.gpx_psi_qmax() == 16

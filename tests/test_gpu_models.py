@@ -1,0 +1,142 @@
+"""Model-level parity: the GPU objective_function / predict of every model against the
+oracle on the same seeded inputs and the same minibatch draws (GPU only).
+
+Bar: normwise relative error <= 1e-9 for the energy and every gradient tensor
+(north star: within 1e-9 relative of the float64 reference).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import aep_oracle as O
+from geepee_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-9
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def nrel(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300)
+
+
+def compare(got, want, tol=TOL):
+    eg, gg = got
+    ew, gw = want
+    assert sorted(gg) == sorted(gw)
+    errs = {k: nrel(gg[k], gw[k]) for k in gw}
+    errs['energy'] = abs(eg - ew) / max(abs(ew), 1e-300)
+    bad = {k: v for k, v in errs.items() if not v <= tol}
+    assert not bad, 'parity failures: %r' % bad
+    return max(errs.values())
+
+
+@pytest.mark.parametrize('nat', [True, False])
+def test_sgpr_parity(nat):
+    from geepee_amd import aep_models as G
+    say('sgpr parity nat=%s' % nat)
+    x, y, p = W.sgpr_problem(N=200, M=20, Din=2, Dout=2, seed=0)
+    model = G.SGPR(x, y, 20, nat_param=nat)
+    np.random.seed(5)
+    got = model.objective_function(p, 200, alpha=0.5)
+    want = O.sgpr_objective(x, y, p, 20, 200, alpha=0.5, nat_param=nat)
+    compare(got, want)
+    # minibatch: same legacy-RNG draw on both sides
+    np.random.seed(11)
+    got = model.objective_function(p, 64, alpha=0.3)
+    np.random.seed(11)
+    want = O.sgpr_objective(x, y, p, 20, 64, alpha=0.3, nat_param=nat)
+    compare(got, want)
+
+
+def test_sgpr_predict():
+    from geepee_amd import aep_models as G
+    say('sgpr predict')
+    x, y, p = W.sgpr_problem(N=100, M=15, Din=1, Dout=1, seed=1)
+    model = G.SGPR(x, y, 15)
+    model.update_hypers(p)
+    xs = np.linspace(-3, 3, 57)[:, None]
+    mf, vf = model.predict_f(xs)
+    rm, rv = O.sgpr_predict(p, 15, 100, xs)
+    assert nrel(mf, rm) < TOL and nrel(vf, rv) < TOL
+    my, vy = model.predict_y(xs)
+    rm, rv = O.sgpr_predict(p, 15, 100, xs, noisy=True)
+    assert nrel(my, rm) < TOL and nrel(vy, rv) < TOL
+
+
+def test_sdgpr_parity_and_predict():
+    from geepee_amd import aep_models as G
+    say('sdgpr parity')
+    x, y, p, Ms, hidden = W.sdgpr_problem(N=300, widths=(4, 3, 2, 1), M=16, seed=2)
+    model = G.SDGPR(x, y, Ms, hidden)
+    np.random.seed(3)
+    got = model.objective_function(p, 100, alpha=0.5)
+    np.random.seed(3)
+    want = O.sdgpr_objective(x, y, p, Ms, 100, alpha=0.5)
+    compare(got, want)
+    xs = np.random.RandomState(4).randn(33, 4)
+    mf, vf = model.predict_f(xs)
+    rm, rv = O.sdgpr_predict(p, Ms, 300, xs)
+    assert nrel(mf, rm) < TOL and nrel(vf, rv) < TOL
+
+
+@pytest.mark.parametrize('nat', [True, False])
+def test_sgplvm_parity(nat):
+    from geepee_amd import aep_models as G
+    say('sgplvm parity nat=%s' % nat)
+    y, p = W.sgplvm_problem(N=150, Q=3, D=4, M=12, seed=3)
+    if not nat:
+        # factor_x2 = exp(2 x2) is the posterior variance when nat_param is False
+        p = dict(p)
+        p['x1'] = p['x1'] * 0.1
+        p['x2'] = np.log(np.random.RandomState(9).uniform(0.05, 0.5, p['x2'].shape)) / 2
+    model = G.SGPLVM(y, 3, 12, nat_param=nat)
+    got = model.objective_function(p, 150, alpha=0.5)
+    want = O.sgplvm_objective(y, p, 12, 150, alpha=0.5, nat_param=nat)
+    compare(got, want)
+    np.random.seed(21)
+    got = model.objective_function(p, 40, alpha=0.5)
+    np.random.seed(21)
+    want = O.sgplvm_objective(y, p, 12, 40, alpha=0.5, nat_param=nat)
+    compare(got, want)
+
+
+def test_sgpssm_parity_linear_emission():
+    from geepee_amd import aep_models as G
+    say('sgpssm parity')
+    y, p = W.sgpssm_problem(T=120, Din=2, Dout=3, M=10, seed=4)
+    model = G.SGPSSM(y, 2, 10)
+    got = model.objective_function(p, 500, alpha=0.5)
+    want = O.sgpssm_objective(y, p, 10, 2, 500, alpha=0.5)
+    compare(got, want)
+    np.random.seed(8)
+    got = model.objective_function(p, 40, alpha=0.5)
+    np.random.seed(8)
+    want = O.sgpssm_objective(y, p, 10, 2, 40, alpha=0.5)
+    compare(got, want)
+
+
+def test_sgpssm_parity_gp_emission_with_control():
+    from geepee_amd import aep_models as G
+    say('sgpssm gp-emission parity')
+    rs = np.random.RandomState(12)
+    T, Din, Dout, M, Dc = 60, 2, 2, 8, 1
+    y = rs.randn(T, Dout)
+    u = rs.randn(T, Dc)
+    p = W.layer_params(rs, Din + Dc, Din, M, x=rs.randn(T, Din + Dc), sfx='_dynamic')
+    p.update(W.layer_params(rs, Din + Dc, Dout, M, x=rs.randn(T, Din + Dc), sfx='_emission'))
+    p['x_factor_1'] = rs.randn(T, Din)
+    p['x_factor_2'] = np.log(rs.uniform(2, 4, (T, Din))) / 2
+    p['sn'] = np.array([np.log(0.3)])
+    p['sn_emission'] = np.array([np.log(0.2)])
+    model = G.SGPSSM(y, Din, M, x_control=u, gp_emi=True)
+    got = model.objective_function(p, 500, alpha=0.5)
+    want = O.sgpssm_objective(y, p, M, Din, 500, alpha=0.5, x_control=u, gp_emi=True)
+    compare(got, want)
