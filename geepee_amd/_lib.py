@@ -51,6 +51,10 @@ SIGNATURES = {
     'gpx_ssm_x_grads': (_i, [_i, _i, _i, _i, _p, _p, _d, _d, _d, _p, _p, _i, _p, _p, _p, _p, _i, _p,
                              _p, _p, _p]),
     'gpx_emission_predict': (_i, [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p]),
+    'gpx_serve_tile': (_i, []),
+    'gpx_serve_desc_sizes': (_i, [_p, _p]),
+    'gpx_serve_validate': (_i, [_p]),
+    'gpx_predict_grouped': (_i, [_i, _p, _p, _p, _p, _p, _i, _p]),
     'gpx_psi_qmax': (_i, []),
     'gpx_psi_fwd': (_i, [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p]),
     'gpx_psi_bwd_workspace': (_l, [_i, _i, _i]),
@@ -81,8 +85,45 @@ def load():
     return _lib
 
 
+_probe = None
+
+
+def set_probe(p):
+    """Install (or clear with None) a Probe that brackets one entry point with HIP
+    events on the launch stream."""
+    global _probe
+    _probe = p
+
+
+class Probe(object):
+    """Times every launch of entry point ``target`` with a pair of HIP events recorded on
+    the stream the kernels are launched on."""
+
+    def __init__(self, target):
+        self.target = target
+        self.pairs = []
+
+    def count(self):
+        return len(self.pairs)
+
+    def mean_ms(self):
+        torch.cuda.synchronize()
+        if not self.pairs:
+            return float('nan')
+        return sum(a.elapsed_time(b) for a, b in self.pairs) / len(self.pairs)
+
+
 def call(name, *args):
-    rc = getattr(load(), name)(*args)
+    p = _probe
+    if p is not None and name == p.target:
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        rc = getattr(load(), name)(*args)
+        b.record()
+        p.pairs.append((a, b))
+    else:
+        rc = getattr(load(), name)(*args)
     if rc != 0:
         raise GpxError('%s failed with status %d' % (name, rc))
     return rc

@@ -66,7 +66,8 @@ class Gauss_Layer(object):
         return dops.gauss_tilted(mout, vout, y, self.sn, alpha, scale, want_dmall)
 
     def output_probabilistic(self, mf, vf, alpha=1.0):
-        sn2 = float(np.exp(2.0 * np.asarray(self.sn_host).ravel()[0]))
+        """mf, vf + exp(2 sn) / alpha (lik_layers.py:238-249)."""
+        sn2 = float(np.exp(2.0 * self.sn.detach().cpu().numpy().ravel()[0]))
         out = torch.empty_like(vf)
         ones = torch.ones(1, dtype=F64, device=vf.device)
         return mf, lincomb(out, 1.0, vf, sn2 / alpha, ones)
@@ -98,14 +99,34 @@ class Base_Model(object):
         self._params_host = params
         return self._dp.views
 
-    def _finish(self, energy_dev, grads_dev, params, divide_by):
-        """Zero fixed params, divide by N, return (float, numpy dict) with one D2H copy."""
+    def objective_function(self, params, mb_size, alpha=1.0, prop_mode=PROP_MM):
+        """(energy, grad dict) as numpy, the reference contract (aep_models.py:610, 694,
+        895, 1024)."""
+        _check_mode(prop_mode)
+        energy, grads, div = self._objective(params, mb_size, alpha)
+        self._check_info(self._layers())
+        return self._finish(energy, grads, params, div)
+
+    def objective_function_device(self, params, mb_size, alpha=1.0, prop_mode=PROP_MM):
+        """Device-resident variant: params is a dict of device tensors; returns one flat
+        device vector [energy, grads in sorted-key order] without any host copy."""
+        _check_mode(prop_mode)
+        energy, grads, div = self._objective(params, mb_size, alpha)
+        return self._flat_device(energy, grads, div)
+
+    def _flat_device(self, energy_dev, grads_dev, divide_by):
         keys = sorted(grads_dev)
         for p in self.fixed_params:
             grads_dev[p] = torch.zeros_like(grads_dev[p])
         flat = torch.cat([energy_dev.reshape(-1)] + [grads_dev[k].reshape(-1) for k in keys])
         if divide_by is not None:
             lincomb(flat, 1.0 / divide_by, flat)
+        return flat
+
+    def _finish(self, energy_dev, grads_dev, params, divide_by):
+        """Zero fixed params, divide by N, return (float, numpy dict) with one D2H copy."""
+        keys = sorted(grads_dev)
+        flat = self._flat_device(energy_dev, grads_dev, divide_by)
         out = flat.cpu().numpy()
         energy = float(out[0])
         grads = {}
@@ -116,6 +137,13 @@ class Base_Model(object):
             grads[k] = out[o:o + n].reshape(shape)
             o += n
         return energy, grads
+
+    def _layers(self):
+        out = []
+        for name in ('sgp_layer', 'dyn_layer', 'emi_layer'):
+            if isinstance(getattr(self, name, None), SGP_Layer):
+                out.append(getattr(self, name))
+        return out + list(getattr(self, 'sgp_layers', []))
 
     def _check_info(self, layers):
         bad = []
@@ -218,18 +246,16 @@ class SGPR(Base_Model):
         dp = self._device_params(params)
         self.sgp_layer.update_hypers(dp)
         self.lik_layer.update_hypers(dp)
-        self.lik_layer.sn_host = np.asarray(params['sn']) if not isinstance(params['sn'], torch.Tensor) \
-            else params['sn'].detach().cpu().numpy()
         return dp
 
-    def objective_function(self, params, mb_size, alpha=1.0, prop_mode=PROP_MM):
-        _check_mode(prop_mode)
+    def _objective(self, params, mb_size, alpha, idx=None):
         N = self.N
-        if mb_size >= N:
+        if idx is None and mb_size < N:
+            idx = torch.as_tensor(np.random.choice(N, mb_size, replace=False), device='cuda')
+        if idx is None:
             xb, yb = self.x_dev, self.y_dev
         else:
-            idxs = torch.as_tensor(np.random.choice(N, mb_size, replace=False), device='cuda')
-            xb, yb = self.x_dev.index_select(0, idxs), self.y_dev.index_select(0, idxs)
+            xb, yb = self.x_dev.index_select(0, idx), self.y_dev.index_select(0, idx)
         B = yb.shape[0]
         scale_logZ = -N * 1.0 / B / alpha
         self.update_hypers(params)
@@ -241,8 +267,7 @@ class SGPR(Base_Model):
         grads['sn'] = scal[1:2]
         phi = layer.compute_phi(alpha)
         energy = lincomb(torch.empty(1, dtype=F64, device='cuda'), scale_logZ, scal[0:1], 1.0, phi)
-        self._check_info([layer])
-        return self._finish(energy, grads, params, N)
+        return energy, grads, N
 
     def predict_f(self, inputs):
         if not self.updated:
@@ -287,18 +312,16 @@ class SDGPR(Base_Model):
         for i, layer in enumerate(self.sgp_layers):
             layer.update_hypers(dp, key_suffix='_%d' % i)
         self.lik_layer.update_hypers(dp)
-        self.lik_layer.sn_host = np.asarray(params['sn']) if not isinstance(params['sn'], torch.Tensor) \
-            else params['sn'].detach().cpu().numpy()
         return dp
 
-    def objective_function(self, params, mb_size, alpha=1.0, prop_mode=PROP_MM):
-        _check_mode(prop_mode)
+    def _objective(self, params, mb_size, alpha, idx=None):
         N = self.N
-        if mb_size >= N:
+        if idx is None and mb_size < N:
+            idx = torch.as_tensor(np.random.choice(N, mb_size, replace=False), device='cuda')
+        if idx is None:
             xb, yb = self.x_dev, self.y_dev
         else:
-            idxs = torch.as_tensor(np.random.choice(N, mb_size, replace=False), device='cuda')
-            xb, yb = self.x_dev.index_select(0, idxs), self.y_dev.index_select(0, idxs)
+            xb, yb = self.x_dev.index_select(0, idx), self.y_dev.index_select(0, idx)
         B = yb.shape[0]
         scale_logZ = -N * 1.0 / B / alpha
         self.update_hypers(params)
@@ -331,8 +354,7 @@ class SDGPR(Base_Model):
         energy = lincomb(torch.empty(1, dtype=F64, device='cuda'), scale_logZ, scal[0:1])
         for layer in self.sgp_layers:
             lincomb(energy, 1.0, energy, 1.0, layer.compute_phi(alpha))
-        self._check_info(self.sgp_layers)
-        return self._finish(energy, grads, params, N)
+        return energy, grads, N
 
     def predict_f(self, inputs):
         x = torch.as_tensor(np.asarray(inputs, dtype=np.float64), device='cuda')
@@ -378,25 +400,20 @@ class SGPLVM(Base_Model):
         dp = self._device_params(params)
         self.sgp_layer.update_hypers(dp)
         self.lik_layer.update_hypers(dp)
-        self.lik_layer.sn_host = np.asarray(params['sn']) if not isinstance(params['sn'], torch.Tensor) \
-            else params['sn'].detach().cpu().numpy()
         return dp
 
-    def objective_function(self, params, mb_size, alpha=1.0, prop_mode=PROP_MM):
-        _check_mode(prop_mode)
+    def _objective(self, params, mb_size, alpha, idx=None):
         N = self.N
-        if mb_size == N:
-            idx_np = None
-            yb = self.y_dev
-        else:
-            idx_np = np.random.choice(N, mb_size, replace=False)
-            yb = self.y_dev.index_select(0, torch.as_tensor(idx_np, device='cuda'))
+        if idx is None and mb_size != N:
+            idx = torch.as_tensor(np.random.choice(N, mb_size, replace=False), dtype=torch.int64,
+                                  device='cuda')
+        yb = self.y_dev if idx is None else self.y_dev.index_select(0, idx)
         B = yb.shape[0]
         scale_logZ = -N * 1.0 / B / alpha
         dp = self.update_hypers(params)
         layer = self.sgp_layer
         layer.compute_cavity(alpha)
-        xs = self._xf.lvm_x_state(dp['x1'], dp['x2'], idx_np, alpha, self.prior_mean,
+        xs = self._xf.lvm_x_state(dp['x1'], dp['x2'], idx, alpha, self.prior_mean,
                                   self.prior_var, self.nat_param)
         mout, vout, psi1, psi2 = layer.forward_prop_thru_cav(xs['mcav'], xs['vcav'])
         dm, dv, dma, scal = self.lik_layer.compute_log_Z(mout, vout, yb, alpha, scale_logZ,
@@ -412,8 +429,7 @@ class SGPLVM(Base_Model):
         grads['x2'] = gx2
         energy = lincomb(torch.empty(1, dtype=F64, device='cuda'), scale_logZ, scal[0:1], 1.0,
                          xcontrib, 1.0, layer.compute_phi(alpha))
-        self._check_info([layer])
-        return self._finish(energy, grads, params, None)
+        return energy, grads, None
 
     def predict_f(self, inputs):
         x = torch.as_tensor(np.asarray(inputs, dtype=np.float64), device='cuda')
@@ -494,8 +510,9 @@ class SGPSSM(Base_Model):
         u = self.u_dev[s:e]
         return (torch.cat([m, u], dim=1), torch.cat([v, torch.zeros_like(u)], dim=1))
 
-    def objective_function(self, params, mb_size, alpha=1.0, prop_mode=PROP_MM):
-        _check_mode(prop_mode)
+    def _objective(self, params, mb_size, alpha, idx=None):
+        if idx is not None:
+            raise NotImplementedError('SGPSSM draws contiguous windows, not index sets')
         N, Din = self.N, self.Din
         if mb_size >= N:
             s, e = 0, N
@@ -555,8 +572,7 @@ class SGPSSM(Base_Model):
         grads['x_factor_1'] = g1
         grads['x_factor_2'] = g2
         lincomb(energy, 1.0, energy, 1.0, xc)
-        self._check_info([dyn] + ([self.emi_layer] if self.gp_emi else []))
-        return self._finish(energy, grads, params, N)
+        return energy, grads, N
 
     @staticmethod
     def _xf():

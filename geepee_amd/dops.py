@@ -4,7 +4,7 @@ on the host or falls back to torch arithmetic."""
 import torch
 
 from . import _lib
-from ._lib import P, S, call, query
+from ._lib import P, S, call, query, Probe, set_probe  # noqa: F401
 
 F64 = torch.float64
 

@@ -91,7 +91,7 @@ def sdgpr_problem(N=20000, widths=(10, 5, 5, 1), M=100, seed=2):
             params.update(layer_params(rs, widths[0], widths[1], M, x=x, sfx='_0'))
         else:
             params.update(layer_params(rs, widths[i], widths[i + 1], M, sfx='_%d' % i,
-                                       ls=np.log(2.0), sf=np.log(1.0)))
+                                       ls=np.log(1.0), sf=np.log(1.0)))
     params['sn'] = np.array([np.log(0.05)])
     hidden = list(widths[1:-1])
     return x, y, params, Ms, hidden
@@ -137,7 +137,7 @@ def serving_models(n_models=64, seed=4, Ms=(32, 64, 128, 256)):
         else:
             H = min(Din, 5)
             p = layer_params(rs, Din, H, M, x=x, sfx='_0')
-            p.update(layer_params(rs, H, 1, M, sfx='_1', ls=np.log(2.0), sf=np.log(1.0)))
+            p.update(layer_params(rs, H, 1, M, sfx='_1', ls=np.log(1.0), sf=np.log(1.0)))
             p['sn'] = np.array([np.log(0.1)])
             models.append(dict(kind='sdgpr', Din=Din, Ms=[M, M], N=Ntr, params=p, hidden=[H]))
     return models

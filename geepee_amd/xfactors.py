@@ -7,10 +7,10 @@ from ._lib import P, S, call
 F64 = torch.float64
 
 
-def lvm_x_state(x1, x2, idx_np, alpha, prior_mean, prior_var, nat_param):
-    """Cavity inputs of the minibatch rows (aep_models.py:840-861)."""
+def lvm_x_state(x1, x2, idx, alpha, prior_mean, prior_var, nat_param):
+    """Cavity inputs of the minibatch rows idx (an int64 device tensor, or None for all
+    rows) (aep_models.py:840-861)."""
     N, Q = x1.shape
-    idx = None if idx_np is None else torch.as_tensor(idx_np, dtype=torch.int64, device=x1.device)
     B = N if idx is None else idx.numel()
     mcav = torch.empty(B, Q, dtype=F64, device=x1.device)
     vcav = torch.empty(B, Q, dtype=F64, device=x1.device)

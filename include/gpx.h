@@ -180,6 +180,21 @@ int gpx_emission_tilted(int Nb, int Dout, int Din, const double *C, const double
 int gpx_emission_predict(int Nb, int Dout, int Din, const double *C, const double *Rp, const double *mf,
                          const double *vf, double *my, double *vy, void *stream);
 
+/* ---- serving ------------------------------------------------------------------ */
+
+/* One launch of predictive mean / variance for many (request, model) work items of at
+ * most gpx_serve_tile() points each, against resident models given by a device table of
+ * GpxServeModel descriptors (layout in geepee_amd/csrc/gpx_serve.hip; sizes from
+ * gpx_serve_desc_sizes).  psi2 is recomputed per point, never stored; per-point results do
+ * not depend on the other points in the launch.  noisy adds exp(2 sn) (predict_y).
+ * Replaces Base_SGPR.predict_f / predict_y and Base_SDGPR.predict_f_mm / predict_y
+ * (geepee/base_models.py:985-1034, 1140-1158, 1264-1275) over many models at once. */
+int gpx_serve_tile(void);
+int gpx_serve_desc_sizes(int *model_bytes, int *item_bytes);
+int gpx_serve_validate(const void *model);
+int gpx_predict_grouped(int nitems, const void *models, const void *items, const double *X,
+                        double *mean, double *var, int noisy, void *stream);
+
 /* ---- latent-variable (x-factor) terms --------------------------------------- */
 
 /* SGPLVM cavity inputs of rows idx (NULL = all rows): get_cavity_x

@@ -37,7 +37,21 @@ JITTER = 1e-5          # geepee/config.py:11
 PROP_MM = 'MM'         # geepee/config.py:13
 
 FAITHFUL = False
+# 'lu': np.linalg.inv as the reference calls it.  'chol': a Cholesky-based inverse,
+# used only to measure how much the oracle's own outputs move when a different (equally
+# valid) inversion algorithm is used -- the conditioning noise floor of a problem.
+INV = 'lu'
 _HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _inv(A):
+    if INV == 'lu':
+        return np.linalg.inv(A)
+    from scipy.linalg import cho_factor, cho_solve
+    A = np.asarray(A)
+    if A.ndim == 2:
+        return cho_solve(cho_factor(A, lower=True), np.eye(A.shape[0]))
+    return np.stack([_inv(a) for a in A])
 
 
 def ein(spec, *ops):
@@ -270,14 +284,14 @@ def layer_update(params, sfx, M, N, nat_param=True):
     L.theta1 = ein('dba,dbc->dac', L.R, L.R)
     L.theta2 = np.array(params['eta2' + sfx], dtype=np.float64)
     L.Kuu = se_kernel(L.ls, L.sf, L.zu, L.zu) + np.diag(JITTER * np.ones(M))
-    L.Kuuinv = np.linalg.inv(L.Kuu)
+    L.Kuuinv = _inv(L.Kuu)
     if nat_param:
         L.Suinv = L.Kuuinv + L.theta1
-        L.Su = np.linalg.inv(L.Suinv)
+        L.Su = _inv(L.Suinv)
         L.mu = ein('dab,db->da', L.Su, L.theta2)
     else:
         L.Su = L.theta1
-        L.Suinv = np.linalg.inv(L.Su)
+        L.Suinv = _inv(L.Su)
         L.mu = L.theta2
     L.Splusmm = L.Su + ein('da,db->dab', L.mu, L.mu)
     L.A = ein('ab,db->da', L.Kuuinv, L.mu)
@@ -292,13 +306,13 @@ def layer_cavity(L, alpha):
     Kuuinv = L.Kuuinv
     if L.nat_param:
         L.Suhatinv = Kuuinv + beta * L.theta1
-        L.Suhat = np.linalg.inv(L.Suhatinv)
+        L.Suhat = _inv(L.Suhatinv)
         L.muhat = ein('dab,db->da', L.Suhat, beta * L.theta2)
     else:
         data_f_1 = L.Suinv - Kuuinv
         data_f_2 = ein('dab,db->da', L.Suinv, L.mu)
         L.Suhatinv = Kuuinv + beta * data_f_1
-        L.Suhat = np.linalg.inv(L.Suhatinv)
+        L.Suhat = _inv(L.Suhatinv)
         L.muhat = ein('dab,db->da', L.Suhat, beta * data_f_2)
     L.Ahat = ein('ab,db->da', Kuuinv, L.muhat)
     L.Splusmmhat = L.Suhat + ein('da,db->dab', L.muhat, L.muhat)
@@ -519,7 +533,7 @@ def emission_tilted(C, Rpar, y, mx, vx, alpha, scale):
     const_term = -Nb * Dout * 0.5 * alpha * np.log(2 * np.pi)
     Rlogdet_term = -0.5 * Nb * alpha * np.sum(np.log(R))
     logZ = const_term + Rlogdet_term + Vlogdet_term + quad_term
-    Vyinv = np.linalg.inv(Vy)
+    Vyinv = _inv(Vy)
     dR = (-0.5 * np.sum(np.diagonal(Vyinv, axis1=1, axis2=2), axis=0)
           + 0.5 * np.sum(VinvY ** 2, axis=0)) / alpha
     dR += 0.5 * Nb * (1 - alpha) / R
