@@ -28,10 +28,33 @@ FP64_PEAK_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
 WORKLOADS = {
     'sgplvm': dict(desc='SGPLVM uncertain-input AEP objective, N=4096 latent Gaussians '
                         'N(0,1)/U(0.01,0.5), Din=8, Dout=32, M=128, alpha=0.5, full batch',
-                   N=4096, Q=8, D=32, M=128),
-    'sdgpr': dict(desc='Deep GP SDGPR 10->5->5->1, N=20000, M=100/layer, minibatch 1000, alpha=0.5',
-                  N=20000, M=100, B=1000),
+                   N=4096, Q=8, D=32, M=128, mb=4096),
+    'sdgpr': dict(desc='Deep GP SDGPR 10->5->5->1, N=20000 inputs N(0,I), M=100 per layer, '
+                       'minibatch 1000, alpha=0.5', N=20000, M=100, mb=1000),
+    'sgpssm': dict(desc='GPSSM latent dim 4, T=10000, Dout=8 linear-Gaussian emission, M=200, '
+                        'alpha=0.5, full batch', T=10000, Q=4, D=8, M=200, mb=10000),
+    'sgpr': dict(desc='SGPR N=1000 x~U(-3,3), y=sin(3x)+N(0,0.01), M=50, alpha=0.5, full batch',
+                 N=1000, M=50, mb=1000),
 }
+
+
+def build_model(name, w):
+    """(model, params) of one evaluation workload, seeded."""
+    from geepee_amd import aep_models as G
+    from geepee_amd import workloads as W
+    if name == 'sgplvm':
+        y, p = W.sgplvm_problem(N=w['N'], Q=w['Q'], D=w['D'], M=w['M'], seed=1)
+        return G.SGPLVM(y, w['Q'], w['M']), p
+    if name == 'sdgpr':
+        x, y, p, Ms, hidden = W.sdgpr_problem(N=w['N'], widths=(10, 5, 5, 1), M=w['M'], seed=2)
+        return G.SDGPR(x, y, Ms, hidden), p
+    if name == 'sgpssm':
+        y, p = W.sgpssm_problem(T=w['T'], Din=w['Q'], Dout=w['D'], M=w['M'], seed=3)
+        return G.SGPSSM(y, w['Q'], w['M']), p
+    if name == 'sgpr':
+        x, y, p = W.sgpr_problem(N=w['N'], M=w['M'], seed=0)
+        return G.SGPR(x, y, w['M']), p
+    raise ValueError(name)
 
 
 def _cpu_info():
@@ -73,39 +96,139 @@ def cpu_baseline_sgplvm(w, budget_s=40.0):
                        % (times[512], times[1024], w['N'], t_full, _cpu_info(), np.__version__))}
 
 
-def run_sgplvm(args, w):
+def mm_layers(name, w):
+    """(N_rows, M, Q, D) of every moment-matched (uncertain-input) layer evaluated per step."""
+    if name == 'sgplvm':
+        return [(w['N'], w['M'], w['Q'], w['D'])]
+    if name == 'sdgpr':
+        return [(w['mb'], w['M'], 5, 5), (w['mb'], w['M'], 5, 1)]
+    if name == 'sgpssm':
+        return [(w['T'] - 1, w['M'], w['Q'], w['Q'])]
+    return []
+
+
+def roofline_model(entry, name, w):
+    """Algorithmic work of one launch of `entry` summed over the workload's MM layers:
+    ('hbm', bytes) for the materialising psi forward, ('mfma', flops) for the fused ones."""
+    tot, launches = 0.0, 0
+    for (N, M, Q, D) in mm_layers(name, w):
+        launches += 1
+        if entry == 'gpx_psi_fwd':
+            # psi1 (N x M) and psi2 (N x M x M) written once; mx, vx (N x Q) and z (M x Q) read once
+            tot += 8.0 * (N * M * M + N * M + 2 * N * Q + M * Q)
+        elif entry == 'gpx_fused_mm_fwd':
+            # per point and unordered pair: the K=2Q+2 exponent product and the D-long
+            # contraction with Bhat (2 flops per multiply-add); exp counted as one flop
+            pairs = M * (M + 1) / 2.0
+            tot += N * pairs * (2.0 * (2 * Q + 2) + 1.0 + 2.0 * D)
+        elif entry == 'gpx_fused_mm_bwd':
+            # both passes recompute the exponents; per-point pass contracts Bhat with dv (2D)
+            # and accumulates T and V (2 + 2Q); per-pair pass forms G, dBhat and LW (4D + 2(Q+1))
+            pairs = M * (M + 1) / 2.0
+            K = 2 * Q + 2
+            tot += N * pairs * (2 * (2.0 * K + 1.0) + 2.0 * D + 2.0 + 2.0 * Q + 4.0 * D + 2.0 * (Q + 1))
+    bound = 'hbm' if entry == 'gpx_psi_fwd' else 'mfma'
+    return bound, (tot / launches if launches else 0.0)
+
+
+def run_evals(args, name, w):
     import torch
-    from geepee_amd import aep_models as G
-    from geepee_amd import dops, workloads as W
-    N, Q, D, M = w['N'], w['Q'], w['D'], w['M']
-    y, p = W.sgplvm_problem(N=N, Q=Q, D=D, M=M, seed=1)
-    model = G.SGPLVM(y, Q, M)
-    dp = model._device_params(p)
+    from geepee_amd import dops
+    from geepee_amd.graphs import GraphedObjective
+    model, p = build_model(name, w)
+    if args.path in ('fused', 'graph'):
+        model.set_fused()
+    mb, alpha = w['mb'], 0.5
+    np.random.seed(0)
+    if args.path == 'graph':
+        g = GraphedObjective(model, p, mb, alpha)
+        step = g
+    else:
+        dp = model._device_params(p)
+        step = lambda: model.objective_function_device(dp, mb, alpha=alpha)
     for _ in range(args.warmup):
-        model.objective_function_device(dp, N, alpha=0.5)
+        step()
     torch.cuda.synchronize()
-    probe = dops.Probe(args.probe) if args.probe else None
+    probe = dops.Probe(args.probe) if (args.probe and args.path != 'graph') else None
     dops.set_probe(probe)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        out = model.objective_function_device(dp, N, alpha=0.5)
+        out = step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     dops.set_probe(None)
     dt = (t1 - t0) / args.steps
     res = {'value': 1.0 / dt, 'ms_per_step': dt * 1e3}
-    if probe is not None:
+    if probe is not None and probe.count():
         ms = probe.mean_ms()
-        # gpx_psi_fwd writes psi1 (N x M) and psi2 (N x M x M) and reads the N x Q inputs
-        bytes_alg = 8.0 * (N * M * M + N * M + 2 * N * Q + M * Q)
-        achieved = bytes_alg / (ms * 1e-3) / 1e9
-        res['roofline'] = {'bound': 'hbm', 'kernel': 'psi2_kernel (gpx_psi_fwd)',
-                           'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                           'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
+        bound, work = roofline_model(args.probe, name, w)
+        if bound == 'hbm':
+            achieved, peak, unit = work / (ms * 1e-3) / 1e9, HBM_PEAK_GBS, 'GB/s'
+        else:
+            achieved, peak, unit = work / (ms * 1e-3) / 1e12, FP64_PEAK_TFLOPS, 'TFLOP/s'
+        res['roofline'] = {'bound': bound, 'kernel': args.probe, 'achieved': achieved, 'peak': peak,
+                           'unit': unit, 'frac': achieved / peak, 'traffic': None,
                            'avg_launch_ms': ms, 'launches': probe.count(),
-                           'bytes_per_launch': bytes_alg}
+                           ('bytes_per_launch' if bound == 'hbm' else 'flops_per_launch'): work}
     _ = out
     return res
+
+
+def run_serving(args):
+    """C5: 64 resident models, Poisson arrivals, log-uniform batch sizes 1..4096.
+    Continuous batching (two launch slots, HBM budget) against each request alone on
+    the per-op plain path, both replaying the same arrival times."""
+    import torch
+    from geepee_amd import workloads as W
+    from geepee_amd.serving import ModelStore, PredictServer, make_requests
+    specs = W.serving_models(n_models=64, seed=4)
+    store = ModelStore(specs)
+    dins = [s['Din'] for s in specs]
+    warm = make_requests(64, 64, rate=1e9, dins=dins, seed=99)
+    server = PredictServer(store)
+    server.run(warm)
+    server.run_plain(warm[:16])
+    torch.cuda.synchronize()
+    out = {}
+    for rate in args.rates:
+        reqs = make_requests(64, args.requests, rate=rate, dins=dins, seed=int(rate))
+        lat, _, wall = server.run(reqs)
+        pts = sum(r[2].shape[0] for r in reqs)
+        out['%g' % rate] = {'batched': {'qps': len(reqs) / wall, 'points_per_s': pts / wall,
+                                        'p50_ms': 1e3 * float(np.percentile(lat, 50)),
+                                        'p99_ms': 1e3 * float(np.percentile(lat, 99))}}
+        nplain = min(len(reqs), args.plain_requests)
+        latp, _, wallp = server.run_plain(reqs[:nplain])
+        out['%g' % rate]['plain'] = {'qps': nplain / wallp, 'requests': nplain,
+                                     'p50_ms': 1e3 * float(np.percentile(latp, 50)),
+                                     'p99_ms': 1e3 * float(np.percentile(latp, 99))}
+        print('rate %g: %s' % (rate, json.dumps(out['%g' % rate])), file=sys.stderr, flush=True)
+    top = out['%g' % args.rates[-1]]
+    return {'metric': 'served predict queries/s (p99 latency beside)', 'value': top['batched']['qps'],
+            'unit': 'queries/s', 'ms_per_step': top['batched']['p99_ms'], 'rates': out,
+            'workload': '64 resident SGPR/2-layer SDGPR models, M in {32,64,128,256}, Din 1-10; '
+                        'Poisson arrivals, %d requests per rate, batch log-uniform 1-4096, N(0,I) points'
+                        % args.requests}
+
+
+def run_copy(args):
+    """Calibration for HBM counters: gpx_lincomb as a plain copy (out = 1 * x) of
+    64 Mi doubles, i.e. 512 MiB read and 512 MiB written per launch."""
+    import torch
+    from geepee_amd import dops
+    n = 64 << 20
+    x = torch.ones(n, dtype=torch.float64, device='cuda')
+    out = torch.empty_like(x)
+    for _ in range(args.warmup):
+        dops.lincomb(out, 1.0, x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dops.lincomb(out, 1.0, x)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    return {'metric': 'copy GB/s (counter calibration)', 'value': 16.0 * n / dt / 1e9, 'unit': 'GB/s',
+            'bytes_read_per_launch': 8 * n, 'bytes_written_per_launch': 8 * n, 'ms_per_step': dt * 1e3}
 
 
 def main():
@@ -113,16 +236,35 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=10)
     ap.add_argument('--warmup', type=int, default=2)
-    ap.add_argument('--workload', default='sgplvm', choices=sorted(WORKLOADS))
-    ap.add_argument('--probe', default='gpx_psi_fwd')
+    ap.add_argument('--workload', default='sgplvm', choices=sorted(WORKLOADS) + ['serving', 'copy'])
+    ap.add_argument('--path', default='plain', choices=['plain', 'fused', 'graph'])
+    ap.add_argument('--probe', default=None,
+                    help='C-ABI entry point timed for the roofline (default: the path\'s dominant one)')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--rates', type=float, nargs='+', default=[1e2, 1e3, 1e4])
+    ap.add_argument('--requests', type=int, default=2000)
+    ap.add_argument('--plain-requests', type=int, default=500)
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU only: this build runs the hot path on a single MI355X '
               '(the multi-GPU sharding is a design note); rerun with --gpus 1', file=sys.stderr)
         sys.exit(2)
+    if args.workload == 'copy':
+        print(json.dumps(run_copy(args)), flush=True)
+        return
+    if args.workload == 'serving':
+        r = run_serving(args)
+        print(json.dumps({'metric': r['metric'], 'value': r['value'], 'unit': r['unit'], 'n_gpus': 1,
+                          'steps': args.requests, 'warmup': 64, 'ms_per_step': r['ms_per_step'],
+                          'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+                          'dtype': 'f64', 'data': 'synthetic (seeded)',
+                          'config': {'workload': r['workload'], 'path': 'continuous batching'},
+                          'rates': r['rates']}), flush=True)
+        return
     w = WORKLOADS[args.workload]
-    res = run_sgplvm(args, w)
+    if args.probe is None:
+        args.probe = 'gpx_psi_fwd' if args.path == 'plain' else 'gpx_fused_mm_bwd'
+    res = run_evals(args, args.workload, w)
     line = {
         'metric': 'AEP energy+grad evals/s',
         'value': res['value'],
@@ -136,11 +278,11 @@ def main():
         'vs_baseline': None,
         'dtype': 'f64',
         'data': 'synthetic (seeded)',
-        'config': {'workload': w['desc'], 'path': 'plain', 'params': 'device-resident'},
+        'config': {'workload': w['desc'], 'path': args.path, 'params': 'device-resident'},
     }
     if 'roofline' in res:
         line['roofline'] = res['roofline']
-    if not args.no_cpu:
+    if not args.no_cpu and args.workload == 'sgplvm':
         line['cpu_baseline'] = cpu_baseline_sgplvm(w)
     print(json.dumps(line), flush=True)
 

@@ -55,6 +55,15 @@ SIGNATURES = {
     'gpx_serve_desc_sizes': (_i, [_p, _p]),
     'gpx_serve_validate': (_i, [_p]),
     'gpx_predict_grouped': (_i, [_i, _p, _p, _p, _p, _p, _i, _p]),
+    'gpx_sym_pairmajor': (_i, [_i, _i, _p, _p, _p]),
+    'gpx_adam_step': (_i, [_l, _p, _p, _p, _p, _d, _d, _d, _d, _i, _p, _p, _p]),
+    'gpx_fused_fwd_workspace': (_l, [_i, _i, _i]),
+    'gpx_fused_mm_fwd': (_i, [_i, _i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p, _p, _p]),
+    'gpx_fused_bwd_workspace': (_l, [_i, _i, _i, _i, _i, _i]),
+    'gpx_fused_mm_bwd': (_i, [_i, _i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p,
+                              _p, _p, _p, _p, _p]),
+    'gpx_psi_bwd_tail': (_i, [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p,
+                              _p, _p]),
     'gpx_psi_qmax': (_i, []),
     'gpx_psi_fwd': (_i, [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p]),
     'gpx_psi_bwd_workspace': (_l, [_i, _i, _i]),
@@ -162,7 +171,7 @@ def gemm(tA, tB, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch
          float(beta), P(C), ldc, sC, batch, splits, P(work), tile, S())
 
 
-def mm(A, B, C, tA=False, tB=False, alpha=1.0, beta=0.0, splits=1, work=None):
+def mm(A, B, C, tA=False, tB=False, alpha=1.0, beta=0.0, splits=1, work=None, tile=0):
     """C = alpha op(A) op(B) + beta C for 2-D or batched 3-D contiguous tensors; a 2-D
     operand is broadcast over the batch of a 3-D one."""
     batch = max(t.shape[0] if t.dim() == 3 else 1 for t in (A, B, C))
@@ -179,7 +188,7 @@ def mm(A, B, C, tA=False, tB=False, alpha=1.0, beta=0.0, splits=1, work=None):
     sC = m * n if C.dim() == 3 else 0
     if splits > 1 and work is None:
         work = torch.empty(splits * batch * m * n, dtype=torch.float64, device=C.device)
-    gemm(tA, tB, m, n, k, alpha, A, ac, sA, B, bc, sB, beta, C, n, sC, batch, splits, work)
+    gemm(tA, tB, m, n, k, alpha, A, ac, sA, B, bc, sB, beta, C, n, sC, batch, splits, work, tile)
     return C
 
 

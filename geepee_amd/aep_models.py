@@ -138,6 +138,12 @@ class Base_Model(object):
             o += n
         return energy, grads
 
+    def set_fused(self, fused=True):
+        """Moment-matched layers recompute psi2 inside fused kernels instead of writing it."""
+        for L in self._layers():
+            L.fused = fused
+        return self
+
     def _layers(self):
         out = []
         for name in ('sgp_layer', 'dyn_layer', 'emi_layer'):
@@ -193,6 +199,40 @@ class Base_Model(object):
         if return_cost and method.lower() == 'adam':
             return final_params, costs
         return final_params
+
+    def optimise_device(self, params=None, maxiter=1000, adam_lr=0.001, alpha=0.5, mb_size=None,
+                        graph=True, b1=0.9, b2=0.999, eps=1e-8):
+        """Adam (utils.py:93-117) with the parameter vector, both moments and the gradient
+        resident on the device: per iteration one objective evaluation (a hipGraph
+        replay when graph=True) and one gpx_adam_step launch; minibatches are drawn
+        with numpy's legacy RNG in the reference's order.  Starts from `params` (default:
+        the current hyper-parameters), updates the model, and returns the final params
+        dict and the objective value of every iteration (one device-to-host copy)."""
+        from ._lib import P, S, call
+        from .graphs import GraphedObjective
+        params = self.get_hypers() if params is None else params
+        mb = self.N if mb_size is None else mb_size
+        if graph:
+            g = GraphedObjective(self, params, mb, alpha)
+            dp, step = g.dp, g
+        else:
+            dp = DeviceParams(_shapes(params))
+            dp.upload(params)
+            step = lambda: self.objective_function_device(dp.views, mb, alpha=alpha)
+        n = dp.total
+        mom = torch.zeros(2, n, dtype=F64, device=dp.buf.device)
+        costs = torch.empty(max(maxiter, 1), dtype=F64, device=dp.buf.device)
+        for i in range(maxiter):
+            out = step()
+            call('gpx_adam_step', n, P(dp.buf), P(out[1:]), P(mom[0]), P(mom[1]), float(adam_lr),
+                 float(b1), float(b2), float(eps), i + 1, P(out), P(costs[i:]), S())
+        vec = dp.buf.cpu().numpy()
+        final, o = {}, 0
+        for k, sz in zip(dp.keys, dp.sizes):
+            final[k] = vec[o:o + sz].reshape(dp.shapes[k]).copy()
+            o += sz
+        self.update_hypers(final)
+        return final, costs[:maxiter].cpu().numpy()
 
     def set_fixed_params(self, params):
         if isinstance(params, (list, tuple)):

@@ -1,0 +1,715 @@
+// Fused moment-matched (uncertain-input) layer: psi2 is never written to HBM.
+//
+// The psi2 exponent is a K = 2Q+2 inner product of per-point and per-pair features,
+//   log psi2[n,a,b] - 4 sf = c_n + P_ab + sum_q [ (2 d_nq mu_nq) zh_abq - d_nq zh_abq^2 ]
+// with d = 1/(2v+l), zh = (z_a+z_b)/2, c_n = sum_q [log(l/(l+2v))/2 - d mu^2] and
+// P_ab = -sum_q (z_a-z_b)^2/(4l) (the expansion of geepee/kernels.py:219-223), so each
+// 16 (pairs) x 16 (points) block of exponents is a short FP64 MFMA GEMM, followed by one
+// exp per entry on the vector ALUs, and the result feeds the next MFMA contraction in the
+// register layout the matrix core produced it in.
+//
+// Pairs are visited as 16x16 tiles (A <= B) of the M x M square; a tile is processed in
+// 16 "row blocks" (fixed a, 16 consecutive b).  Off-diagonal tiles count each unordered
+// pair once with weight 2; diagonal tiles hold the full square with weight 1.
+//
+// Forward (aep_models.py:182-199): v[n,d] = sf2 + sum_ab Bhat[d,a,b] psi2[n,a,b] - m^2.
+// Backward (aep_models.py:201-304 with kernels.py:302-444): per point the statistics
+// T_na = sum_b L_nab and V_nq = sum_ab L_nab z_aq z_bq of L = dpsi2 * psi2, and over the
+// points dBhat[d] = sum_n dv_nd psi2_n, Lsum = sum_n L_n, W_q = sum_n L_n d_nq, where
+// dpsi2_n = sum_d dv_nd Bsym[d].
+#include "gpx_common.h"
+
+#define FQ_MAX 16
+#define FM_MAX 256
+#define FKS_MAX 9  // ceil((2*FQ_MAX+2)/4)
+
+namespace {
+
+__device__ __forceinline__ void decode_tile(int t, int T, int &A, int &B) {
+  A = 0;
+  int rem = t;
+  while (rem >= T - A) {
+    rem -= T - A;
+    ++A;
+  }
+  B = A + rem;
+}
+
+// Per-point exponent feature k (B operand of the exponent GEMM).
+__device__ __forceinline__ double point_feature(int k, int Q, const double *mu, const double *d, double c) {
+  if (k < Q) return 2.0 * d[k] * mu[k];
+  if (k < 2 * Q) return d[k - Q];
+  if (k == 2 * Q) return 1.0;
+  if (k == 2 * Q + 1) return c;
+  return 0.0;
+}
+
+// Per-pair exponent feature k (A operand of the exponent GEMM); zs rows in LDS.
+__device__ __forceinline__ double pair_feature(int k, int Q, const double *za, const double *zb,
+                                               const double *il4) {
+  if (k < Q) return 0.5 * (za[k] + zb[k]);
+  if (k < 2 * Q) {
+    const double h = 0.5 * (za[k - Q] + zb[k - Q]);
+    return -h * h;
+  }
+  if (k == 2 * Q) {
+    double p = 0.0;
+    for (int q = 0; q < Q; ++q) {
+      const double dz = za[q] - zb[q];
+      p -= dz * dz * il4[q];
+    }
+    return p;
+  }
+  if (k == 2 * Q + 1) return 1.0;
+  return 0.0;
+}
+
+// Per-point constants: mu, d = 1/(2v+l), c = sum [log(l/(l+2v))/2 - d mu^2].
+__device__ __forceinline__ double point_consts(int n, int N, int Q, const double *mx, const double *vx,
+                                               const double *lam, double *mu, double *d) {
+  double c = 0.0;
+  for (int q = 0; q < Q; ++q) {
+    const double m = (n < N) ? mx[(long)n * Q + q] : 0.0;
+    const double v = (n < N) ? vx[(long)n * Q + q] : 0.0;
+    const double dd = 1.0 / (2.0 * v + lam[q]);
+    mu[q] = m;
+    d[q] = dd;
+    c += 0.5 * log(lam[q] / (lam[q] + 2.0 * v)) - dd * m * m;
+  }
+  return c;
+}
+
+// ---------------------------------------------------------------------------
+// forward: part[s][n][d] = sum over the tiles of split s of w * Bp[a,b,d] psi2[n,a,b]
+// One block = NSUB x 16 points; the Bp loads of a row block are shared by the NSUB
+// point tiles.  Bp is the symmetrised Bhat in pair-major layout (M x M x D) so the D
+// values of a pair are contiguous.
+// ---------------------------------------------------------------------------
+template <int DB, int NSUB>
+__global__ __launch_bounds__(256) void fused_fwd_kernel(int N, int M, int Q, int D,
+                                                        const double *__restrict__ mx,
+                                                        const double *__restrict__ vx,
+                                                        const double *__restrict__ z,
+                                                        const double *__restrict__ ls,
+                                                        const double *__restrict__ sf,
+                                                        const double *__restrict__ Bp, int S,
+                                                        double *__restrict__ part) {
+  __shared__ double zs[FM_MAX][FQ_MAX + 1];
+  __shared__ double lam[FQ_MAX], il4[FQ_MAX];
+  __shared__ double red[4][DB * 16][17];
+  const int n0 = blockIdx.x * 16 * NSUB, split = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int e = tid; e < M * Q; e += 256) zs[e / Q][e % Q] = z[e];
+  if (tid < Q) {
+    const double l = exp(2.0 * ls[tid]);
+    lam[tid] = l;
+    il4[tid] = 0.25 / l;
+  }
+  __syncthreads();
+  const int KS = (2 * Q + 2 + 3) / 4;
+  double bf[NSUB][FKS_MAX];
+  bool nvalid[NSUB];
+#pragma unroll
+  for (int u = 0; u < NSUB; ++u) {
+    const int n = n0 + 16 * u + (lane & 15);
+    nvalid[u] = n < N;
+    double mu[FQ_MAX], dq[FQ_MAX];
+    const double c = point_consts(n, N, Q, mx, vx, lam, mu, dq);
+#pragma unroll
+    for (int s = 0; s < FKS_MAX; ++s)
+      bf[u][s] = (s < KS) ? point_feature((lane >> 4) + 4 * s, Q, mu, dq, c) : 0.0;
+  }
+  const double sf2 = exp(2.0 * sf[0]), sf4 = sf2 * sf2;
+  gpx_d4 acc[NSUB][DB];
+#pragma unroll
+  for (int u = 0; u < NSUB; ++u)
+#pragma unroll
+    for (int i = 0; i < DB; ++i) acc[u][i] = gpx_d4{0.0, 0.0, 0.0, 0.0};
+  const int T = (M + 15) / 16, ntiles = T * (T + 1) / 2;
+  for (int t = split; t < ntiles; t += S) {
+    int A, B;
+    decode_tile(t, T, A, B);
+    const double w = (A == B) ? sf4 : 2.0 * sf4;
+    for (int i = wave; i < 16; i += 4) {
+      const int a = 16 * A + i;
+      if (a >= M) break;
+      const int ja = 16 * B + (lane & 15);
+      double af[FKS_MAX];
+#pragma unroll
+      for (int s = 0; s < FKS_MAX; ++s)
+        af[s] = (s < KS && ja < M) ? pair_feature((lane >> 4) + 4 * s, Q, zs[a], zs[ja], il4) : 0.0;
+      double bv[DB][4];
+#pragma unroll
+      for (int db = 0; db < DB; ++db)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int d = 16 * db + (lane & 15), b = 16 * B + (lane >> 4) + 4 * r;
+          bv[db][r] = (d < D && b < M) ? Bp[((long)a * M + b) * D + d] : 0.0;
+        }
+#pragma unroll
+      for (int u = 0; u < NSUB; ++u) {
+        gpx_d4 e = gpx_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < FKS_MAX; ++s)
+          if (s < KS) e = gpx_mfma16(af[s], bf[u][s], e);
+        double p[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int b = 16 * B + (lane >> 4) + 4 * r;
+          p[r] = (b < M && nvalid[u]) ? w * exp(e[r]) : 0.0;
+        }
+#pragma unroll
+        for (int db = 0; db < DB; ++db)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[u][db] = gpx_mfma16(bv[db][r], p[r], acc[u][db]);
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < NSUB; ++u) {
+#pragma unroll
+    for (int db = 0; db < DB; ++db)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) red[wave][db * 16 + (lane >> 4) + 4 * r][lane & 15] = acc[u][db][r];
+    __syncthreads();
+    for (int idx = tid; idx < DB * 16 * 16; idx += 256) {
+      const int dd = idx >> 4, nn = idx & 15, n = n0 + 16 * u + nn;
+      const double s = red[0][dd][nn] + red[1][dd][nn] + red[2][dd][nn] + red[3][dd][nn];
+      if (dd < D && n < N) part[((long)split * N + n) * D + dd] = s;
+    }
+    __syncthreads();
+  }
+}
+
+// out[(a*M + b)*D + d] = (B[d,a,b] + B[d,b,a]) / 2
+__global__ void sym_pairmajor_kernel(int D, int M, const double *__restrict__ B, double *__restrict__ out) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long MM = (long)M * M;
+  if (e >= D * MM) return;
+  const long ab = e / D;
+  const int d = (int)(e % D), a = (int)(ab / M), b = (int)(ab % M);
+  out[e] = 0.5 * (B[d * MM + (long)a * M + b] + B[d * MM + (long)b * M + a]);
+}
+
+
+// ---------------------------------------------------------------------------
+// backward, per-point pass ([pair x n] orientation): for 16 points per block and the
+// tiles of split s, T_na (row sums + column sums of off-diagonal tiles) and V_nq.
+// Each inducing index a is owned by wave (a % 16) % 4 for row sums; column sums are
+// merged in wave order after each tile, so every sum has a fixed order.
+// ---------------------------------------------------------------------------
+template <int DS>
+__global__ __launch_bounds__(256) void fused_bwd_points_kernel(
+    int N, int M, int Q, int D, const double *__restrict__ mx, const double *__restrict__ vx,
+    const double *__restrict__ z, const double *__restrict__ ls, const double *__restrict__ sf,
+    const double *__restrict__ Bp, const double *__restrict__ dv, int S,
+    double *__restrict__ Tpart, double *__restrict__ Vpart) {
+  __shared__ double zs[FM_MAX][FQ_MAX + 1];
+  __shared__ double lam[FQ_MAX], il4[FQ_MAX];
+  __shared__ double Tacc[16][FM_MAX + 1];
+  __shared__ double vred[4][16][17];
+  const int n0 = blockIdx.x * 16, split = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int e = tid; e < M * Q; e += 256) zs[e / Q][e % Q] = z[e];
+  for (int e = tid; e < 16 * (FM_MAX + 1); e += 256) Tacc[e / (FM_MAX + 1)][e % (FM_MAX + 1)] = 0.0;
+  if (tid < Q) {
+    const double l = exp(2.0 * ls[tid]);
+    lam[tid] = l;
+    il4[tid] = 0.25 / l;
+  }
+  __syncthreads();
+  const int n = n0 + (lane & 15);
+  const bool nvalid = n < N;
+  const int KS = (2 * Q + 2 + 3) / 4;
+  double bf[FKS_MAX];
+  {
+    double mu[FQ_MAX], dq[FQ_MAX];
+    const double c = point_consts(n, N, Q, mx, vx, lam, mu, dq);
+#pragma unroll
+    for (int s = 0; s < FKS_MAX; ++s) bf[s] = (s < KS) ? point_feature((lane >> 4) + 4 * s, Q, mu, dq, c) : 0.0;
+  }
+  // dv[n][d] for d = (lane>>4) + 4s  (B operand of the G contraction)
+  double dvr[DS];
+#pragma unroll
+  for (int s = 0; s < DS; ++s) {
+    const int d = (lane >> 4) + 4 * s;
+    dvr[s] = (nvalid && d < D) ? dv[(long)n * D + d] : 0.0;
+  }
+  const double sf2 = exp(2.0 * sf[0]), sf4 = sf2 * sf2;
+  gpx_d4 vacc = gpx_d4{0.0, 0.0, 0.0, 0.0};
+  const int T = (M + 15) / 16, ntiles = T * (T + 1) / 2;
+  for (int t = split; t < ntiles; t += S) {
+    int A, B;
+    decode_tile(t, T, A, B);
+    const bool off = (A != B);
+    const double w = off ? 2.0 : 1.0;
+    double colp[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = wave; i < 16; i += 4) {
+      const int a = 16 * A + i;
+      if (a >= M) break;
+      const int ja = 16 * B + (lane & 15);
+      gpx_d4 e = gpx_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < FKS_MAX; ++s) {
+        if (s < KS) {
+          const double af = (ja < M) ? pair_feature((lane >> 4) + 4 * s, Q, zs[a], zs[ja], il4) : 0.0;
+          e = gpx_mfma16(af, bf[s], e);
+        }
+      }
+      gpx_d4 g = gpx_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < DS; ++s) {
+        const int d = (lane >> 4) + 4 * s;
+        const double bsv = (ja < M && d < D) ? Bp[((long)a * M + ja) * D + d] : 0.0;
+        g = gpx_mfma16(bsv, dvr[s], g);
+      }
+      double L[4];
+      double rs = 0.0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int b = 16 * B + (lane >> 4) + 4 * r;
+        const double p = (b < M && nvalid) ? sf4 * exp(e[r]) : 0.0;
+        L[r] = g[r] * p;
+        rs += L[r];
+        colp[r] += L[r];
+      }
+      rs += __shfl_xor(rs, 16, 64);
+      rs += __shfl_xor(rs, 32, 64);
+      if ((lane >> 4) == 0) Tacc[lane & 15][a] += rs;
+      // V[q x n] += w z_aq z_bq L[pair x n]
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int q = lane & 15, b = 16 * B + (lane >> 4) + 4 * r;
+        const double zz = (q < Q && b < M) ? w * zs[a][q] * zs[b][q] : 0.0;
+        vacc = gpx_mfma16(zz, L[r], vacc);
+      }
+    }
+    if (off) {
+      for (int ww = 0; ww < 4; ++ww) {
+        __syncthreads();
+        if (wave == ww) {
+          // column sums of this wave's rows: each lane holds distinct (b, n) entries
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int b = 16 * B + (lane >> 4) + 4 * r;
+            if (b < M) Tacc[lane & 15][b] += colp[r];
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // V: lane holds V[q = (lane>>4) + 4r][n = lane & 15]
+#pragma unroll
+  for (int r = 0; r < 4; ++r) vred[wave][(lane >> 4) + 4 * r][lane & 15] = vacc[r];
+  __syncthreads();
+  for (int idx = tid; idx < 16 * 16; idx += 256) {
+    const int q = idx >> 4, nn = idx & 15;
+    if (q < Q && n0 + nn < N)
+      Vpart[((long)split * N + n0 + nn) * Q + q] = vred[0][q][nn] + vred[1][q][nn] + vred[2][q][nn] + vred[3][q][nn];
+  }
+  for (int e = tid; e < 16 * M; e += 256) {
+    const int nn = e / M, a = e % M;
+    if (n0 + nn < N) Tpart[((long)split * N + n0 + nn) * M + a] = Tacc[nn][a];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// backward, per-pair pass ([n x pair] orientation): each wave owns one row block
+// (a, B-tile) and sums over the points of its range:
+//   dBhat[d, a, b] += sum_n dv_nd psi2_nab,  LW[a, b, q'] += sum_n [1 | d_nq'] L_nab
+// ---------------------------------------------------------------------------
+template <int DB, int DS>
+__global__ __launch_bounds__(256) void fused_bwd_pairs_kernel(
+    int N, int M, int Q, int D, const double *__restrict__ mx, const double *__restrict__ vx,
+    const double *__restrict__ z, const double *__restrict__ ls, const double *__restrict__ sf,
+    const double *__restrict__ Bp, const double *__restrict__ dv, int nsplit, int nchunk,
+    double *__restrict__ dBpart, double *__restrict__ LWpart) {
+  __shared__ double zs[FM_MAX][FQ_MAX + 1];
+  __shared__ double lam[FQ_MAX], il4[FQ_MAX];
+  __shared__ double pf[16][4 * FKS_MAX + 1];  // point features
+  __shared__ double dvs[16][DB * 16 + 1];
+  __shared__ double dq1[16][FQ_MAX + 2];      // [1 | d]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int T = (M + 15) / 16;
+  const int rb = blockIdx.x * 4 + wave;       // row block id: tile * 16 + i
+  const int t = rb / 16, i = rb % 16;
+  int A, B;
+  decode_tile(t, T, A, B);
+  const int a = 16 * A + i;
+  const bool active = a < M;
+  const int ns = blockIdx.y;
+  const int nbeg = ns * nchunk, nend = min(N, nbeg + nchunk);
+  for (int e = tid; e < M * Q; e += 256) zs[e / Q][e % Q] = z[e];
+  if (tid < Q) {
+    const double l = exp(2.0 * ls[tid]);
+    lam[tid] = l;
+    il4[tid] = 0.25 / l;
+  }
+  __syncthreads();
+  const int KS = (2 * Q + 2 + 3) / 4;
+  const int jb = 16 * B + (lane & 15);       // this lane's pair column b
+  const bool bvalid = active && jb < M;
+  double af[FKS_MAX], bs[DS];
+#pragma unroll
+  for (int s = 0; s < FKS_MAX; ++s)
+    af[s] = (s < KS && bvalid) ? pair_feature((lane >> 4) + 4 * s, Q, zs[active ? a : 0], zs[bvalid ? jb : 0], il4) : 0.0;
+#pragma unroll
+  for (int s = 0; s < DS; ++s) {
+    const int d = (lane >> 4) + 4 * s;
+    bs[s] = (bvalid && d < D) ? Bp[((long)a * M + jb) * D + d] : 0.0;
+  }
+  const double sf2 = exp(2.0 * sf[0]), sf4 = sf2 * sf2;
+  gpx_d4 dacc[DB], lacc = gpx_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int db = 0; db < DB; ++db) dacc[db] = gpx_d4{0.0, 0.0, 0.0, 0.0};
+  for (int n0 = nbeg; n0 < nend; n0 += 16) {
+    // stage point data of 16 points
+    if (tid < 16) {
+      const int n = n0 + tid;
+      double mu[FQ_MAX], dd[FQ_MAX];
+      const double c = point_consts(n, N, Q, mx, vx, lam, mu, dd);
+      for (int k = 0; k < 4 * FKS_MAX; ++k) pf[tid][k] = (n < nend) ? point_feature(k, Q, mu, dd, c) : 0.0;
+      dq1[tid][0] = (n < nend) ? 1.0 : 0.0;
+      for (int q = 0; q < Q; ++q) dq1[tid][1 + q] = (n < nend) ? dd[q] : 0.0;
+      for (int q = Q + 1; q < FQ_MAX + 2; ++q) dq1[tid][q] = 0.0;
+    }
+    for (int e = tid; e < 16 * DB * 16; e += 256) {
+      const int nn = e / (DB * 16), d = e % (DB * 16), n = n0 + nn;
+      dvs[nn][d] = (n < nend && d < D) ? dv[(long)n * D + d] : 0.0;
+    }
+    __syncthreads();
+    if (active) {
+      gpx_d4 e = gpx_d4{0.0, 0.0, 0.0, 0.0}, g = gpx_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < FKS_MAX; ++s)
+        if (s < KS) e = gpx_mfma16(pf[lane & 15][(lane >> 4) + 4 * s], af[s], e);
+#pragma unroll
+      for (int s = 0; s < DS; ++s) g = gpx_mfma16(dvs[lane & 15][(lane >> 4) + 4 * s], bs[s], g);
+      double p[4], L[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int nn = (lane >> 4) + 4 * r;
+        p[r] = (n0 + nn < nend && bvalid) ? sf4 * exp(e[r]) : 0.0;
+        L[r] = g[r] * p[r];
+      }
+#pragma unroll
+      for (int db = 0; db < DB; ++db)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          dacc[db] = gpx_mfma16(dvs[(lane >> 4) + 4 * r][16 * db + (lane & 15)], p[r], dacc[db]);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) lacc = gpx_mfma16(dq1[(lane >> 4) + 4 * r][lane & 15], L[r], lacc);
+    }
+    __syncthreads();
+  }
+  if (!active) return;
+  // dacc[db]: lane holds dBhat[d = 16db + (lane>>4) + 4r][pair b = jb]
+#pragma unroll
+  for (int db = 0; db < DB; ++db)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int d = 16 * db + (lane >> 4) + 4 * r;
+      if (d < D && jb < M) dBpart[(((long)ns * D + d) * M + a) * M + jb] = dacc[db][r];
+    }
+  // lacc: lane holds LW[q' = (lane>>4) + 4r][pair b = jb]
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int qq = (lane >> 4) + 4 * r;
+    if (qq <= Q && jb < M) LWpart[(((long)ns * M + a) * M + jb) * (Q + 1) + qq] = lacc[r];
+  }
+}
+
+// Sum the per-range partials and mirror the off-diagonal tiles:
+// dBhat (D x M x M) and LW ((M*M) x (Q+1)) in the layouts of the materialising path.
+__global__ void fused_pairs_finish(int M, int Q, int D, int nsplit, const double *__restrict__ dBpart,
+                                   const double *__restrict__ LWpart, double *__restrict__ dBhat,
+                                   double *__restrict__ LW) {
+  const long MM = (long)M * M;
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long W = (long)D + Q + 1;
+  if (e >= MM * W) return;
+  const long ab = e / W;
+  const int k = (int)(e % W);
+  int a = (int)(ab / M), b = (int)(ab % M);
+  // the tiles cover blocks A <= B; read the stored half
+  const int sa = (a / 16 <= b / 16) ? a : b, sb = (a / 16 <= b / 16) ? b : a;
+  double s = 0.0;
+  if (k < D) {
+    for (int p = 0; p < nsplit; ++p) s += dBpart[(((long)p * D + k) * M + sa) * M + sb];
+    dBhat[(long)k * MM + (long)a * M + b] = s;
+  } else {
+    const int q = k - D;
+    for (int p = 0; p < nsplit; ++p) s += LWpart[(((long)p * M + sa) * M + sb) * (Q + 1) + q];
+    LW[((long)a * M + b) * (Q + 1) + q] = s;
+  }
+}
+
+
+// Per-point finish of the fused adjoint: T = sum_s Tpart, V = sum_s Vpart, Lp = dpsi1 psi1,
+// then the same per-point formulas (and outputs) as the materialising path's row pass.
+constexpr int FRED = 4 * FQ_MAX + 2;
+__global__ __launch_bounds__(256) void fused_rows_finish(
+    int N, int M, int Q, int S, const double *__restrict__ mx, const double *__restrict__ vx,
+    const double *__restrict__ z, const double *__restrict__ ls, const double *__restrict__ Tpart,
+    const double *__restrict__ Vpart, const double *__restrict__ dpsi1, const double *__restrict__ psi1,
+    double *__restrict__ dmx, double *__restrict__ dvx, double *__restrict__ Tout,
+    double *__restrict__ Lpout, double *__restrict__ Hout, double *__restrict__ s0out,
+    double *__restrict__ Dn, double *__restrict__ Pn, double *__restrict__ En) {
+  __shared__ double red[4][FRED];
+  __shared__ double tot[FRED];
+  const int n = blockIdx.x, b = threadIdx.x, lane = b & 63, w = b >> 6;
+  double mu[FQ_MAX];
+#pragma unroll
+  for (int q = 0; q < FQ_MAX; ++q) mu[q] = (q < Q) ? mx[(long)n * Q + q] : 0.0;
+  double T = 0.0, Lp = 0.0;
+  if (b < M) {
+    for (int s = 0; s < S; ++s) T += Tpart[((long)s * N + n) * M + b];
+    Lp = dpsi1[(long)n * M + b] * psi1[(long)n * M + b];
+    Tout[(long)n * M + b] = T;
+    Lpout[(long)n * M + b] = Lp;
+  }
+  // [0] S0, [1] sum Lp, [2+q] S1, [2+Q+q] sum T z^2, [2+2Q+q] B1, [2+3Q+q] C1
+  double c[FRED];
+#pragma unroll
+  for (int i = 0; i < FRED; ++i) c[i] = 0.0;
+  c[0] = T;
+  c[1] = Lp;
+  if (b < M) {
+#pragma unroll
+    for (int q = 0; q < FQ_MAX; ++q)
+      if (q < Q) {
+        const double zq = z[b * Q + q], zm = zq - mu[q];
+        c[2 + q] = T * zq;
+        c[2 + FQ_MAX + q] = T * zq * zq;
+        c[2 + 2 * FQ_MAX + q] = Lp * zm;
+        c[2 + 3 * FQ_MAX + q] = Lp * zm * zm;
+      }
+  }
+#pragma unroll
+  for (int i = 0; i < FRED; ++i) {
+    if (i >= 2 && (i - 2) % FQ_MAX >= Q) continue;
+    const double s = gpx_wave_sum(c[i]);
+    if (lane == 0) red[w][i] = s;
+  }
+  __syncthreads();
+  for (int i = b; i < FRED; i += 256) tot[i] = red[0][i] + red[1][i] + red[2][i] + red[3][i];
+  __syncthreads();
+  if (b < Q) {
+    const int q = b;
+    double V = 0.0;
+    for (int s = 0; s < S; ++s) V += Vpart[((long)s * N + n) * Q + q];
+    const double S0 = tot[0], A1 = tot[1];
+    const double S1 = tot[2 + q], TZ2 = tot[2 + FQ_MAX + q];
+    const double B1 = tot[2 + 2 * FQ_MAX + q], C1 = tot[2 + 3 * FQ_MAX + q];
+    const double l = exp(2.0 * ls[q]);
+    const double m = mx[(long)n * Q + q], vv = vx[(long)n * Q + q];
+    const double ell = exp(ls[q]);
+    const double d = 1.0 / (2.0 * vv + l), e = 1.0 / (vv + l);
+    const double S2 = 0.5 * (TZ2 + V);
+    const double quad = m * m * S0 - 2.0 * m * S1 + S2;
+    dmx[(long)n * Q + q] = -2.0 * d * (m * S0 - S1) + e * B1;
+    dvx[(long)n * Q + q] = 2.0 * d * d * quad - d * S0 + 0.5 * e * (e * C1 - A1);
+    Hout[(long)n * Q + q] = e * ell * (e * C1 + (vv / l) * A1) +
+                            2.0 * ell * ((vv / l) * d * S0 + d * d * quad + (TZ2 - V) / (2.0 * l * l));
+    Dn[(long)n * (Q + 1) + 1 + q] = d;
+    Pn[(long)n * 2 * Q + q] = d * m;
+    Pn[(long)n * 2 * Q + Q + q] = d;
+    En[(long)n * 2 * Q + q] = e;
+    En[(long)n * 2 * Q + Q + q] = e * m;
+  }
+  if (b == 0) {
+    Dn[(long)n * (Q + 1)] = 1.0;
+    s0out[2 * n] = tot[0];
+    s0out[2 * n + 1] = tot[1];
+  }
+}
+
+// psi1, m = psi1 Ahat^T and v = sf2 + sum_s part - m^2 for 16 points per block.
+__global__ __launch_bounds__(256) void fused_fwd_finish(int N, int M, int Q, int D, const double *__restrict__ mx,
+                                                        const double *__restrict__ vx,
+                                                        const double *__restrict__ z,
+                                                        const double *__restrict__ ls,
+                                                        const double *__restrict__ sf,
+                                                        const double *__restrict__ Ahat,
+                                                        const double *__restrict__ part, int S,
+                                                        double *__restrict__ psi1, double *__restrict__ mout,
+                                                        double *__restrict__ vout) {
+  __shared__ double p1[16][FM_MAX + 1];
+  const int n0 = blockIdx.x * 16, tid = threadIdx.x;
+  const double sf2 = exp(2.0 * sf[0]);
+  for (int e = tid; e < 16 * M; e += 256) {
+    const int nn = e / M, m = e % M, n = n0 + nn;
+    double val = 0.0;
+    if (n < N) {
+      double acc = 0.0, lc = 0.0;
+      for (int q = 0; q < Q; ++q) {
+        const double l = exp(2.0 * ls[q]);
+        const double v = vx[(long)n * Q + q];
+        const double dlt = mx[(long)n * Q + q] - z[m * Q + q];
+        acc += -dlt * dlt / (2.0 * (v + l));
+        lc += 0.5 * log(l / (l + v));
+      }
+      val = sf2 * exp(acc + lc);
+      psi1[(long)n * M + m] = val;
+    }
+    p1[nn][m] = val;
+  }
+  __syncthreads();
+  for (int e = tid; e < 16 * D; e += 256) {
+    const int nn = e / D, d = e % D, n = n0 + nn;
+    if (n >= N) continue;
+    double m = 0.0;
+    for (int k = 0; k < M; ++k) m += p1[nn][k] * Ahat[(long)d * M + k];
+    double s = 0.0;
+    for (int sp = 0; sp < S; ++sp) s += part[((long)sp * N + n) * D + d];
+    mout[(long)n * D + d] = m;
+    vout[(long)n * D + d] = sf2 + s - m * m;
+  }
+}
+
+}  // namespace
+
+extern "C" int gpx_gemm_batched(int transA, int transB, int m, int n, int k, double alpha,
+                                const double *A, int lda, long strideA, const double *B, int ldb,
+                                long strideB, double beta, double *C, int ldc, long strideC,
+                                int batch, int splits, double *work, int tile, void *stream);
+extern "C" int gpx_psi_bwd_tail(int N, int M, int Q, const double *z, const double *ls, const double *sf,
+                                const double *T, const double *Lp, const double *H, const double *s0,
+                                const double *Dn, const double *Pn, const double *En, const double *LW,
+                                double *TP, double *LE, double *cols0, double *dsf2, double *dell, double *dZ,
+                                void *stream);
+
+extern "C" {
+
+long gpx_fused_bwd_workspace(int N, int M, int Q, int D, int splits, int nsplit) {
+  const long MM = (long)M * M;
+  long s = (long)nsplit * D * MM + (long)nsplit * MM * (Q + 1) + MM * (Q + 1);  // dBpart, LWpart, LW
+  s += (long)splits * N * M + (long)splits * N * Q;                           // Tpart, Vpart
+  s += 3L * N * M + (long)N * Q + 2L * N;                                     // dpsi1, T, Lp, H, s0
+  s += (long)N * (Q + 1) + 4L * N * Q + 4L * M * Q + 2;                       // Dn, Pn, En, TP, LE, cols0
+  return s;
+}
+
+// Adjoint of the moment-matched layer without materialising psi2 or dL/dpsi2:
+// given dm_all = dm - 2 dv m and dv (N x D), Ahat (D x M), Bp (pair-major symmetrised
+// Bhat, M x M x D) and psi1 from the forward: dBhat (D x M x M), dsf2, dell, dZ, dmx, dvx.
+// dL/dpsi2 = sum_d dv_nd Bsym[d] is symmetric by construction (kernels.py:415-418).
+int gpx_fused_mm_bwd(int N, int M, int Q, int D, const double *mx, const double *vx, const double *z,
+                     const double *ls, const double *sf, const double *Ahat, const double *Bp,
+                     const double *psi1, const double *dm_all, const double *dv, int splits, int nsplit,
+                     double *dBhat, double *dsf2, double *dell, double *dZ, double *dmx, double *dvx,
+                     double *work, void *stream) {
+  // Q <= FQ_MAX - 1: the [1 | d_n] contraction fills the Q+1 rows of one 16-row MFMA tile
+  if (N < 1 || M < 1 || M > FM_MAX || Q < 1 || Q > FQ_MAX - 1 || D < 1 || D > 32 || splits < 1 ||
+      nsplit < 1)
+    return GPX_EINVAL;
+  if (!mx || !vx || !z || !ls || !sf || !Ahat || !Bp || !psi1 || !dm_all || !dv || !dBhat || !dsf2 ||
+      !dell || !dZ || !dmx || !dvx || !work)
+    return GPX_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  const int T = (M + 15) / 16, ntiles = T * (T + 1) / 2;
+  if (splits > ntiles) return GPX_EINVAL;
+  const long MM = (long)M * M;
+  double *dBpart = work;
+  double *LWpart = dBpart + (long)nsplit * D * MM;
+  double *LW = LWpart + (long)nsplit * MM * (Q + 1);
+  double *Tpart = LW + MM * (Q + 1);
+  double *Vpart = Tpart + (long)splits * N * M;
+  double *dpsi1 = Vpart + (long)splits * N * Q;
+  double *Tn = dpsi1 + (long)N * M;
+  double *Lp = Tn + (long)N * M;
+  double *H = Lp + (long)N * M;
+  double *s0 = H + (long)N * Q;
+  double *Dn = s0 + 2L * N;
+  double *Pn = Dn + (long)N * (Q + 1);
+  double *En = Pn + 2L * N * Q;
+  double *TP = En + 2L * N * Q;
+  double *LE = TP + 2L * M * Q;
+  double *cols0 = LE + 2L * M * Q;
+  // per-pair pass over point ranges
+  const int nchunk = ((N + nsplit - 1) / nsplit + 15) / 16 * 16;
+  dim3 gp(ntiles * 4, nsplit);
+  if (D <= 16)
+    hipLaunchKernelGGL((fused_bwd_pairs_kernel<1, 4>), gp, dim3(256), 0, st, N, M, Q, D, mx, vx, z, ls, sf, Bp,
+                       dv, nsplit, nchunk, dBpart, LWpart);
+  else
+    hipLaunchKernelGGL((fused_bwd_pairs_kernel<2, 8>), gp, dim3(256), 0, st, N, M, Q, D, mx, vx, z, ls, sf, Bp,
+                       dv, nsplit, nchunk, dBpart, LWpart);
+  GPX_CHECK_LAUNCH();
+  {
+    const long n = MM * (D + Q + 1);
+    hipLaunchKernelGGL(fused_pairs_finish, dim3(gpx_cdiv(n, 256)), dim3(256), 0, st, M, Q, D, nsplit,
+                       (const double *)dBpart, (const double *)LWpart, dBhat, LW);
+    GPX_CHECK_LAUNCH();
+  }
+  // per-point pass over tile splits
+  dim3 gq(gpx_cdiv(N, 16), splits);
+  if (D <= 16)
+    hipLaunchKernelGGL((fused_bwd_points_kernel<4>), gq, dim3(256), 0, st, N, M, Q, D, mx, vx, z, ls, sf, Bp, dv,
+                       splits, Tpart, Vpart);
+  else
+    hipLaunchKernelGGL((fused_bwd_points_kernel<8>), gq, dim3(256), 0, st, N, M, Q, D, mx, vx, z, ls, sf, Bp, dv,
+                       splits, Tpart, Vpart);
+  GPX_CHECK_LAUNCH();
+  // dpsi1 = dm_all Ahat
+  int rc = gpx_gemm_batched(0, 0, N, M, D, 1.0, dm_all, D, 0, Ahat, M, 0, 0.0, dpsi1, M, 0, 1, 1, nullptr, 0,
+                            stream);
+  if (rc) return rc;
+  hipLaunchKernelGGL(fused_rows_finish, dim3(N), dim3(256), 0, st, N, M, Q, splits, mx, vx, z, ls,
+                     (const double *)Tpart, (const double *)Vpart, (const double *)dpsi1, psi1, dmx, dvx, Tn, Lp,
+                     H, s0, Dn, Pn, En);
+  GPX_CHECK_LAUNCH();
+  return gpx_psi_bwd_tail(N, M, Q, z, ls, sf, Tn, Lp, H, s0, Dn, Pn, En, LW, TP, LE, cols0, dsf2, dell, dZ,
+                          stream);
+}
+
+}  // extern "C"
+
+extern "C" {
+
+
+// Symmetrised Bhat in pair-major layout: out[a, b, d] = (B[d,a,b] + B[d,b,a]) / 2.
+int gpx_sym_pairmajor(int D, int M, const double *B, double *out, void *stream) {
+  if (D < 1 || M < 1 || !B || !out || B == out) return GPX_EINVAL;
+  const long n = (long)D * M * M;
+  hipLaunchKernelGGL(sym_pairmajor_kernel, dim3(gpx_cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream, D, M, B, out);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+long gpx_fused_fwd_workspace(int N, int D, int splits) { return (long)splits * N * D; }
+
+// Moment-matched layer forward without materialising psi2:
+// psi1 (N x M), mout = psi1 Ahat^T, vout = sf2 + <Bsym[d], psi2_n> - mout^2 (N x D).
+// Bp is the symmetrised Bhat in pair-major layout (gpx_sym_pairmajor); `splits`
+// spreads the pair tiles over grid.y and `nsub` (1, 2 or 4) sets 16*nsub points per block.
+int gpx_fused_mm_fwd(int N, int M, int Q, int D, const double *mx, const double *vx, const double *z,
+                     const double *ls, const double *sf, const double *Ahat, const double *Bp,
+                     int splits, int nsub, double *psi1, double *mout, double *vout, double *work,
+                     void *stream) {
+  if (N < 1 || M < 1 || M > FM_MAX || Q < 1 || Q > FQ_MAX || D < 1 || D > 32 || splits < 1)
+    return GPX_EINVAL;
+  if (nsub != 1 && nsub != 2 && nsub != 4) return GPX_EINVAL;
+  if (!mx || !vx || !z || !ls || !sf || !Ahat || !Bp || !psi1 || !mout || !vout || !work)
+    return GPX_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  const int T = (M + 15) / 16, ntiles = T * (T + 1) / 2;
+  if (splits > ntiles) return GPX_EINVAL;
+  dim3 grid(gpx_cdiv(N, 16 * nsub), splits);
+#define GPX_FWD(DBV, NS) hipLaunchKernelGGL((fused_fwd_kernel<DBV, NS>), grid, dim3(256), 0, st, N, M, Q, D, mx, \
+                                            vx, z, ls, sf, Bp, splits, work)
+  if (D <= 16) {
+    if (nsub == 1) GPX_FWD(1, 1); else if (nsub == 2) GPX_FWD(1, 2); else GPX_FWD(1, 4);
+  } else {
+    if (nsub == 1) GPX_FWD(2, 1); else if (nsub == 2) GPX_FWD(2, 2); else GPX_FWD(2, 4);
+  }
+#undef GPX_FWD
+  GPX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(fused_fwd_finish, dim3(gpx_cdiv(N, 16)), dim3(256), 0, st, N, M, Q, D, mx, vx, z, ls, sf,
+                     Ahat, (const double *)work, splits, psi1, mout, vout);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+}  // extern "C"

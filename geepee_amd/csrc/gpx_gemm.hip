@@ -10,8 +10,57 @@
 
 namespace {
 
-constexpr int BK = 16;
+constexpr int BK = 32;
 
+// Stage one BK-slice of op(A) (BM x BK) and op(B) (BK x BN) into registers.
+template <int BM, int BN>
+__device__ __forceinline__ void gemm_load(double *ra, double *rb, const double *__restrict__ A,
+                                          const double *__restrict__ B, int tA, int tB, int lda, int ldb,
+                                          int m, int n, int m0, int n0, int k0, int kend) {
+  constexpr int LA = BM * BK / 256, LB = BN * BK / 256;
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < LA; ++u) {
+    const int e = tid + 256 * u;
+    int mm, kk;
+    if (tA) { kk = e / BM; mm = e % BM; } else { mm = e / BK; kk = e % BK; }
+    const int gm = m0 + mm, gk = k0 + kk;
+    ra[u] = (gm < m && gk < kend) ? (tA ? A[(long)gk * lda + gm] : A[(long)gm * lda + gk]) : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < LB; ++u) {
+    const int e = tid + 256 * u;
+    int nn, kk;
+    if (tB) { nn = e / BK; kk = e % BK; } else { kk = e / BN; nn = e % BN; }
+    const int gn = n0 + nn, gk = k0 + kk;
+    rb[u] = (gn < n && gk < kend) ? (tB ? B[(long)gn * ldb + gk] : B[(long)gk * ldb + gn]) : 0.0;
+  }
+}
+
+template <int BM, int BN>
+__device__ __forceinline__ void gemm_store(const double *ra, const double *rb, double (*As)[BM + 1],
+                                           double (*Bs)[BN + 1], int tA, int tB) {
+  constexpr int LA = BM * BK / 256, LB = BN * BK / 256;
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < LA; ++u) {
+    const int e = tid + 256 * u;
+    int mm, kk;
+    if (tA) { kk = e / BM; mm = e % BM; } else { mm = e / BK; kk = e % BK; }
+    As[kk][mm] = ra[u];
+  }
+#pragma unroll
+  for (int u = 0; u < LB; ++u) {
+    const int e = tid + 256 * u;
+    int nn, kk;
+    if (tB) { nn = e / BK; kk = e % BK; } else { kk = e / BN; nn = e % BN; }
+    Bs[kk][nn] = rb[u];
+  }
+}
+
+// 4 waves as 2x2; each wave owns (BM/2) x (BN/2) of 16x16 MFMA tiles.  K advances in
+// slices of BK through two LDS buffers; the next slice is fetched from global memory into
+// registers while the current one feeds the matrix cores (one barrier per slice).
 template <int BM, int BN>
 __global__ __launch_bounds__(256) void gemm_kernel(
     int m, int n, int k, int tA, int tB, double alpha,
@@ -19,10 +68,10 @@ __global__ __launch_bounds__(256) void gemm_kernel(
     const double *__restrict__ B, int ldb, long sB,
     double beta, double *__restrict__ C, int ldc, long sC,
     int batch, int splits, int kchunk, double *__restrict__ part) {
-  // 4 waves as 2x2; each wave owns a (BM/2) x (BN/2) block of 16x16 MFMA tiles.
   constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
-  __shared__ double As[BK][BM + 1];
-  __shared__ double Bs[BK][BN + 1];
+  constexpr int LA = BM * BK / 256, LB = BN * BK / 256;
+  __shared__ double As[2][BK][BM + 1];
+  __shared__ double Bs[2][BK][BN + 1];
   const int z = blockIdx.z;
   const int bidx = z / splits, sidx = z % splits;
   if (bidx >= batch) return;
@@ -30,46 +79,38 @@ __global__ __launch_bounds__(256) void gemm_kernel(
   B += bidx * sB;
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
   const int kbeg = sidx * kchunk, kend = min(k, kbeg + kchunk);
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   gpx_d4 acc[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = gpx_d4{0.0, 0.0, 0.0, 0.0};
-
+  double ra[LA], rb[LB];
+  int cur = 0;
+  if (kbeg < kend) {
+    gemm_load<BM, BN>(ra, rb, A, B, tA, tB, lda, ldb, m, n, m0, n0, kbeg, kend);
+    gemm_store<BM, BN>(ra, rb, As[0], Bs[0], tA, tB);
+  }
+  __syncthreads();
   for (int k0 = kbeg; k0 < kend; k0 += BK) {
-    // stage A tile as As[kk][mm]
-    for (int e = tid; e < BM * BK; e += 256) {
-      int mm, kk;
-      if (tA) { kk = e / BM; mm = e % BM; } else { mm = e / BK; kk = e % BK; }
-      const int gm = m0 + mm, gk = k0 + kk;
-      double v = 0.0;
-      if (gm < m && gk < kend) v = tA ? A[(long)gk * lda + gm] : A[(long)gm * lda + gk];
-      As[kk][mm] = v;
-    }
-    for (int e = tid; e < BN * BK; e += 256) {
-      int nn, kk;
-      if (tB) { nn = e / BK; kk = e % BK; } else { kk = e / BN; nn = e % BN; }
-      const int gn = n0 + nn, gk = k0 + kk;
-      double v = 0.0;
-      if (gn < n && gk < kend) v = tB ? B[(long)gn * ldb + gk] : B[(long)gk * ldb + gn];
-      Bs[kk][nn] = v;
-    }
-    __syncthreads();
+    const bool more = k0 + BK < kend;
+    if (more) gemm_load<BM, BN>(ra, rb, A, B, tA, tB, lda, ldb, m, n, m0, n0, k0 + BK, kend);
 #pragma unroll
     for (int kq = 0; kq < BK; kq += 4) {
       double a[TM], b[TN];
 #pragma unroll
-      for (int i = 0; i < TM; ++i) a[i] = As[kq + (lane >> 4)][wm * WM + i * 16 + (lane & 15)];
+      for (int i = 0; i < TM; ++i) a[i] = As[cur][kq + (lane >> 4)][wm * WM + i * 16 + (lane & 15)];
 #pragma unroll
-      for (int j = 0; j < TN; ++j) b[j] = Bs[kq + (lane >> 4)][wn * WN + j * 16 + (lane & 15)];
+      for (int j = 0; j < TN; ++j) b[j] = Bs[cur][kq + (lane >> 4)][wn * WN + j * 16 + (lane & 15)];
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[i][j] = gpx_mfma16(a[i], b[j], acc[i][j]);
     }
+    if (more) gemm_store<BM, BN>(ra, rb, As[cur ^ 1], Bs[cur ^ 1], tA, tB);
     __syncthreads();
+    cur ^= 1;
   }
 
   if (part) {

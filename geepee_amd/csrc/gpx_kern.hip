@@ -171,6 +171,17 @@ __global__ void dtrace_final(int M, int Q, const double *__restrict__ part, doub
   if (j == 0) gsf[0] = s; else gls[j - 1] = s;
 }
 
+// One block per column, threads stride over the rows (fixed-order block reduction).
+__global__ __launch_bounds__(256) void colsum_tall_kernel(int rows, int cols, const double *__restrict__ P,
+                                                          double alpha, double beta, double *__restrict__ out) {
+  __shared__ double red[16];
+  const int j = blockIdx.x;
+  double s = 0.0;
+  for (long i = threadIdx.x; i < rows; i += 256) s += P[i * cols + j];
+  s = gpx_block_sum(s, red);
+  if (threadIdx.x == 0) out[j] = (beta == 0.0) ? alpha * s : alpha * s + beta * out[j];
+}
+
 __global__ void colsum_kernel(int rows, int cols, const double *__restrict__ P, double alpha,
                               double beta, double *__restrict__ out) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -200,22 +211,52 @@ __global__ void eta_to_R_kernel(int D, int M, const double *__restrict__ eta, do
   R[e] = v;
 }
 
-// out[d, triu(i,j)] = (sum_k R_ik (G_kj + G_jk)) * (i == j ? R_ii : 1)
-__global__ void pack_dR_kernel(int D, int M, const double *__restrict__ R, const double *__restrict__ G,
-                               double *__restrict__ out) {
-  const long K = (long)M * (M + 1) / 2;
-  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (e >= D * K) return;
-  const int d = (int)(e / K);
-  long p = e % K;
-  int i = 0;
-  while (p >= M - i) { p -= M - i; ++i; }
-  const int j = i + (int)p;
+// out[d, triu(i,j)] = (sum_{k>=i} R_ik (G_kj + G_jk)) * (i == j ? R_ii : 1)
+// One workgroup per (d, 16 rows i, 64 columns j); k advances in slices of 16 staged in
+// LDS, the G_jk half read along k (rows of G) so both halves load coalesced.
+__global__ __launch_bounds__(256) void pack_dR_kernel(int D, int M, const double *__restrict__ R,
+                                                      const double *__restrict__ G, double *__restrict__ out) {
+  __shared__ double Rs[16][17];
+  __shared__ double Ss[16][65];
+  const int d = blockIdx.z, i0 = blockIdx.y * 16, j0 = blockIdx.x * 64;
+  if (j0 + 63 < i0) return;  // tile entirely below the diagonal
+  const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15;
   const double *Rd = R + (long)d * M * M, *Gd = G + (long)d * M * M;
-  double s = 0.0;
-  for (int k = i; k < M; ++k) s += Rd[(long)i * M + k] * (Gd[(long)k * M + j] + Gd[(long)j * M + k]);
-  if (i == j) s *= Rd[(long)i * M + i];
-  out[e] = s;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int k0 = i0; k0 < M; k0 += 16) {
+    {
+      const int r = tid >> 4, kk = tid & 15, i = i0 + r, k = k0 + kk;
+      Rs[r][kk] = (i < M && k < M && k >= i) ? Rd[(long)i * M + k] : 0.0;
+    }
+    for (int e = tid; e < 16 * 64; e += 256) {
+      const int kk = e >> 6, jj = e & 63, k = k0 + kk, j = j0 + jj;
+      Ss[kk][jj] = (k < M && j < M) ? Gd[(long)k * M + j] : 0.0;
+    }
+    __syncthreads();
+    for (int e = tid; e < 16 * 64; e += 256) {
+      const int jj = e >> 4, kk = e & 15, k = k0 + kk, j = j0 + jj;
+      if (k < M && j < M) Ss[kk][jj] += Gd[(long)j * M + k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+      const double r = Rs[ti][kk];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[c] += r * Ss[kk][tj + 16 * c];
+    }
+    __syncthreads();
+  }
+  const int i = i0 + ti;
+  if (i >= M) return;
+  const long K = (long)M * (M + 1) / 2;
+  const long rowoff = (long)i * M - (long)i * (i - 1) / 2;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int j = j0 + tj + 16 * c;
+    if (j < i || j >= M) continue;
+    const double v = (i == j) ? acc[c] * Rd[(long)i * M + i] : acc[c];
+    out[(long)d * K + rowoff + (j - i)] = v;
+  }
 }
 
 __global__ void lincomb_kernel(long n, double a, const double *__restrict__ x, long nx, double b,
@@ -275,20 +316,20 @@ __global__ void batched_dot_kernel(int batch, int n, const double *__restrict__ 
 // scal[0] = logZ (with the alpha constant), scal[1] = dsn (scaled).
 // dm/dv outputs are scaled by `scale`; dmall = dm - 2 dv m when requested.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void gauss_tilted_kernel(int B, int D, const double *__restrict__ m,
-                                                            const double *__restrict__ v,
-                                                            const double *__restrict__ y,
-                                                            const double *__restrict__ sn, double alpha,
-                                                            double scale, double *__restrict__ dm,
-                                                            double *__restrict__ dv,
-                                                            double *__restrict__ dmall,
-                                                            double *__restrict__ scal) {
+__global__ __launch_bounds__(256) void gauss_tilted_kernel(int B, int D, const double *__restrict__ m,
+                                                           const double *__restrict__ v,
+                                                           const double *__restrict__ y,
+                                                           const double *__restrict__ sn, double alpha,
+                                                           double scale, double *__restrict__ dm,
+                                                           double *__restrict__ dv,
+                                                           double *__restrict__ dmall,
+                                                           double *__restrict__ part) {
   __shared__ double red[16];
   const double sn2 = exp(2.0 * sn[0]);
   const double two_pi = 6.283185307179586;
   double lz = 0.0, sdv = 0.0;
   const long n = (long)B * D;
-  for (long e = threadIdx.x; e < n; e += blockDim.x) {
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
     const double vt = v[e] + sn2 / alpha;
     const double r = y[e] - m[e];
     lz += -0.5 * (log(two_pi * vt) + r * r / vt);
@@ -301,10 +342,25 @@ __global__ __launch_bounds__(1024) void gauss_tilted_kernel(int B, int D, const 
   lz = gpx_block_sum(lz, red);
   sdv = gpx_block_sum(sdv, red);
   if (threadIdx.x == 0) {
-    lz += (double)B * D * (0.5 * log(two_pi * sn2 / alpha) - 0.5 * alpha * log(two_pi * sn2));
-    scal[0] = lz;
-    scal[1] = (sdv * 2.0 * sn2 / alpha + (double)B * D * (1.0 - alpha)) * scale;
+    part[2 * blockIdx.x] = lz;
+    part[2 * blockIdx.x + 1] = sdv;
   }
+}
+
+__global__ void gauss_tilted_finish(int nblk, int B, int D, const double *__restrict__ part,
+                                    const double *__restrict__ sn, double alpha, double scale,
+                                    double *__restrict__ scal) {
+  if (threadIdx.x != 0) return;
+  const double sn2 = exp(2.0 * sn[0]);
+  const double two_pi = 6.283185307179586;
+  double lz = 0.0, sdv = 0.0;
+  for (int b = 0; b < nblk; ++b) {
+    lz += part[2 * b];
+    sdv += part[2 * b + 1];
+  }
+  lz += (double)B * D * (0.5 * log(two_pi * sn2 / alpha) - 0.5 * alpha * log(two_pi * sn2));
+  scal[0] = lz;
+  scal[1] = (sdv * 2.0 * sn2 / alpha + (double)B * D * (1.0 - alpha)) * scale;
 }
 
 // SGPSSM.compute_transition_tilted, 2-D branch (aep_models.py:1317-1343).
@@ -619,6 +675,12 @@ int gpx_dtrace_kzz(int M, int Q, const double *Mm, const double *z, const double
 int gpx_colsum(int rows, int cols, const double *P, double alpha, double beta, double *out, void *stream) {
   if (rows < 0 || cols < 0 || !P || !out) return GPX_EINVAL;
   if (cols == 0) return GPX_OK;
+  if (rows >= 512 && cols <= 1024) {
+    hipLaunchKernelGGL(colsum_tall_kernel, dim3(cols), dim3(256), 0, (hipStream_t)stream, rows, cols, P, alpha,
+                       beta, out);
+    GPX_CHECK_LAUNCH();
+    return GPX_OK;
+  }
   hipLaunchKernelGGL(colsum_kernel, dim3(gpx_cdiv(cols, 256)), dim3(256), 0, (hipStream_t)stream, rows, cols,
                      P, alpha, beta, out);
   GPX_CHECK_LAUNCH();
@@ -638,7 +700,9 @@ int gpx_pack_dR(int D, int M, const double *R, const double *G, double *out, voi
   if (D < 0 || M < 1 || !R || !G || !out) return GPX_EINVAL;
   long n = (long)D * M * (M + 1) / 2;
   if (n == 0) return GPX_OK;
-  hipLaunchKernelGGL(pack_dR_kernel, dim3(gpx_cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream, D, M, R, G, out);
+  if (D > 65535) return GPX_EINVAL;
+  hipLaunchKernelGGL(pack_dR_kernel, dim3(gpx_cdiv(M, 64), gpx_cdiv(M, 16), D), dim3(256), 0, (hipStream_t)stream, D,
+                     M, R, G, out);
   GPX_CHECK_LAUNCH();
   return GPX_OK;
 }
@@ -693,12 +757,20 @@ int gpx_batched_dot(int batch, int n, const double *x, const double *y, double *
   return GPX_OK;
 }
 
+// scal needs 2 + 2 * 64 doubles: [logZ, dsn] followed by per-block partials.
 int gpx_gauss_tilted(int B, int D, const double *m, const double *v, const double *y, const double *sn,
                      double alpha, double scale, double *dm, double *dv, double *dmall, double *scal,
                      void *stream) {
   if (B < 1 || D < 1 || !(alpha > 0) || !m || !v || !y || !sn || !dm || !dv || !scal) return GPX_EINVAL;
-  hipLaunchKernelGGL(gauss_tilted_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, B, D, m, v, y, sn, alpha,
-                     scale, dm, dv, dmall, scal);
+  hipStream_t st = (hipStream_t)stream;
+  long n = (long)B * D;
+  int nblk = (int)gpx_cdiv(n, 1024);
+  if (nblk > 64) nblk = 64;
+  hipLaunchKernelGGL(gauss_tilted_kernel, dim3(nblk), dim3(256), 0, st, B, D, m, v, y, sn, alpha, scale, dm, dv,
+                     dmall, scal + 2);
+  GPX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(gauss_tilted_finish, dim3(1), dim3(64), 0, st, nblk, B, D, (const double *)(scal + 2), sn,
+                     alpha, scale, scal);
   GPX_CHECK_LAUNCH();
   return GPX_OK;
 }

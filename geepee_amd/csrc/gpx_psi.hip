@@ -50,20 +50,24 @@ __global__ __launch_bounds__(256) void psi1_kernel(int N, int M, int Q, const do
   psi1[e] = sf2 * exp(acc + lc);
 }
 
-// One block per (point n, tile pair ta >= tb) of 32 x 32 entries.
+// One block per (tile pair ta >= tb of 32 x 32 entries, group of PSI2_PPB points): the
+// z tiles are staged once and reused for every point of the group, the per-point
+// constants are formed in parallel, and an off-diagonal tile is mirrored through LDS
+// (two buffers, one barrier per point) so both triangles are stored as contiguous rows.
+constexpr int PSI2_PPB = 8;
+
 __global__ __launch_bounds__(256) void psi2_kernel(int N, int M, int Q, const double *__restrict__ mx,
                                                     const double *__restrict__ vx,
                                                     const double *__restrict__ z,
                                                     const double *__restrict__ ls,
                                                     const double *__restrict__ sf,
                                                     double *__restrict__ psi2) {
-  __shared__ double za[32][GPX_QMAX], zb[32][GPX_QMAX];
-  __shared__ double mu[GPX_QMAX], dd[GPX_QMAX], i4l[GPX_QMAX];
-  __shared__ double tile[32][33];
-  __shared__ double lc_s;
-  const int n = blockIdx.x;
+  __shared__ double za[32][GPX_QMAX + 1], zb[32][GPX_QMAX + 1];
+  __shared__ double mu[PSI2_PPB][GPX_QMAX], dd[PSI2_PPB][GPX_QMAX], lq[PSI2_PPB][GPX_QMAX];
+  __shared__ double lam[GPX_QMAX], i4l[GPX_QMAX];
+  __shared__ double tile[2][32][33];
   // decode the tile pair index p -> (ta, tb), ta >= tb
-  int p = blockIdx.y, ta = 0;
+  int p = blockIdx.x, ta = 0;
   while (p > ta) { p -= ta + 1; ++ta; }
   const int tb = p;
   const int a0 = ta * 32, b0 = tb * 32, tid = threadIdx.x;
@@ -74,42 +78,58 @@ __global__ __launch_bounds__(256) void psi2_kernel(int N, int M, int Q, const do
   }
   if (tid < Q) {
     const double l = exp(2.0 * ls[tid]);
-    const double v = vx[(long)n * Q + tid];
-    mu[tid] = mx[(long)n * Q + tid];
-    dd[tid] = 1.0 / (2.0 * v + l);
+    lam[tid] = l;
     i4l[tid] = 1.0 / (4.0 * l);
-  }
-  if (tid == 0) {
-    double lc = 0.0;
-    for (int q = 0; q < Q; ++q) {
-      const double l = exp(2.0 * ls[q]);
-      lc += 0.5 * log(l / (l + 2.0 * vx[(long)n * Q + q]));
-    }
-    lc_s = lc;
   }
   __syncthreads();
   const int j = tid & 31;
   const double sf2 = exp(2.0 * sf[0]), sf4 = sf2 * sf2;
-  double *out = psi2 + (long)n * M * M;
-#pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const int i = (tid >> 5) + 8 * s;
-    double e = lc_s;
-    for (int q = 0; q < Q; ++q) {
-      const double x = za[i][q], y = zb[j][q];
-      const double dz = x - y, h = mu[q] - 0.5 * (x + y);
-      e -= dz * dz * i4l[q] + h * h * dd[q];
+  const bool mirror = ta != tb;
+  const int ngroups = (N + PSI2_PPB - 1) / PSI2_PPB;
+  int buf = 0;
+  for (int g = blockIdx.y; g < ngroups; g += gridDim.y) {
+    const int nb = g * PSI2_PPB;
+    if (tid < PSI2_PPB * Q) {
+      const int pt = tid / Q, q = tid % Q, n = nb + pt;
+      if (n < N) {
+        const double v = vx[(long)n * Q + q], l = lam[q];
+        mu[pt][q] = mx[(long)n * Q + q];
+        dd[pt][q] = 1.0 / (2.0 * v + l);
+        lq[pt][q] = 0.5 * log(l / (l + 2.0 * v));
+      }
     }
-    const double val = sf4 * exp(e);
-    tile[i][j] = val;
-    if (a0 + i < M && b0 + j < M) out[(long)(a0 + i) * M + b0 + j] = val;
-  }
-  if (ta == tb) return;
-  __syncthreads();
+    __syncthreads();
+    for (int pt = 0; pt < PSI2_PPB; ++pt) {
+      const int n = nb + pt;
+      if (n >= N) break;
+      double lc = 0.0;
+      for (int q = 0; q < Q; ++q) lc += lq[pt][q];
+      double *out = psi2 + (long)n * M * M;
 #pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const int i = (tid >> 5) + 8 * s;  // row within the mirrored tile (b index)
-    if (b0 + i < M && a0 + j < M) out[(long)(b0 + i) * M + a0 + j] = tile[j][i];
+      for (int s = 0; s < 4; ++s) {
+        const int i = (tid >> 5) + 8 * s;
+        double e = lc;
+        for (int q = 0; q < Q; ++q) {
+          const double x = za[i][q], y = zb[j][q];
+          const double dz = x - y, h = mu[pt][q] - 0.5 * (x + y);
+          e -= dz * dz * i4l[q] + h * h * dd[pt][q];
+        }
+        const double val = sf4 * exp(e);
+        tile[buf][i][j] = val;
+        if (a0 + i < M && b0 + j < M) __builtin_nontemporal_store(val, out + (long)(a0 + i) * M + b0 + j);
+      }
+      if (mirror) {
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const int i = (tid >> 5) + 8 * s;  // row within the mirrored tile (b index)
+          if (b0 + i < M && a0 + j < M)
+            __builtin_nontemporal_store(tile[buf][j][i], out + (long)(b0 + i) * M + a0 + j);
+        }
+        buf ^= 1;
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -281,13 +301,15 @@ __global__ __launch_bounds__(256) void psi_bwd_final_kernel(
   }
 }
 
-// out[j] = sum_i P[i*cols + j] over `rows`, in a fixed order.
-__global__ void colsum_kernel(int rows, int cols, const double *__restrict__ P, double *__restrict__ out) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= cols) return;
+// out[j] = sum_i P[i*cols + j] over `rows`: one block per column, fixed-order reduction.
+__global__ __launch_bounds__(256) void colsum_kernel(int rows, int cols, const double *__restrict__ P,
+                                                     double *__restrict__ out) {
+  __shared__ double red[16];
+  const int j = blockIdx.x;
   double s = 0.0;
-  for (int i = 0; i < rows; ++i) s += P[(long)i * cols + j];
-  out[j] = s;
+  for (long i = threadIdx.x; i < rows; i += 256) s += P[i * cols + j];
+  s = gpx_block_sum(s, red);
+  if (threadIdx.x == 0) out[j] = s;
 }
 
 }  // namespace
@@ -298,6 +320,33 @@ extern "C" int gpx_gemm_batched(int transA, int transB, int m, int n, int k, dou
                                 int batch, int splits, double *work, int tile, void *stream);
 
 extern "C" {
+
+// Shared tail of the psi adjoint (materialising and fused paths): from the per-point
+// outputs T, Lp (N x M), H (N x Q), s0 (N x 2), Dn, Pn, En and the pair sums LW
+// ((M*M) x (Q+1)) finish dsf2, dell (w.r.t. ell) and dZ.  TP, LE (M x 2Q) and cols0 (2)
+// are workspace.
+int gpx_psi_bwd_tail(int N, int M, int Q, const double *z, const double *ls, const double *sf,
+                     const double *T, const double *Lp, const double *H, const double *s0,
+                     const double *Dn, const double *Pn, const double *En, const double *LW,
+                     double *TP, double *LE, double *cols0, double *dsf2, double *dell, double *dZ,
+                     void *stream) {
+  if (N < 1 || M < 1 || M > 256 || Q < 1 || Q > GPX_QMAX) return GPX_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  int rc = gpx_gemm_batched(1, 0, M, 2 * Q, N, 1.0, T, M, 0, Pn, 2 * Q, 0, 0.0, TP, 2 * Q, 0, 1, 1,
+                            nullptr, 0, stream);
+  if (rc) return rc;
+  rc = gpx_gemm_batched(1, 0, M, 2 * Q, N, 1.0, Lp, M, 0, En, 2 * Q, 0, 0.0, LE, 2 * Q, 0, 1, 1,
+                        nullptr, 0, stream);
+  if (rc) return rc;
+  hipLaunchKernelGGL(colsum_kernel, dim3(Q), dim3(256), 0, st, N, Q, H, dell);
+  GPX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(colsum_kernel, dim3(2), dim3(256), 0, st, N, 2, s0, cols0);
+  GPX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(psi_bwd_final_kernel, dim3(M + 1), dim3(256), 0, st, M, Q, z, ls, sf, LW, TP, LE,
+                     (const double *)nullptr, (const double *)cols0, dsf2, dell, dZ);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
 
 int gpx_psi_qmax(void) { return GPX_QMAX; }
 
@@ -314,7 +363,9 @@ int gpx_psi_fwd(int N, int M, int Q, const double *mx, const double *vx, const d
   GPX_CHECK_LAUNCH();
   if (psi2) {
     const int T = (M + 31) / 32;
-    hipLaunchKernelGGL(psi2_kernel, dim3(N, T * (T + 1) / 2), dim3(256), 0, st, N, M, Q, mx, vx, z,
+    unsigned groups = gpx_cdiv(N, PSI2_PPB);
+    if (groups > 65535) groups = 65535;
+    hipLaunchKernelGGL(psi2_kernel, dim3(T * (T + 1) / 2, groups), dim3(256), 0, st, N, M, Q, mx, vx, z,
                        ls, sf, psi2);
     GPX_CHECK_LAUNCH();
   }
@@ -357,25 +408,12 @@ int gpx_psi_bwd(int N, int M, int Q, const double *mx, const double *vx, const d
   hipLaunchKernelGGL(psi_bwd_rows_kernel, dim3(N), dim3(256), 0, st, N, M, Q, mx, vx, z, ls, dpsi1,
                      psi1, dpsi2, psi2, dmx, dvx, L, T, Lp, H, s0, Dn, Pn, En);
   GPX_CHECK_LAUNCH();
-  int rc;
   // LW = L^T [1 | d]  (M^2 x (Q+1)), reduction over n
-  rc = gpx_gemm_batched(1, 0, M * M, Q + 1, N, 1.0, L, M * M, 0, Dn, Q + 1, 0, 0.0, LW, Q + 1, 0, 1,
-                        1, nullptr, 0, stream);
+  int rc = gpx_gemm_batched(1, 0, M * M, Q + 1, N, 1.0, L, M * M, 0, Dn, Q + 1, 0, 0.0, LW, Q + 1, 0, 1,
+                            1, nullptr, 0, stream);
   if (rc) return rc;
-  rc = gpx_gemm_batched(1, 0, M, 2 * Q, N, 1.0, T, M, 0, Pn, 2 * Q, 0, 0.0, TP, 2 * Q, 0, 1, 1,
-                        nullptr, 0, stream);
-  if (rc) return rc;
-  rc = gpx_gemm_batched(1, 0, M, 2 * Q, N, 1.0, Lp, M, 0, En, 2 * Q, 0, 0.0, LE, 2 * Q, 0, 1, 1,
-                        nullptr, 0, stream);
-  if (rc) return rc;
-  hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(64), 0, st, N, Q, H, dell);
-  GPX_CHECK_LAUNCH();
-  hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(64), 0, st, N, 2, s0, cols0);
-  GPX_CHECK_LAUNCH();
-  hipLaunchKernelGGL(psi_bwd_final_kernel, dim3(M + 1), dim3(256), 0, st, M, Q, z, ls, sf, LW, TP,
-                     LE, colH, cols0, dsf2, dell, dZ);
-  GPX_CHECK_LAUNCH();
-  return GPX_OK;
+  return gpx_psi_bwd_tail(N, M, Q, z, ls, sf, T, Lp, H, s0, Dn, Pn, En, LW, TP, LE, cols0, dsf2, dell, dZ,
+                          stream);
 }
 
 }  // extern "C"

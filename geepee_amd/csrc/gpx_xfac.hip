@@ -3,6 +3,9 @@
 // reduction, so each is a single grid-stride launch.
 #include "gpx_common.h"
 
+// Workgroups of the x-gradient kernels; scal holds 2 + 2 * GPX_XF_BLOCKS doubles.
+#define GPX_XF_BLOCKS 64
+
 namespace {
 
 // SGPLVM get_cavity_x (aep_models.py:840-861) for rows idx (or all rows).
@@ -30,7 +33,7 @@ __global__ void lvm_cav_kernel(int B, int Q, const long *__restrict__ idx, const
 // SGPLVM x gradients and log-partition terms (aep_models.py:784-805, 817-838, 863-867;
 // base_models.py:913-929).  g1/g2 are N x Q (rows outside idx left untouched).
 // scal[0] = phi_prior_x + sc_cav phi(cav) + sc_post phi(post).
-__global__ __launch_bounds__(1024) void lvm_grad_kernel(int N, int B, int Q, const long *__restrict__ idx,
+__global__ __launch_bounds__(256) void lvm_grad_kernel(int N, int B, int Q, const long *__restrict__ idx,
                                                         const double *__restrict__ x1,
                                                         const double *__restrict__ x2,
                                                         const double *__restrict__ gmx,
@@ -41,7 +44,7 @@ __global__ __launch_bounds__(1024) void lvm_grad_kernel(int N, int B, int Q, con
   __shared__ double red[16];
   const double px1 = pm / pv, px2 = 1.0 / pv;
   double pc = 0.0, pp = 0.0;
-  for (long e = threadIdx.x; e < (long)B * Q; e += blockDim.x) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)B * Q; e += (long)gridDim.x * blockDim.x) {
     const long r = idx ? idx[e / Q] : e / Q;
     const int q = (int)(e % Q);
     const double f1 = x1[r * Q + q], f2 = exp(2.0 * x2[r * Q + q]);
@@ -88,9 +91,22 @@ __global__ __launch_bounds__(1024) void lvm_grad_kernel(int N, int B, int Q, con
   pc = gpx_block_sum(pc, red);
   pp = gpx_block_sum(pp, red);
   if (threadIdx.x == 0) {
-    const double prior = 0.5 * (pm * pm / pv + log(pv)) * (double)N * Q;
-    scal[0] = prior + sc_cav * pc + sc_post * pp;
+    scal[2 + 2 * blockIdx.x] = pc;
+    scal[3 + 2 * blockIdx.x] = pp;
   }
+}
+
+// scal[0] from the per-block partials, summed in block order.
+__global__ void lvm_grad_finish(int nblk, int N, int Q, double pm, double pv, double sc_cav, double sc_post,
+                                double *__restrict__ scal) {
+  if (threadIdx.x != 0) return;
+  double pc = 0.0, pp = 0.0;
+  for (int b = 0; b < nblk; ++b) {
+    pc += scal[2 + 2 * b];
+    pp += scal[3 + 2 * b];
+  }
+  const double prior = 0.5 * (pm * pm / pv + log(pv)) * (double)N * Q;
+  scal[0] = prior + sc_cav * pc + sc_post * pp;
 }
 
 // SGPSSM cavity of every x_t (aep_models.py:1376-1387 with base_models.py:1719-1725).
@@ -113,7 +129,7 @@ __global__ void ssm_cav_kernel(int N, int Q, const double *__restrict__ xf1, con
 // 1389-1437).  Window [s, e) of emission rows; transitions s..e-2 -> s+1..e-1.
 // dup/vup: emission input grads (rows s..e-1, leading dim ldu), dmt/dvt: transition
 // target grads (rows s+1..e-1, ld Q), dmn/dvn: dynamics input grads (rows s..e-2, ld ldn).
-__global__ __launch_bounds__(1024) void ssm_grad_kernel(
+__global__ __launch_bounds__(256) void ssm_grad_kernel(
     int N, int Q, int s, int e, const double *__restrict__ xf1, const double *__restrict__ x2,
     double alpha, double px1, double px2, const double *__restrict__ dup, const double *__restrict__ vup,
     int ldu, const double *__restrict__ dmt, const double *__restrict__ dvt,
@@ -122,7 +138,7 @@ __global__ __launch_bounds__(1024) void ssm_grad_kernel(
   __shared__ double red[16];
   double pp = 0.0, pc = 0.0;
   const double ia = 1.0 / alpha;
-  for (long i = threadIdx.x; i < (long)N * Q; i += blockDim.x) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < (long)N * Q; i += (long)gridDim.x * blockDim.x) {
     const int t = (int)(i / Q), q = (int)(i % Q);
     const double f1 = xf1[i], f2 = exp(2.0 * x2[i]);
     const bool end = (t == 0 || t == N - 1);
@@ -165,9 +181,20 @@ __global__ __launch_bounds__(1024) void ssm_grad_kernel(
   pp = gpx_block_sum(pp, red);
   pc = gpx_block_sum(pc, red);
   if (threadIdx.x == 0) {
-    const double m0 = px1 / px2, v0 = 1.0 / px2;
-    scal[0] = 0.5 * Q * (m0 * m0 / v0 + log(v0)) + pp + pc;
+    scal[2 + 2 * blockIdx.x] = pp;
+    scal[3 + 2 * blockIdx.x] = pc;
   }
+}
+
+__global__ void ssm_grad_finish(int nblk, int Q, double px1, double px2, double *__restrict__ scal) {
+  if (threadIdx.x != 0) return;
+  double pp = 0.0, pc = 0.0;
+  for (int b = 0; b < nblk; ++b) {
+    pp += scal[2 + 2 * b];
+    pc += scal[3 + 2 * b];
+  }
+  const double m0 = px1 / px2, v0 = 1.0 / px2;
+  scal[0] = 0.5 * Q * (m0 * m0 / v0 + log(v0)) + pp + pc;
 }
 
 }  // namespace
@@ -193,8 +220,13 @@ int gpx_lvm_x_grads(int N, int B, int Q, const long *idx, const double *x1, cons
   if (N < 1 || B < 1 || Q < 1 || !x1 || !x2 || !gmx || !gvx || !g1 || !g2 || !scal) return GPX_EINVAL;
   const double sc_cav = -(double)N / B / alpha;
   const double sc_post = -(double)N / B * (1.0 - 1.0 / alpha);
-  hipLaunchKernelGGL(lvm_grad_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, N, B, Q, idx, x1, x2, gmx,
+  unsigned nblk = gpx_cdiv((long)B * Q, 256);
+  if (nblk > GPX_XF_BLOCKS) nblk = GPX_XF_BLOCKS;
+  hipLaunchKernelGGL(lvm_grad_kernel, dim3(nblk), dim3(256), 0, (hipStream_t)stream, N, B, Q, idx, x1, x2, gmx,
                      gvx, alpha, prior_mean, prior_var, nat_param, sc_cav, sc_post, g1, g2, scal);
+  GPX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(lvm_grad_finish, dim3(1), dim3(64), 0, (hipStream_t)stream, (int)nblk, N, Q, prior_mean,
+                     prior_var, sc_cav, sc_post, scal);
   GPX_CHECK_LAUNCH();
   return GPX_OK;
 }
@@ -217,9 +249,14 @@ int gpx_ssm_x_grads(int N, int Q, int s, int e, const double *xf1, const double 
   if (N < 2 || Q < 1 || s < 0 || e > N || e - s < 2 || !xf1 || !x2 || !dup || !vup || !dmt || !dvt ||
       !dmn || !dvn || !g1 || !g2 || !scal || ldu < Q || ldn < Q)
     return GPX_EINVAL;
-  hipLaunchKernelGGL(ssm_grad_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, N, Q, s, e, xf1, x2, alpha,
+  unsigned nblk = gpx_cdiv((long)N * Q, 256);
+  if (nblk > GPX_XF_BLOCKS) nblk = GPX_XF_BLOCKS;
+  hipLaunchKernelGGL(ssm_grad_kernel, dim3(nblk), dim3(256), 0, (hipStream_t)stream, N, Q, s, e, xf1, x2, alpha,
                      prior_mean / prior_var, 1.0 / prior_var, dup, vup, ldu, dmt, dvt, dmn, dvn, ldn, g1, g2,
                      scal);
+  GPX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(ssm_grad_finish, dim3(1), dim3(64), 0, (hipStream_t)stream, (int)nblk, Q,
+                     prior_mean / prior_var, 1.0 / prior_var, scal);
   GPX_CHECK_LAUNCH();
   return GPX_OK;
 }

@@ -22,15 +22,21 @@ def mm(A, B, C=None, tA=False, tB=False, alpha=1.0, beta=0.0, splits=None):
         n = B.shape[-2] if tB else B.shape[-1]
         C = empty(batch, m, n, like=A) if (A.dim() == 3 or B.dim() == 3) else empty(m, n, like=A)
         beta = 0.0
+    tile = 0
     if splits is None:
         m, n = C.shape[-2:]
         k = A.shape[-2] if tA else A.shape[-1]
         batch = C.shape[0] if C.dim() == 3 else 1
-        tiles = -(-m // 64) * -(-n // 64) * batch
-        splits = 1
-        if tiles < 256 and k >= 1024:
-            splits = max(1, min(-(-512 // tiles), k // 256, 64))
-    return _lib.mm(A, B, C, tA=tA, tB=tB, alpha=alpha, beta=beta, splits=splits)
+        from . import autotune
+        hit = autotune.lookup('gemm', (m, n, k, batch, int(tA), int(tB)))
+        if hit:
+            splits, tile = hit['splits'], hit['tile']
+        else:
+            tiles = -(-m // 64) * -(-n // 64) * batch
+            splits = 1
+            if tiles < 256 and k >= 1024:
+                splits = max(1, min(-(-512 // tiles), k // 256, 64))
+    return _lib.mm(A, B, C, tA=tA, tB=tB, alpha=alpha, beta=beta, splits=splits, tile=tile)
 
 
 def spd_inv(A, out=None):
@@ -189,7 +195,7 @@ def gauss_tilted(m, v, y, sn, alpha, scale, want_dmall=False):
     B, D = m.shape
     dm, dv = torch.empty_like(m), torch.empty_like(m)
     dma = torch.empty_like(m) if want_dmall else None
-    scal = empty(2, like=m)
+    scal = empty(2 + 2 * 64, like=m)
     call('gpx_gauss_tilted', B, D, P(m), P(v), P(y), P(sn), float(alpha), float(scale), P(dm), P(dv),
          P(dma), P(scal), S())
     return dm, dv, dma, scal
@@ -222,3 +228,63 @@ def emission_predict(C, Rp, mf, vf):
     vy = empty(Nb, Dout, like=mf)
     call('gpx_emission_predict', Nb, Dout, Din, P(C), P(Rp), P(mf), P(vf), P(my), P(vy), S())
     return my, vy
+
+
+def fused_fits(M, Q, D):
+    """Shape limits of the fused moment-matched kernels (larger layers use the
+    materialising kernels)."""
+    return M <= 256 and Q <= 15 and D <= 32
+
+
+def fused_plan(N, M, Q=None, D=None, tuned=True):
+    """Launch shape of the fused moment-matched kernels: points per block (16*nsub), pair-tile
+    splits of the forward / per-point pass, point ranges of the per-pair pass.  A plan
+    stored by the autotuner for (N, M, Q, D) wins over the heuristic."""
+    if tuned and Q is not None:
+        from . import autotune
+        hit = autotune.lookup('fused', (N, M, Q, D))
+        if hit:
+            return hit
+    T = (M + 15) // 16
+    ntiles = T * (T + 1) // 2
+    nsub = 4 if N >= 4096 else (2 if N >= 512 else 1)
+    fwd_blocks = -(-N // (16 * nsub))
+    splits_f = max(1, min(ntiles, -(-1024 // fwd_blocks)))
+    pt_blocks = -(-N // 16)
+    splits_b = max(1, min(ntiles, -(-1024 // pt_blocks)))
+    nsplit = max(1, min(-(-N // 16), -(-1024 // (ntiles * 4))))
+    return {'nsub': nsub, 'splits_f': splits_f, 'splits_b': splits_b, 'nsplit': nsplit}
+
+
+def sym_pairmajor(B):
+    D, M, _ = B.shape
+    out = empty(M, M, D, like=B)
+    call('gpx_sym_pairmajor', D, M, P(B), P(out), S())
+    return out
+
+
+def fused_mm_fwd(mx, vx, z, ls, sf, Ahat, Bp, plan=None):
+    N, Q = mx.shape
+    M = z.shape[0]
+    D = Ahat.shape[0]
+    plan = plan or fused_plan(N, M, Q, D)
+    psi1 = empty(N, M, like=mx)
+    mout = empty(N, D, like=mx)
+    vout = empty(N, D, like=mx)
+    work = empty(query('gpx_fused_fwd_workspace', N, D, plan['splits_f']), like=mx)
+    call('gpx_fused_mm_fwd', N, M, Q, D, P(mx), P(vx), P(z), P(ls), P(sf), P(Ahat), P(Bp),
+         plan['splits_f'], plan['nsub'], P(psi1), P(mout), P(vout), P(work), S())
+    return psi1, mout, vout
+
+
+def fused_mm_bwd(mx, vx, z, ls, sf, Ahat, Bp, psi1, dm_all, dv, plan=None):
+    N, Q = mx.shape
+    M = z.shape[0]
+    D = Ahat.shape[0]
+    plan = plan or fused_plan(N, M, Q, D)
+    outs = [empty(D, M, M, like=mx), empty(1, like=mx), empty(Q, like=mx), empty(M, Q, like=mx),
+            empty(N, Q, like=mx), empty(N, Q, like=mx)]
+    work = empty(query('gpx_fused_bwd_workspace', N, M, Q, D, plan['splits_b'], plan['nsplit']), like=mx)
+    call('gpx_fused_mm_bwd', N, M, Q, D, P(mx), P(vx), P(z), P(ls), P(sf), P(Ahat), P(Bp), P(psi1),
+         P(dm_all), P(dv), plan['splits_b'], plan['nsplit'], *[P(o) for o in outs], P(work), S())
+    return outs

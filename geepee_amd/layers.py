@@ -40,6 +40,9 @@ class SGP_Layer(object):
         self.M = no_pseudo
         self.nat_param = nat_param
         self._pred_ready = False
+        # fused=True: moment-matched propagation never writes psi2 to HBM; the forward
+        # returns None in psi2's slot and the adjoint recomputes it.
+        self.fused = False
 
     # ------------------------------------------------------------------
     # q(u) and Kuu (base_models.py:454-488, 630-658)
@@ -143,6 +146,10 @@ class SGP_Layer(object):
     def _forward_prop_random_thru_cav_mm(self, mx, vx):
         N = mx.shape[0]
         M, D = self.M, self.Dout
+        if self.fused and dops.fused_fits(M, mx.shape[1], D):
+            self._Bp = dops.sym_pairmajor(self.Bhat_sto)
+            psi1, mout, vout = dops.fused_mm_fwd(mx, vx, self.zu, self.ls, self.sf, self.Ahat, self._Bp)
+            return mout, vout, psi1, None
         psi1, psi2 = dops.psi_fwd(mx, vx, self.zu, self.ls, self.sf)
         mout = mm(psi1, self.Ahat, tB=True)
         Bp = mm(psi2.reshape(N, M * M), self.Bhat_sto.reshape(D, M * M), tB=True)
@@ -266,13 +273,17 @@ class SGP_Layer(object):
         if dm_all is None:
             dm_all = dops.dmall(dm, dv, m)
         dAhat = mm(dm_all, psi1, tA=True)
-        dBhat = mm(dv, psi2.reshape(B, M * M), tA=True).reshape(D, M, M)
-        dpsi1 = mm(dm_all, self.Ahat)
-        Bsym = dops.sym(self.Bhat_sto, 0.5)
-        dpsi2 = mm(dv, Bsym.reshape(D, M * M)).reshape(B, M, M)
-        dsf2, dell, dzu, dmx, dvx = dops.psi_bwd(mx, vx, self.zu, self.ls, self.sf,
-                                                 dpsi1, psi1, dpsi2, psi2)
-        del dpsi2
+        if psi2 is None:
+            dBhat, dsf2, dell, dzu, dmx, dvx = dops.fused_mm_bwd(
+                mx, vx, self.zu, self.ls, self.sf, self.Ahat, self._Bp, psi1, dm_all, dv)
+        else:
+            dBhat = mm(dv, psi2.reshape(B, M * M), tA=True).reshape(D, M, M)
+            dpsi1 = mm(dm_all, self.Ahat)
+            Bsym = dops.sym(self.Bhat_sto, 0.5)
+            dpsi2 = mm(dv, Bsym.reshape(D, M * M)).reshape(B, M, M)
+            dsf2, dell, dzu, dmx, dvx = dops.psi_bwd(mx, vx, self.zu, self.ls, self.sf,
+                                                     dpsi1, psi1, dpsi2, psi2)
+            del dpsi2
         sumdv = colsum(dv, B * D, 1)
         dvcav = _sandwich(Ki, dBhat, Ki)
         dmcav = mm(dvcav, _col(self.muhat), alpha=2.0).reshape(D, M)

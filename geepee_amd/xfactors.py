@@ -25,7 +25,7 @@ def lvm_x_grads(xs, gmx, gvx, N, B, alpha, prior_mean, prior_var, Din, nat_param
     x1, x2, idx = xs['x1'], xs['x2'], xs['idx']
     g1 = torch.zeros_like(x1) if idx is not None else torch.empty_like(x1)
     g2 = torch.zeros_like(x2) if idx is not None else torch.empty_like(x2)
-    scal = torch.empty(1, dtype=F64, device=x1.device)
+    scal = torch.empty(2 + 2 * 64, dtype=F64, device=x1.device)
     call('gpx_lvm_x_grads', N, B, Din, P(idx), P(x1), P(x2), P(gmx.contiguous()), P(gvx.contiguous()),
          float(alpha), float(prior_mean), float(prior_var), int(bool(nat_param)), P(g1), P(g2),
          P(scal), S())
@@ -45,7 +45,7 @@ def ssm_grads(xf1, x2, alpha, prior_mean, prior_var, s, e, dup, vup, dmt, dvt, d
     N, Q = xf1.shape
     g1 = torch.empty_like(xf1)
     g2 = torch.empty_like(xf1)
-    scal = torch.empty(1, dtype=F64, device=xf1.device)
+    scal = torch.empty(2 + 2 * 64, dtype=F64, device=xf1.device)
     call('gpx_ssm_x_grads', N, Q, s, e, P(xf1), P(x2), float(alpha), float(prior_mean),
          float(prior_var), P(dup), P(vup), dup.stride(0), P(dmt), P(dvt), P(dmn), P(dvn),
          dmn.stride(0), P(g1), P(g2), P(scal), S())

@@ -151,7 +151,7 @@ int gpx_phi(int D, const double *ldK, const double *ldP, double sgnP, const doub
 /* ---- likelihoods ----------------------------------------------------------- */
 
 /* Gaussian tilted log-normaliser and its adjoint: scal[0] = logZ (incl. the alpha
- * constant), scal[1] = dsn * scale; dm, dv = scale * dlogZ/dm, dlogZ/dv; dmall (optional)
+ * constant), scal[1] = dsn * scale (scal holds 130 doubles: the rest are block partials); dm, dv = scale * dlogZ/dm, dlogZ/dv; dmall (optional)
  * = dm - 2 dv m.  Replaces Gauss_Layer.compute_log_Z + backprop_grads
  * (geepee/lik_layers.py:104-181). */
 int gpx_gauss_tilted(int B, int D, const double *m, const double *v, const double *y,
@@ -180,6 +180,42 @@ int gpx_emission_tilted(int Nb, int Dout, int Din, const double *C, const double
 int gpx_emission_predict(int Nb, int Dout, int Din, const double *C, const double *Rp, const double *mf,
                          const double *vf, double *my, double *vy, void *stream);
 
+/* ---- fused moment-matched layer (psi2 never written to HBM) -------------------- */
+
+/* Symmetrised Bhat in pair-major layout, out[a, b, d] = (B[d,a,b] + B[d,b,a]) / 2: the operand
+ * layout of the fused kernels (the symmetrisation of geepee/kernels.py:415-418). */
+int gpx_sym_pairmajor(int D, int M, const double *B, double *out, void *stream);
+
+/* Forward of a Gaussian-input layer: psi1 (N x M), mout = psi1 Ahat^T and
+ * vout = sf2 + <Bhat[d], psi2_n> - mout^2 with psi2 recomputed per 16x16 tile (FP64 MFMA for
+ * the exponent GEMM and the Bhat contraction).  splits (grid.y over pair tiles) and nsub
+ * (1, 2, 4: 16*nsub points per block) are tuning knobs.  Replaces compute_psi_weave followed by
+ * the einsums of geepee/aep_models.py:193-198 (and base_models.py:298-304). */
+long gpx_fused_fwd_workspace(int N, int D, int splits);
+int gpx_fused_mm_fwd(int N, int M, int Q, int D, const double *mx, const double *vx, const double *z,
+                     const double *ls, const double *sf, const double *Ahat, const double *Bp,
+                     int splits, int nsub, double *psi1, double *mout, double *vout, double *work,
+                     void *stream);
+
+/* Adjoint of the same layer without psi2 or dL/dpsi2 in HBM: from dm_all = dm - 2 dv m and dv
+ * (N x D) produces dBhat = sum_n dv_nd psi2_n (D x M x M), and the psi adjoints dsf2, dell, dZ,
+ * dmx, dvx.  Replaces geepee/aep_models.py:235-243 with compute_psi_derivatives
+ * (geepee/kernels.py:302-309, 355-444). */
+long gpx_fused_bwd_workspace(int N, int M, int Q, int D, int splits, int nsplit);
+int gpx_fused_mm_bwd(int N, int M, int Q, int D, const double *mx, const double *vx, const double *z,
+                     const double *ls, const double *sf, const double *Ahat, const double *Bp,
+                     const double *psi1, const double *dm_all, const double *dv, int splits, int nsplit,
+                     double *dBhat, double *dsf2, double *dell, double *dZ, double *dmx, double *dvx,
+                     double *work, void *stream);
+
+/* Shared final stage of both psi adjoints (pair sums LW and per-point statistics in, dsf2,
+ * dell, dZ out); the tail of compute_psi_derivatives (geepee/kernels.py:430-442, 371-376). */
+int gpx_psi_bwd_tail(int N, int M, int Q, const double *z, const double *ls, const double *sf,
+                     const double *T, const double *Lp, const double *H, const double *s0,
+                     const double *Dn, const double *Pn, const double *En, const double *LW,
+                     double *TP, double *LE, double *cols0, double *dsf2, double *dell, double *dZ,
+                     void *stream);
+
 /* ---- serving ------------------------------------------------------------------ */
 
 /* One launch of predictive mean / variance for many (request, model) work items of at
@@ -204,7 +240,8 @@ int gpx_lvm_x_cavity(int B, int Q, const long *idx, const double *x1, const doub
                      void *stream);
 
 /* SGPLVM x gradients (rows idx of the N x Q outputs) and scal[0] = the x log-partition
- * energy: aep_models.py:784-805, 817-838, 863-867 and base_models.py:913-929. */
+ * energy (scal holds 130 doubles: the rest are block partials):
+ * aep_models.py:784-805, 817-838, 863-867 and base_models.py:913-929. */
 int gpx_lvm_x_grads(int N, int B, int Q, const long *idx, const double *x1, const double *x2,
                     const double *gmx, const double *gvx, double alpha, double prior_mean,
                     double prior_var, int nat_param, double *g1, double *g2, double *scal,
@@ -216,11 +253,23 @@ int gpx_ssm_x_cavity(int N, int Q, const double *xf1, const double *x2, double a
                      double prior_var, double *cm, double *cv, void *stream);
 
 /* SGPSSM x-factor gradients and scal[0] = phi_prior_x + phi_post_x + phi_cav_x for the
- * window [s, e): aep_models.py:1170-1196, 1208-1315, 1389-1437. */
+ * window [s, e) (scal holds 130 doubles: the rest are block partials):
+ * aep_models.py:1170-1196, 1208-1315, 1389-1437. */
 int gpx_ssm_x_grads(int N, int Q, int s, int e, const double *xf1, const double *x2, double alpha,
                     double prior_mean, double prior_var, const double *dup, const double *vup, int ldu,
                     const double *dmt, const double *dvt, const double *dmn, const double *dvn, int ldn,
                     double *g1, double *g2, double *scal, void *stream);
+
+/* ---- optimiser ------------------------------------------------------------------ */
+
+/* One Adam iteration t (1-based) in place on the flat parameter vector x with moments
+ * m, v from the flat gradient g, all n doubles in sorted-key order: the update of
+ * adam (geepee/utils.py:93-117; b1, b2, eps, bias correction 1 - b^t), with non-finite
+ * gradient entries taken as 0 as ObjectiveWrapper does (utils.py:46-53).  The same
+ * IEEE operation sequence as the reference (no contraction).  cost (optional)
+ * receives energy[0]. */
+int gpx_adam_step(long n, double *x, const double *g, double *m, double *v, double lr, double b1, double b2,
+                  double eps, int t, const double *energy, double *cost, void *stream);
 
 #ifdef __cplusplus
 }

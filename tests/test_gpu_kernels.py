@@ -223,13 +223,13 @@ def test_gauss_tilted():
     sn = np.array([np.log(0.3)])
     logZ, dm, dv = O.gauss_logZ(sn, m, v, y, alpha)
     dsn = O.gauss_dsn(sn, dv, B, D, alpha, scale)
-    o = [dev(np.zeros((B, D))) for _ in range(3)] + [dev(np.zeros(2))]
+    o = [dev(np.zeros((B, D))) for _ in range(3)] + [dev(np.zeros(130))]
     L.call('gpx_gauss_tilted', B, D, L.P(dev(m)), L.P(dev(v)), L.P(dev(y)), L.P(dev(sn)), alpha, scale,
            *[L.P(t) for t in o], L.S())
     assert nrel(host(o[0]), scale * dm) < 1e-14
     assert nrel(host(o[1]), scale * dv) < 1e-14
     assert nrel(host(o[2]), scale * dm - 2 * scale * dv * m) < 1e-14
-    sc = host(o[3])
+    sc = host(o[3])[:2]
     assert abs(sc[0] - logZ) < 1e-12 * abs(logZ)
     assert abs(sc[1] - dsn[0]) < 1e-12 * abs(dsn[0])
 

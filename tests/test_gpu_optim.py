@@ -1,0 +1,75 @@
+"""On-device Adam (GPU only): the update kernel against the reference's numpy update,
+and the SGPR workload of N=1000 points, M=50 (one eval then 200 Adam steps) against
+the oracle's objective driven by the oracle's Adam."""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import aep_oracle as O
+from geepee_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def nrel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.linalg.norm(np.ravel(a - b)) / max(np.linalg.norm(np.ravel(b)), 1e-300)
+
+
+def test_adam_step_matches_numpy_update():
+    from geepee_amd import _lib
+    say('adam step')
+    rs = np.random.RandomState(0)
+    n = 10007
+    x, m, v = rs.randn(n), 0.1 * rs.randn(n), rs.uniform(0, 0.1, n)
+    g = rs.randn(n)
+    g[::97] = np.nan
+    g[5] = np.inf
+    dev = lambda a: torch.tensor(a, dtype=torch.float64, device='cuda')
+    xd, md, vd, gd = dev(x), dev(m), dev(v), dev(g)
+    e = dev([3.5])
+    cost = dev([0.0])
+    lr, b1, b2, eps = 1e-3, 0.9, 0.999, 1e-8
+    for t in (1, 7):
+        _lib.call('gpx_adam_step', n, _lib.P(xd), _lib.P(gd), _lib.P(md), _lib.P(vd), lr, b1, b2, eps, t,
+                  _lib.P(e), _lib.P(cost), _lib.S())
+        gg = np.where(np.isfinite(g), g, 0.0)
+        m = (1 - b1) * gg + b1 * m
+        v = (1 - b2) * (gg ** 2) + b2 * v
+        mhat = m / (1 - b1 ** t)
+        vhat = v / (1 - b2 ** t)
+        x = x - lr * mhat / (np.sqrt(vhat) + eps)
+        assert np.max(np.abs(xd.cpu().numpy() - x)) <= 1e-15 * np.max(np.abs(x))
+        assert np.max(np.abs(md.cpu().numpy() - m)) <= 1e-15 * np.max(np.abs(m))
+        assert cost.cpu().numpy()[0] == 3.5
+
+
+@pytest.mark.parametrize('graph', [True, False])
+def test_sgpr_eval_then_200_adam_steps(graph):
+    from geepee_amd import aep_models as G
+    say('sgpr 200 adam steps graph=%s' % graph)
+    x, y, p = W.sgpr_problem(N=1000, M=50, seed=0)
+    model = G.SGPR(x, y, 50)
+    e0, g0 = model.objective_function(p, 1000, alpha=0.5)
+    want0 = O.sgpr_objective(x, y, p, 50, 1000, alpha=0.5)
+    assert abs(e0 - want0[0]) <= 1e-9 * abs(want0[0])
+    final, costs = model.optimise_device(p, maxiter=200, adam_lr=1e-3, alpha=0.5, graph=graph)
+    vec0, args = O.flatten_dict(p)
+
+    def f(vec):
+        e, g = O.sgpr_objective(x, y, O.unflatten_dict(vec, args), 50, 1000, alpha=0.5)
+        return e, O.flatten_dict(g)[0]
+
+    xo, fs = O.adam(f, vec0, 200, step_size=1e-3)
+    assert nrel(costs, fs) < 1e-9, nrel(costs, fs)
+    got = O.flatten_dict(final)[0]
+    assert nrel(got - vec0, xo - vec0) < 1e-7, nrel(got - vec0, xo - vec0)
+    assert nrel(got, xo) < 1e-9
+    assert fs[-1] < fs[0]
