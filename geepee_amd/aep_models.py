@@ -17,6 +17,7 @@ import torch
 from . import _lib, dops
 from .config import PROP_MM, PROP_MC
 from .dops import mm, lincomb, colsum
+from . import init
 from .layers import SGP_Layer
 from .utils import ObjectiveWrapper, flatten_dict, unflatten_dict, adam
 
@@ -323,10 +324,9 @@ class SGPR(Base_Model):
         return my.cpu().numpy(), vy.cpu().numpy()
 
     def init_hypers(self, y_train=None):
-        from .workloads import layer_params
-        rs = np.random.RandomState(0)
-        p = layer_params(rs, self.Din, self.Dout, self.M, x=self.x_train)
-        p['sn'] = np.log(0.01) * np.ones(1)
+        """base_models.py:1036-1049: the layer on the training inputs, sn = log 0.01."""
+        p = init.sgp_layer(self.N, self.M, self.Din, self.Dout, self.nat_param, x_train=self.x_train)
+        p.update(init.gauss_lik())
         return p
 
 
@@ -346,6 +346,16 @@ class SDGPR(Base_Model):
         self.lik_layer = Gauss_Layer(self.N, self.Dout)
         self.sgp_layers = [SGP_Layer(self.N, self.size[i], self.size[i + 1], self.Ms[i])
                            for i in range(self.L)]
+
+    def init_hypers(self, y_train=None):
+        """base_models.py:1290-1312: layer 0 on the training inputs, the hidden layers
+        without data, sn = log 0.01."""
+        p = {}
+        for i, layer in enumerate(self.sgp_layers):
+            p.update(init.sgp_layer(self.N, layer.M, layer.Din, layer.Dout, True,
+                                    x_train=self.x_train if i == 0 else None, key_suffix='_%d' % i))
+        p.update(init.gauss_lik())
+        return p
 
     def update_hypers(self, params):
         dp = self._device_params(params)
@@ -435,6 +445,11 @@ class SGPLVM(Base_Model):
         self.sgp_layer = SGP_Layer(self.N, self.Din, self.Dout, self.M, nat_param)
         from . import xfactors
         self._xf = xfactors
+
+    def init_hypers(self, y_train=None):
+        """base_models.py:839-873 (q(u) at the regression optimum instead of an L-BFGS fit)."""
+        y = self.y_train if y_train is None else np.asarray(y_train, dtype=np.float64)
+        return init.sgplvm(y, self.Din, self.M, self.nat_param)
 
     def update_hypers(self, params):
         dp = self._device_params(params)
@@ -534,6 +549,12 @@ class SGPSSM(Base_Model):
             self.lik_layer = Gauss_Layer(self.N, self.Dout)
         else:
             self.emi_layer = Gauss_Emis(self.y_dev, self.Dout, self.Din + self.Dcon_emi)
+
+    def init_hypers(self, y_train=None):
+        """base_models.py:1597-1683 (latent path from y or its PCA projection; q(u) at the
+        regression optimum instead of L-BFGS fits)."""
+        y = self.y_train if y_train is None else np.asarray(y_train, dtype=np.float64)
+        return init.sgpssm(y, self.Din, self.M, self.x_control, self.gp_emi, self.Dcon_emi > 0)
 
     def update_hypers(self, params):
         dp = self._device_params(params)

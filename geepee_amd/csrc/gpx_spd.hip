@@ -3,23 +3,32 @@
 // Replaces the per-output-dim np.linalg.inv / slogdet / solve calls of the layer
 // algebra (base_models.py:462-476; aep_models.py:62-95, 523-533).  The matrix stays in
 // registers for the whole factorisation: thread (ty, tx) of a G x G block owns the
-// strided sub-grid rows ty + G*a, cols tx + G*c (a, c < TB), so a step touches every
-// element once and reads only one row and one column of broadcast data from LDS.
-// The algorithm is Gauss-Jordan elimination without pivoting, which is stable for SPD
-// input; its pivots are the D of A = L D L^T, so sum(log pivot) is log det A and the
-// first non-positive pivot flags a matrix that is not positive definite.
+// strided sub-grid rows ty + G*a, cols tx + G*c (a, c < TB).
+//
+// Algorithm: Gauss-Jordan elimination without pivoting (stable for SPD input), NB
+// pivots per step.  With K the next NB indices and P = A[K,K]:
+//   A[i,j] -= A[i,K] P^-1 A[K,j],  A[K,j] <- P^-1 A[K,j],  A[i,K] <- -A[i,K] P^-1,
+//   A[K,K] <- P^-1,
+// which is NB scalar Gauss-Jordan steps merged, so one barrier serves NB pivots.
+// P^-1 comes from a scalar Gauss-Jordan on the NB x NB block, whose pivots are the
+// pivots of the full elimination: the D of A = L D L^T.  Hence sum(log pivot) is
+// log det A, and the first non-positive pivot flags a matrix that is not positive
+// definite.  The matrix is padded with the identity up to a multiple of NB.
 #include "gpx_common.h"
 
 namespace {
 
-template <int G, int TB>
+template <int G, int TB, int NB>
 __global__ __launch_bounds__(G *G) void spd_gj_kernel(int M, int batch, const double *__restrict__ A,
                                                       int lda, long sA, double *__restrict__ out,
                                                       int ldo, long sO, double *__restrict__ logdet,
                                                       int *__restrict__ info) {
   constexpr int W = G * TB;
-  __shared__ double rowk[2][W];
-  __shared__ double colk[2][W];
+  static_assert(G % NB == 0, "pivot blocks must not straddle thread rows");
+  __shared__ double rowk[2][NB][W];
+  __shared__ double colk[2][NB][W];
+  __shared__ double piv[W];
+  __shared__ double red[16];
   const int b = blockIdx.x;
   if (b >= batch) return;
   const double *Ab = A + b * sA;
@@ -32,65 +41,134 @@ __global__ __launch_bounds__(G *G) void spd_gj_kernel(int M, int batch, const do
       const int i = ty + G * a, j = tx + G * c;
       r[a][c] = (i < M && j < M) ? Ab[(long)i * lda + j] : (i == j ? 1.0 : 0.0);
     }
-  double ld = 0.0;
   int fail = 0;
-  for (int k = 0; k < M; ++k) {
-    const int buf = k & 1, kb = k / G, km = k % G;
-    if (ty == km) {
+  const int Mp = (M + NB - 1) / NB * NB;
+  for (int k = 0; k < Mp; k += NB) {
+    const int buf = (k / NB) & 1, kb = k / G, km = k % G;
+    // rows K (owned by ty in [km, km+NB)) and columns K (tx in [km, km+NB)) to LDS
+    if (ty >= km && ty < km + NB) {
 #pragma unroll
       for (int a = 0; a < TB; ++a)
         if (a == kb)
 #pragma unroll
-          for (int c = 0; c < TB; ++c) rowk[buf][tx + G * c] = r[a][c];
+          for (int c = 0; c < TB; ++c) rowk[buf][ty - km][tx + G * c] = r[a][c];
     }
-    if (tx == km) {
+    if (tx >= km && tx < km + NB) {
 #pragma unroll
       for (int c = 0; c < TB; ++c)
         if (c == kb)
 #pragma unroll
-          for (int a = 0; a < TB; ++a) colk[buf][ty + G * a] = r[a][c];
+          for (int a = 0; a < TB; ++a) colk[buf][tx - km][ty + G * a] = r[a][c];
     }
     __syncthreads();
-    const double p = rowk[buf][k];
-    if (!(p > 0.0) || !(p < INFINITY)) {
-      fail = k + 1;
+    // P^-1 by scalar Gauss-Jordan on the NB x NB pivot block (every thread, same order)
+    double Pi[NB][NB];
+#pragma unroll
+    for (int t = 0; t < NB; ++t)
+#pragma unroll
+      for (int u = 0; u < NB; ++u) Pi[t][u] = rowk[buf][t][k + u];
+    int bad = 0;
+#pragma unroll
+    for (int s = 0; s < NB; ++s) {
+      const double p = Pi[s][s];
+      if (!bad && !(p > 0.0 && p < INFINITY)) bad = s + 1;
+      const double rp = 1.0 / p;
+      if (threadIdx.x == 0) piv[k + s] = p;
+      double rs[NB], cs[NB];
+#pragma unroll
+      for (int u = 0; u < NB; ++u) rs[u] = Pi[s][u] * rp;
+#pragma unroll
+      for (int t = 0; t < NB; ++t) cs[t] = Pi[t][s];
+#pragma unroll
+      for (int t = 0; t < NB; ++t)
+#pragma unroll
+        for (int u = 0; u < NB; ++u) Pi[t][u] = fma(-cs[t], rs[u], Pi[t][u]);
+#pragma unroll
+      for (int u = 0; u < NB; ++u) Pi[s][u] = rs[u];
+#pragma unroll
+      for (int t = 0; t < NB; ++t) Pi[t][s] = -cs[t] * rp;
+      Pi[s][s] = rp;
+    }
+    if (bad) {
+      fail = k + bad;
       break;
     }
-    const double rp = 1.0 / p;
-    ld += log(p);
-    double cv[TB], rv[TB];
-#pragma unroll
-    for (int a = 0; a < TB; ++a) cv[a] = colk[buf][ty + G * a];
-#pragma unroll
-    for (int c = 0; c < TB; ++c) rv[c] = rowk[buf][tx + G * c] * rp;
+    double cv[TB][NB];
 #pragma unroll
     for (int a = 0; a < TB; ++a)
 #pragma unroll
-      for (int c = 0; c < TB; ++c) r[a][c] = fma(-cv[a], rv[c], r[a][c]);
-    // row k becomes A[k][j] / p, column k becomes -A[i][k] / p, the pivot 1 / p
-    if (ty == km) {
+      for (int t = 0; t < NB; ++t) cv[a][t] = colk[buf][t][ty + G * a];
+    const bool rowK = ty >= km && ty < km + NB, colK = tx >= km && tx < km + NB;
+    const int tr = ty - km, uc = tx - km;
+    // column by column: rv = P^-1 A[K, j], then A[i, j] -= A[i, K] rv
 #pragma unroll
-      for (int a = 0; a < TB; ++a)
-        if (a == kb)
+    for (int c = 0; c < TB; ++c) {
+      double rk[NB], rv[NB];
 #pragma unroll
-          for (int c = 0; c < TB; ++c) r[a][c] = rv[c];
+      for (int u = 0; u < NB; ++u) rk[u] = rowk[buf][u][tx + G * c];
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        double v = 0.0;
+#pragma unroll
+        for (int u = 0; u < NB; ++u) v = fma(Pi[t][u], rk[u], v);
+        rv[t] = v;
+      }
+#pragma unroll
+      for (int a = 0; a < TB; ++a) {
+        double v = r[a][c];
+#pragma unroll
+        for (int t = 0; t < NB; ++t) v = fma(-cv[a][t], rv[t], v);
+        r[a][c] = v;
+      }
+      // rows K become P^-1 A[K, :]
+      if (rowK) {
+        double mine = rv[0];
+#pragma unroll
+        for (int t = 1; t < NB; ++t)
+          if (t == tr) mine = rv[t];
+#pragma unroll
+        for (int a = 0; a < TB; ++a)
+          if (a == kb) r[a][c] = mine;
+      }
     }
-    if (tx == km) {
+    // columns K become -A[:, K] P^-1, the block K x K becomes P^-1
+    if (colK) {
+      double pc[NB];
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        pc[t] = Pi[t][0];
+#pragma unroll
+        for (int u = 1; u < NB; ++u)
+          if (u == uc) pc[t] = Pi[t][u];
+      }
 #pragma unroll
       for (int c = 0; c < TB; ++c)
-        if (c == kb)
+        if (c == kb) {
 #pragma unroll
-          for (int a = 0; a < TB; ++a) r[a][c] = -cv[a] * rp;
-    }
-    if (ty == km && tx == km) {
+          for (int a = 0; a < TB; ++a) {
+            double v = 0.0;
 #pragma unroll
-      for (int a = 0; a < TB; ++a)
-        if (a == kb)
+            for (int t = 0; t < NB; ++t) v = fma(-cv[a][t], pc[t], v);
+            r[a][c] = v;
+          }
+          if (rowK) {
+            double v = pc[0];
 #pragma unroll
-          for (int c = 0; c < TB; ++c)
-            if (c == kb) r[a][c] = rp;
+            for (int t = 1; t < NB; ++t)
+              if (t == tr) v = pc[t];
+#pragma unroll
+            for (int a = 0; a < TB; ++a)
+              if (a == kb) r[a][c] = v;
+          }
+        }
     }
   }
+  // log det = sum of log pivots, one log per thread and a fixed-order block sum (the
+  // logs are kept out of the elimination loop, which is latency-bound)
+  __syncthreads();  // the last pivots were written after the last barrier of the loop
+  double ld = 0.0;
+  for (int k = threadIdx.x; k < (fail ? 0 : M); k += G * G) ld += log(piv[k]);
+  ld = gpx_block_sum(ld, red);
   if (threadIdx.x == 0) {
     if (logdet) logdet[b] = fail ? NAN : ld;
     if (info) info[b] = fail;
@@ -106,10 +184,13 @@ __global__ __launch_bounds__(G *G) void spd_gj_kernel(int M, int batch, const do
     }
 }
 
+// NB = 4 pivots per step where the registers allow it (G = 16); the 1024-thread
+// variants for M > 128 have 128 registers per lane and take one pivot per step.
 template <int G, int TB>
 int launch_gj(int M, int batch, const double *A, int lda, long sA, double *out, int ldo, long sO,
               double *logdet, int *info, hipStream_t st) {
-  hipLaunchKernelGGL((spd_gj_kernel<G, TB>), dim3(batch), dim3(G * G), 0, st, M, batch, A, lda, sA,
+  constexpr int NB = (G == 16) ? 4 : 1;
+  hipLaunchKernelGGL((spd_gj_kernel<G, TB, NB>), dim3(batch), dim3(G * G), 0, st, M, batch, A, lda, sA,
                      out, ldo, sO, logdet, info);
   GPX_CHECK_LAUNCH();
   return GPX_OK;

@@ -22,6 +22,15 @@ def nrel(a, b):
     return np.linalg.norm(np.ravel(a - b)) / max(np.linalg.norm(np.ravel(b)), 1e-300)
 
 
+def _spread(M, Q):
+    """Inducing inputs on a grid (Q <= 2), so Kuu stays well conditioned."""
+    if Q == 1:
+        return np.linspace(-3, 3, M)[:, None]
+    k = int(np.ceil(np.sqrt(M)))
+    g = np.stack(np.meshgrid(np.linspace(-2.5, 2.5, k), np.linspace(-2.5, 2.5, k)), -1).reshape(-1, 2)
+    return g[np.random.RandomState(1).permutation(len(g))[:M]]
+
+
 @pytest.mark.parametrize('N,M,Q,D', [(37, 20, 3, 2), (130, 40, 2, 5), (300, 100, 5, 5),
                                      (256, 128, 8, 32), (64, 33, 1, 17)])
 def test_fused_layer_matches_plain(N, M, Q, D):
@@ -30,7 +39,9 @@ def test_fused_layer_matches_plain(N, M, Q, D):
     from geepee_amd.layers import SGP_Layer
     say('fused layer', N, M, Q, D)
     rs = np.random.RandomState(N + M)
-    p = W.layer_params(rs, Q, D, M, ls=np.log(1.2), sf=np.log(0.8))
+    p = W.layer_params(rs, Q, D, M, ls=np.log(0.3 if Q > 2 else 0.2), sf=np.log(0.8))
+    if Q <= 2:
+        p['zu'] = _spread(M, Q)
     mx = rs.randn(N, Q)
     vx = rs.uniform(0.05, 0.5, (N, Q))
     dp = DeviceParams(_shapes(p))
@@ -58,15 +69,15 @@ def test_fused_layer_matches_plain(N, M, Q, D):
     plain = [dBhat] + list(dops.psi_bwd(mxd, vxd, L.zu, L.ls, L.sf, dops.mm(dm_all, L.Ahat), p10, dpsi2, p20))
     for a, b, name in zip(fused, plain, ('dBhat', 'dsf2', 'dell', 'dZ', 'dmx', 'dvx')):
         assert nrel(a.cpu().numpy(), b.cpu().numpy()) < 1e-12, name
-    # layer level: the rest of the chain is shared code; it passes through Kuu^-1, whose
-    # conditioning (up to ~1e6 here) amplifies last-bit differences, so the bar is the
-    # float64 parity bar
+    # layer level: the rest of the chain is shared code through Kuu^-1; the problems are
+    # chosen so that a 1e-15 relative change of psi2 moves no layer gradient by more than
+    # ~1e-12 (inducing points spread out against the lengthscale)
     L.fused = False
     g0, i0 = L.backprop_grads_lvm_mm(m0, v0, dm, dv, p10, p20, mxd, vxd, 0.5)
     L.fused = True
     g1, i1 = L.backprop_grads_lvm_mm(m1, v1, dm, dv, p11, None, mxd, vxd, 0.5)
     for k in g0:
-        assert nrel(g1[k].cpu().numpy(), g0[k].cpu().numpy()) < 1e-9, k
+        assert nrel(g1[k].cpu().numpy(), g0[k].cpu().numpy()) < 1e-10, k
     for k in i0:
         assert nrel(i1[k].cpu().numpy(), i0[k].cpu().numpy()) < 1e-11, k
 

@@ -106,6 +106,15 @@ def test_spd_reports_non_spd_member():
     L.spd_inv(dev(A), out, None, info)
     assert host(info).tolist() == [0, 3]
     assert np.allclose(host(out)[0], np.eye(6))
+    # a failing pivot inside a later pivot block, and padding of M to the block size
+    d = np.ones(13)
+    d[9] = -1.0
+    A = np.stack([np.diag(d), np.diag(np.arange(1.0, 14.0))])
+    out = dev(np.zeros_like(A))
+    info = torch.zeros(2, dtype=torch.int32, device='cuda')
+    L.spd_inv(dev(A), out, None, info)
+    assert host(info).tolist() == [10, 0]
+    assert np.allclose(host(out)[1], np.diag(1.0 / np.arange(1.0, 14.0)))
 
 
 def test_se_kernel():

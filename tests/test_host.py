@@ -64,3 +64,82 @@ def test_make_requests_shape():
     assert abs(len(reqs) / t[-1] - 1000.0) < 200.0
     for _, mid, X in reqs:
         assert 0 <= mid < 8 and X.shape[1] == 1 + mid and 1 <= X.shape[0] <= 4096
+
+
+def _unpack_R(eta, M):
+    R = np.zeros((M, M))
+    R[np.triu_indices(M)] = eta
+    R[np.diag_indices(M)] = np.exp(np.diag(R))
+    return R
+
+
+def test_layer_init_matches_reference_recipe():
+    """base_models.py:518-597: kmeans2 on x, median-distance lengthscales, sf = log 0.5,
+    theta1 = diag(1 / (0.5 U)) packed as an upper Cholesky factor with log diagonal,
+    theta2 = theta1 linspace(-1, 1); the draws come from the global legacy RNG."""
+    from geepee_amd import init
+    from scipy.cluster.vq import kmeans2
+    from scipy.spatial.distance import cdist
+    rs = np.random.RandomState(0)
+    x = rs.randn(40, 3)
+    np.random.seed(5)
+    p = init.sgp_layer(40, 7, 3, 2, True, x_train=x, key_suffix='_0')
+    np.random.seed(5)
+    zu, _ = kmeans2(x, 7, minit='points')
+    alphas = [0.5 * np.random.rand(7) for _ in range(2)]
+    assert np.array_equal(p['zu_0'], zu)
+    d = cdist(x, x)
+    assert np.allclose(p['ls_0'], np.log(np.median(d[np.triu_indices(40)]) / 2))
+    assert np.allclose(p['sf_0'], np.log(0.5))
+    for dd in range(2):
+        R = _unpack_R(p['eta1_R_0'][dd], 7)
+        assert np.allclose(R.T.dot(R), np.diag(1 / alphas[dd]))
+        assert np.allclose(p['eta2_0'][dd], np.linspace(-1, 1, 7) / alphas[dd])
+    np.random.seed(5)
+    q = init.sgp_layer(40, 7, 3, 2, True, x_train=x, key_suffix='_0')
+    assert all(np.array_equal(p[k], q[k]) for k in p)
+
+
+def test_optimal_qu_is_the_collapsed_optimum():
+    """Sigma = Kuu (Kuu + Kuf Kfu / sn2)^-1 Kuu and mu = Sigma Kuu^-1 Kuf y / sn2 (the
+    optimum of the collapsed bound), in both parameterisations."""
+    from geepee_amd import init
+    rs = np.random.RandomState(1)
+    x, y = rs.randn(30, 2), rs.randn(30, 2)
+    np.random.seed(0)
+    base = init.sgp_layer(30, 6, 2, 2, True, x_train=x)
+    sn = np.log(0.3)
+    Kuu = init._se(base['ls'], base['sf'], base['zu'], base['zu']) + 1e-5 * np.eye(6)
+    Kfu = init._se(base['ls'], base['sf'], x, base['zu'])
+    S = Kuu.dot(np.linalg.solve(Kuu + Kfu.T.dot(Kfu) / np.exp(2 * sn), Kuu))
+    mu = S.dot(np.linalg.solve(Kuu, Kfu.T.dot(y))) / np.exp(2 * sn)
+    for nat in (True, False):
+        p = init.optimal_qu(dict(base), '', x, y, 6, nat, sn)
+        for d in range(2):
+            R = _unpack_R(p['eta1_R'][d], 6)
+            t1 = R.T.dot(R)
+            if nat:
+                Sig = np.linalg.inv(np.linalg.inv(Kuu) + t1)
+                m = Sig.dot(p['eta2'][d])
+            else:
+                Sig, m = t1, p['eta2'][d]
+            assert np.allclose(Sig, S, rtol=1e-6, atol=1e-9)
+            assert np.allclose(m, mu[:, d], rtol=1e-6, atol=1e-9)
+
+
+def test_model_init_keys_and_shapes():
+    from geepee_amd import init, workloads as W
+    y, want = W.sgplvm_problem(N=60, Q=3, D=5, M=8, seed=2)
+    np.random.seed(0)
+    got = init.sgplvm(y, 3, 8)
+    assert sorted(got) == sorted(want)
+    assert all(np.shape(got[k]) == np.shape(want[k]) for k in want)
+    y, want = W.sgpssm_problem(T=80, Din=2, Dout=3, M=6, seed=3)
+    np.random.seed(0)
+    got = init.sgpssm(y, 2, 6)
+    assert sorted(got) == sorted(want)
+    assert all(np.shape(got[k]) == np.shape(want[k]) for k in want)
+    # the PCA projection is the reference's: leading eigenvectors of cov(y)
+    P = init.pca_reduce(y, 2)
+    w, V = np.linalg.eigh(np.cov(y.T))
+    assert np.allclose(np.abs(P), np.abs((y - y.mean(0)).dot(V[:, ::-1][:, :2])))
