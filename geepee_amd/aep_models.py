@@ -299,9 +299,10 @@ class SGPR(Base_Model):
             xb, yb = self.x_dev.index_select(0, idx), self.y_dev.index_select(0, idx)
         B = yb.shape[0]
         scale_logZ = -N * 1.0 / B / alpha
-        self.update_hypers(params)
+        dp = self._device_params(params)
         layer = self.sgp_layer
-        layer.compute_cavity(alpha)
+        layer.prepare(dp, alpha)
+        self.lik_layer.update_hypers(dp)
         mout, vout, kfu = layer.forward_prop_thru_cav(xb)
         dm, dv, _, scal = self.lik_layer.compute_log_Z(mout, vout, yb, alpha, scale_logZ)
         grads = layer.backprop_grads_reg(mout, vout, dm, dv, kfu, xb, alpha)
@@ -374,9 +375,7 @@ class SDGPR(Base_Model):
             xb, yb = self.x_dev.index_select(0, idx), self.y_dev.index_select(0, idx)
         B = yb.shape[0]
         scale_logZ = -N * 1.0 / B / alpha
-        self.update_hypers(params)
-        for layer in self.sgp_layers:
-            layer.compute_cavity(alpha)
+        phis = self._prepare_layers(params, alpha)
         ms, vs, p1, p2 = [], [], [], []
         for i, layer in enumerate(self.sgp_layers):
             if i == 0:
@@ -402,9 +401,32 @@ class SDGPR(Base_Model):
                 grads[k + '_%d' % i] = val
         grads['sn'] = scal[1:2]
         energy = lincomb(torch.empty(1, dtype=F64, device='cuda'), scale_logZ, scal[0:1])
-        for layer in self.sgp_layers:
-            lincomb(energy, 1.0, energy, 1.0, layer.compute_phi(alpha))
+        for phi in phis:
+            lincomb(energy, 1.0, energy, 1.0, phi)
         return energy, grads, N
+
+    def _prepare_layers(self, params, alpha):
+        """update_hypers, compute_cavity and compute_phi of every layer.  The layers'
+        chains are independent, so each runs on a stream of its own forked from the
+        current one and joined back (parallel branches when captured in a hipGraph)."""
+        dp = self._device_params(params)
+        self.lik_layer.update_hypers(dp)
+        cur = torch.cuda.current_stream()
+        if self.L == 1:
+            L = self.sgp_layers[0]
+            L.prepare(dp, alpha, key_suffix='_0')
+            return [L.compute_phi(alpha)]
+        if getattr(self, '_streams', None) is None:
+            self._streams = [torch.cuda.Stream() for _ in range(self.L)]
+        phis = []
+        for i, (layer, st) in enumerate(zip(self.sgp_layers, self._streams)):
+            st.wait_stream(cur)
+            with torch.cuda.stream(st):
+                layer.prepare(dp, alpha, key_suffix='_%d' % i)
+                phis.append(layer.compute_phi(alpha))
+        for st in self._streams:
+            cur.wait_stream(st)
+        return phis
 
     def predict_f(self, inputs):
         x = torch.as_tensor(np.asarray(inputs, dtype=np.float64), device='cuda')
@@ -465,9 +487,10 @@ class SGPLVM(Base_Model):
         yb = self.y_dev if idx is None else self.y_dev.index_select(0, idx)
         B = yb.shape[0]
         scale_logZ = -N * 1.0 / B / alpha
-        dp = self.update_hypers(params)
+        dp = self._device_params(params)
         layer = self.sgp_layer
-        layer.compute_cavity(alpha)
+        layer.prepare(dp, alpha)
+        self.lik_layer.update_hypers(dp)
         xs = self._xf.lvm_x_state(dp['x1'], dp['x2'], idx, alpha, self.prior_mean,
                                   self.prior_var, self.nat_param)
         mout, vout, psi1, psi2 = layer.forward_prop_thru_cav(xs['mcav'], xs['vcav'])
@@ -583,11 +606,15 @@ class SGPSSM(Base_Model):
         Bd, Be = e - 1 - s, e - s
         scale_dyn = -(N - 1) * 1.0 / Bd / alpha
         scale_emi = -N * 1.0 / Be / alpha
-        dp = self.update_hypers(params)
+        dp = self._device_params(params)
         dyn = self.dyn_layer
-        dyn.compute_cavity(alpha)
+        dyn.prepare(dp, alpha, key_suffix='_dynamic')
         if self.gp_emi:
-            self.emi_layer.compute_cavity(alpha)
+            self.emi_layer.prepare(dp, alpha, key_suffix='_emission')
+            self.lik_layer.update_hypers(dp, key_suffix='_emission')
+        else:
+            self.emi_layer.update_hypers(dp, key_suffix='_emission')
+        self.sn = dp['sn']
         cm, cv = self._xf().ssm_cavity(dp['x_factor_1'], dp['x_factor_2'], alpha, self.prior_mean,
                                        self.prior_var)
         tm1_m, tm1_v = self._with_control(cm[s:e - 1], cv[s:e - 1], s, e - 1, self.Dcon_dyn)

@@ -2,8 +2,8 @@
 
 ``PsiStatistics``  (mx, vx, z, ls, sf) -> (psi1, psi2)
     forward gpx_psi_fwd (geepee/kernels.py:181-240), backward gpx_psi_bwd
-    (kernels.py:302-309, 355-444).  psi2's incoming gradient is symmetrised by the
-    kernel, as the reference does at kernels.py:415-418.
+    (kernels.py:302-309, 355-444).  psi2's incoming gradient is symmetrised first
+    (gpx_sym), as the reference does at kernels.py:415-418.
 
 ``MomentMatch``  (mx, vx, z, ls, sf, Ahat, Bhat) -> (m, v)
     the uncertain-input projection of geepee/aep_models.py:182-199,
@@ -47,7 +47,9 @@ class PsiStatistics(torch.autograd.Function):
     def backward(ctx, dpsi1, dpsi2):
         mx, vx, z, ls, sf, psi1, psi2 = ctx.saved_tensors
         dpsi1 = _zeros(psi1.shape, psi1) if dpsi1 is None else _c(dpsi1)
-        dpsi2 = _zeros(psi2.shape, psi2) if dpsi2 is None else _c(dpsi2)
+        # gpx_psi_bwd takes a symmetric dL/dpsi2; psi2 is symmetric, so the symmetrised
+        # incoming gradient gives the same adjoint (kernels.py:415-418)
+        dpsi2 = _zeros(psi2.shape, psi2) if dpsi2 is None else dops.sym(_c(dpsi2), 0.5)
         dsf2, dell, dz, dmx, dvx = dops.psi_bwd(mx, vx, z, ls, sf, dpsi1, psi1, dpsi2, psi2)
         dls, dsf = _loghyper_grads(ls, sf, dell, dsf2, _zeros(1, sf))
         return dmx, dvx, dz, dls, dsf.reshape(sf.shape)

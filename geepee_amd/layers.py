@@ -58,6 +58,40 @@ class SGP_Layer(object):
         self.compute_kuu()
         self.update_posterior()
 
+    def prepare(self, params, alpha, key_suffix=''):
+        """update_hypers followed by compute_cavity(alpha).  In the natural
+        parameterisation both precisions, Kuu^-1 + theta1 and Kuu^-1 + beta theta1, are
+        known once Kuu^-1 is, so their 2D inverses run as one batched SPD launch."""
+        if not self.nat_param:
+            self.update_hypers(params, key_suffix)
+            self.compute_cavity(alpha)
+            return
+        self.ls = params['ls' + key_suffix]
+        self.sf = params['sf' + key_suffix]
+        self.zu = params['zu' + key_suffix]
+        self.theta_2 = params['eta2' + key_suffix]
+        self.theta_1_R = dops.eta_to_R(params['eta1_R' + key_suffix], self.M)
+        self.theta_1 = mm(self.theta_1_R, self.theta_1_R, tA=True)
+        self.compute_kuu()
+        D, M = self.Dout, self.M
+        beta = (self.N - alpha) * 1.0 / self.N
+        Ki = self.Kuuinv
+        prec = dops.empty(2 * D, M, M, like=Ki)
+        lincomb(prec[:D], 1.0, Ki, 1.0, self.theta_1)
+        lincomb(prec[D:], 1.0, Ki, beta, self.theta_1)
+        cov, ld, info = dops.spd_inv(prec)
+        self.Suinv, self.Suhatinv = prec[:D], prec[D:]
+        self.Su, self.Suhat = cov[:D], cov[D:]
+        self.ld_Suinv, self.ld_Suhatinv = ld[:D], ld[D:]
+        self.info_Su, self.info_Suhat = info[:D], info[D:]
+        self.mu = mm(self.Su, _col(self.theta_2)).reshape(D, M)
+        self._ld_post, self._sgn_post = self.ld_Suinv, -1.0
+        self.Splusmm = self.Su.clone()
+        _add_outer(self.Splusmm, self.mu, self.mu, 1.0)
+        self._pred_ready = False
+        self.muhat = mm(self.Suhat, _col(self.theta_2), alpha=beta).reshape(D, M)
+        self._cavity_tail()
+
     def compute_kuu(self):
         self.Kuu = dops.se(self.zu, self.zu, self.ls, self.sf, jitter=JITTER)
         self.Kuuinv, self.ld_Kuu, self.info_Kuu = dops.spd_inv(self.Kuu)
@@ -107,6 +141,11 @@ class SGP_Layer(object):
             self.Suhat, self.ld_Suhatinv, self.info_Suhat = dops.spd_inv(self.Suhatinv)
             f2 = mm(self.Suinv, _col(self.mu))
             self.muhat = mm(self.Suhat, f2, alpha=beta).reshape(D, M)
+        self._cavity_tail()
+
+    def _cavity_tail(self):
+        """Ahat, Bhat_sto, Bhat_det from muhat, Suhat (aep_models.py:535-546)."""
+        Ki = self.Kuuinv
         self.Ahat = mm(self.muhat, Ki, tB=True)
         self.Splusmmhat = self.Suhat.clone()
         _add_outer(self.Splusmmhat, self.muhat, self.muhat, 1.0)

@@ -73,3 +73,26 @@ def test_sgpr_eval_then_200_adam_steps(graph):
     assert nrel(got - vec0, xo - vec0) < 1e-7, nrel(got - vec0, xo - vec0)
     assert nrel(got, xo) < 1e-9
     assert fs[-1] < fs[0]
+
+
+def test_optimise_from_init_hypers_matches_oracle():
+    """optimise() with the reference defaults (reinit_hypers=True: init_hypers, then the
+    host Adam loop over objective_function) against the oracle from the same start."""
+    from geepee_amd import aep_models as G
+    say('optimise from init_hypers')
+    x, y, _ = W.sgpr_problem(N=200, M=15, Din=2, Dout=1, seed=4)
+    model = G.SGPR(x, y, 15)
+    np.random.seed(1)
+    p0 = model.init_hypers(y)
+    np.random.seed(1)
+    final = model.optimise(method='adam', maxiter=20, alpha=0.5, adam_lr=1e-3, disp=False)
+    vec0, args = O.flatten_dict(p0)
+
+    def f(vec):
+        e, g = O.sgpr_objective(x, y, O.unflatten_dict(vec, args), 15, 200, alpha=0.5)
+        return e, O.flatten_dict(g)[0]
+
+    xo, fs = O.adam(f, vec0, 20, step_size=1e-3)
+    got = O.flatten_dict(final)[0]
+    assert nrel(got - vec0, xo - vec0) < 1e-7
+    assert fs[-1] < fs[0]
