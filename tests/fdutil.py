@@ -19,14 +19,16 @@ def fd4(f, params, key, idx, h):
     return (-vals[0] + 8 * vals[1] - 8 * vals[2] + vals[3]) / (12 * h)
 
 
-def check_grads(f, params, h=2e-3, rtol=2e-6, max_per_key=12, seed=0, skip=()):
+def check_grads(f, params, h=2e-3, rtol=2e-6, max_per_key=12, seed=0, skip=(), scale='key'):
     """Compare the analytic gradient dict of ``f(params) -> (energy, grads)`` with
     fourth-order finite differences on up to ``max_per_key`` coordinates per key.
-    The error of each coordinate is measured against the key's gradient norm
-    (normwise), so cancelling tiny coordinates do not fail on rounding."""
+    The error of each coordinate is measured against the largest gradient entry of its
+    key (scale='key') or of the whole gradient (scale='global'), so cancelling tiny
+    coordinates do not fail on rounding."""
     rs = np.random.RandomState(seed)
     _, g = f(params)
     assert sorted(g.keys()) == sorted(params.keys())
+    gmax = max(float(np.max(np.abs(np.asarray(v)))) for v in g.values())
     report = []
     for key in sorted(params):
         if key in skip:
@@ -37,10 +39,10 @@ def check_grads(f, params, h=2e-3, rtol=2e-6, max_per_key=12, seed=0, skip=()):
         if len(idxs) > max_per_key:
             idxs = [idxs[i] for i in rs.choice(len(idxs), max_per_key, replace=False)]
         gk = np.asarray(g[key], dtype=np.float64)
-        scale = max(np.max(np.abs(gk)), 1e-8)
+        den = max(np.max(np.abs(gk)) if scale == 'key' else gmax, 1e-8)
         for idx in idxs:
             num = fd4(f, params, key, idx, h)
-            err = abs(num - gk[idx]) / scale
+            err = abs(num - gk[idx]) / den
             report.append((err, key, idx, gk[idx], num))
     report.sort(key=lambda t: -t[0])
     worst = report[0]
