@@ -28,6 +28,8 @@ SIGNATURES = {
     'gpx_dtrace_workspace': (_l, [_i, _i]),
     'gpx_dtrace_kzz': (_i, [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
     'gpx_colsum': (_i, [_i, _i, _p, _d, _d, _p, _p]),
+    'gpx_colsum_workspace': (_l, [_i, _i]),
+    'gpx_colsum_tall': (_i, [_i, _i, _p, _d, _d, _p, _p, _p]),
     'gpx_eta_to_R': (_i, [_i, _i, _p, _p, _p]),
     'gpx_pack_dR': (_i, [_i, _i, _p, _p, _p, _p]),
     'gpx_lincomb': (_i, [_l, _d, _p, _l, _d, _p, _l, _d, _p, _l, _p, _p]),

@@ -8,7 +8,7 @@ sums), so any of them meets the same tolerance.
 
 Knobs
   fused forward   nsub (16*nsub points per workgroup), splits_f (pair-tile splits)
-  fused backward  splits_b (pair-tile splits of the per-point pass),
+  fused backward  splits_b (column-tile splits of the per-point pass),
                   nsplit (point ranges of the per-pair pass)
   gemm            tile (32 or 64 rows/cols per workgroup), split-K factor
 
@@ -117,9 +117,9 @@ def fused_candidates(N, M):
     ntiles = T * (T + 1) // 2
     fwd = [{'nsub': ns, 'splits_f': s} for ns in (1, 2, 4) for s in _pow2_upto(1, min(ntiles, 64))
            if -(-N // (16 * ns)) * s >= 64]
-    bwd_points = [s for s in _pow2_upto(1, min(ntiles, 64)) if -(-N // 16) * s >= 64]
+    bwd_points = [s for s in _pow2_upto(1, min(T, 64)) if -(-N // 16) * s >= 64]
     bwd_pairs = [s for s in _pow2_upto(1, min(-(-N // 16), 64)) if ntiles * 4 * s >= 64]
-    return fwd or [{'nsub': 1, 'splits_f': min(ntiles, 64)}], bwd_points or [min(ntiles, 64)], \
+    return fwd or [{'nsub': 1, 'splits_f': min(ntiles, 64)}], bwd_points or [min(T, 64)], \
         bwd_pairs or [min(-(-N // 16), 64)]
 
 

@@ -194,9 +194,10 @@ __global__ void sym_pairmajor_kernel(int D, int M, const double *__restrict__ B,
 
 // ---------------------------------------------------------------------------
 // backward, per-point pass ([pair x n] orientation): for 16 points per block and the
-// tiles of split s, T_na (row sums + column sums of off-diagonal tiles) and V_nq.
-// Each inducing index a is owned by wave (a % 16) % 4 for row sums; column sums are
-// merged in wave order after each tile, so every sum has a fixed order.
+// column tiles B of split s (B = s, s + S, ...), visited column-major with A <= B:
+// T_na (row sums, plus column sums of the off-diagonal tiles) and V_nq.  Row a is
+// owned by wave (a % 16) % 4, so row sums need no barrier; a column tile's sums are
+// kept in registers over its A tiles and merged once, in wave order.
 // ---------------------------------------------------------------------------
 template <int DS>
 __global__ __launch_bounds__(256) void fused_bwd_points_kernel(
@@ -207,7 +208,7 @@ __global__ __launch_bounds__(256) void fused_bwd_points_kernel(
   __shared__ double zs[FM_MAX][FQ_MAX + 1];
   __shared__ double lam[FQ_MAX], il4[FQ_MAX];
   __shared__ double Tacc[16][FM_MAX + 1];
-  __shared__ double vred[4][16][17];
+  __shared__ double cst[4][16][17];  // column-sum staging; reused for the V reduction
   const int n0 = blockIdx.x * 16, split = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int e = tid; e < M * Q; e += 256) zs[e / Q][e % Q] = z[e];
@@ -237,76 +238,75 @@ __global__ __launch_bounds__(256) void fused_bwd_points_kernel(
   }
   const double sf2 = exp(2.0 * sf[0]), sf4 = sf2 * sf2;
   gpx_d4 vacc = gpx_d4{0.0, 0.0, 0.0, 0.0};
-  const int T = (M + 15) / 16, ntiles = T * (T + 1) / 2;
-  for (int t = split; t < ntiles; t += S) {
-    int A, B;
-    decode_tile(t, T, A, B);
-    const bool off = (A != B);
-    const double w = off ? 2.0 : 1.0;
+  const int T = (M + 15) / 16;
+  for (int B = split; B < T; B += S) {
     double colp[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int i = wave; i < 16; i += 4) {
-      const int a = 16 * A + i;
-      if (a >= M) break;
-      const int ja = 16 * B + (lane & 15);
-      gpx_d4 e = gpx_d4{0.0, 0.0, 0.0, 0.0};
+    for (int A = 0; A <= B; ++A) {
+      const bool off = (A != B);
+      const double w = off ? 2.0 : 1.0;
+      for (int i = wave; i < 16; i += 4) {
+        const int a = 16 * A + i;
+        if (a >= M) break;
+        const int ja = 16 * B + (lane & 15);
+        gpx_d4 e = gpx_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int s = 0; s < FKS_MAX; ++s) {
-        if (s < KS) {
-          const double af = (ja < M) ? pair_feature((lane >> 4) + 4 * s, Q, zs[a], zs[ja], il4) : 0.0;
-          e = gpx_mfma16(af, bf[s], e);
-        }
-      }
-      gpx_d4 g = gpx_d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int s = 0; s < DS; ++s) {
-        const int d = (lane >> 4) + 4 * s;
-        const double bsv = (ja < M && d < D) ? Bp[((long)a * M + ja) * D + d] : 0.0;
-        g = gpx_mfma16(bsv, dvr[s], g);
-      }
-      double L[4];
-      double rs = 0.0;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int b = 16 * B + (lane >> 4) + 4 * r;
-        const double p = (b < M && nvalid) ? sf4 * exp(e[r]) : 0.0;
-        L[r] = g[r] * p;
-        rs += L[r];
-        colp[r] += L[r];
-      }
-      rs += __shfl_xor(rs, 16, 64);
-      rs += __shfl_xor(rs, 32, 64);
-      if ((lane >> 4) == 0) Tacc[lane & 15][a] += rs;
-      // V[q x n] += w z_aq z_bq L[pair x n]
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int q = lane & 15, b = 16 * B + (lane >> 4) + 4 * r;
-        const double zz = (q < Q && b < M) ? w * zs[a][q] * zs[b][q] : 0.0;
-        vacc = gpx_mfma16(zz, L[r], vacc);
-      }
-    }
-    if (off) {
-      for (int ww = 0; ww < 4; ++ww) {
-        __syncthreads();
-        if (wave == ww) {
-          // column sums of this wave's rows: each lane holds distinct (b, n) entries
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int b = 16 * B + (lane >> 4) + 4 * r;
-            if (b < M) Tacc[lane & 15][b] += colp[r];
+        for (int s = 0; s < FKS_MAX; ++s) {
+          if (s < KS) {
+            const double af = (ja < M) ? pair_feature((lane >> 4) + 4 * s, Q, zs[a], zs[ja], il4) : 0.0;
+            e = gpx_mfma16(af, bf[s], e);
           }
         }
+        gpx_d4 g = gpx_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < DS; ++s) {
+          const int d = (lane >> 4) + 4 * s;
+          const double bsv = (ja < M && d < D) ? Bp[((long)a * M + ja) * D + d] : 0.0;
+          g = gpx_mfma16(bsv, dvr[s], g);
+        }
+        double L[4];
+        double rs = 0.0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int b = 16 * B + (lane >> 4) + 4 * r;
+          const double p = (b < M && nvalid) ? sf4 * exp(e[r]) : 0.0;
+          L[r] = g[r] * p;
+          rs += L[r];
+          if (off) colp[r] += L[r];
+        }
+        rs += __shfl_xor(rs, 16, 64);
+        rs += __shfl_xor(rs, 32, 64);
+        if ((lane >> 4) == 0) Tacc[lane & 15][a] += rs;
+        // V[q x n] += w z_aq z_bq L[pair x n]
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int q = lane & 15, b = 16 * B + (lane >> 4) + 4 * r;
+          const double zz = (q < Q && b < M) ? w * zs[a][q] * zs[b][q] : 0.0;
+          vacc = gpx_mfma16(zz, L[r], vacc);
+        }
       }
     }
-    __syncthreads();
+    if (B > 0) {
+      // merge the column sums of tile column B: stage per wave, then one thread per
+      // (point, column) adds the four waves in order
+#pragma unroll
+      for (int r = 0; r < 4; ++r) cst[wave][lane & 15][(lane >> 4) + 4 * r] = colp[r];
+      __syncthreads();
+      {
+        const int nn = tid >> 4, bb = tid & 15, b = 16 * B + bb;
+        if (b < M) Tacc[nn][b] += cst[0][nn][bb] + cst[1][nn][bb] + cst[2][nn][bb] + cst[3][nn][bb];
+      }
+      __syncthreads();
+    }
   }
+  __syncthreads();
   // V: lane holds V[q = (lane>>4) + 4r][n = lane & 15]
 #pragma unroll
-  for (int r = 0; r < 4; ++r) vred[wave][(lane >> 4) + 4 * r][lane & 15] = vacc[r];
+  for (int r = 0; r < 4; ++r) cst[wave][(lane >> 4) + 4 * r][lane & 15] = vacc[r];
   __syncthreads();
   for (int idx = tid; idx < 16 * 16; idx += 256) {
     const int q = idx >> 4, nn = idx & 15;
     if (q < Q && n0 + nn < N)
-      Vpart[((long)split * N + n0 + nn) * Q + q] = vred[0][q][nn] + vred[1][q][nn] + vred[2][q][nn] + vred[3][q][nn];
+      Vpart[((long)split * N + n0 + nn) * Q + q] = cst[0][q][nn] + cst[1][q][nn] + cst[2][q][nn] + cst[3][q][nn];
   }
   for (int e = tid; e < 16 * M; e += 256) {
     const int nn = e / M, a = e % M;
@@ -314,22 +314,39 @@ __global__ __launch_bounds__(256) void fused_bwd_points_kernel(
   }
 }
 
+// Per-point features of the per-pair pass, formed once per launch: for each point n,
+// pf[n][k] (k < 4*FKS_MAX, the exponent operand) and dq[n][j] = [1 | d_n | 0...] (16).
+constexpr int PF_W = 4 * FKS_MAX + 16;
+
+__global__ __launch_bounds__(256) void fused_point_feat_kernel(int N, int Q, const double *__restrict__ mx,
+                                                               const double *__restrict__ vx,
+                                                               const double *__restrict__ ls,
+                                                               double *__restrict__ pf) {
+  __shared__ double lam[FQ_MAX];
+  if (threadIdx.x < Q) lam[threadIdx.x] = exp(2.0 * ls[threadIdx.x]);
+  __syncthreads();
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  double mu[FQ_MAX], dd[FQ_MAX];
+  const double c = point_consts(n, N, Q, mx, vx, lam, mu, dd);
+  double *o = pf + (long)n * PF_W;
+  for (int k = 0; k < 4 * FKS_MAX; ++k) o[k] = point_feature(k, Q, mu, dd, c);
+  o[4 * FKS_MAX] = 1.0;
+  for (int q = 0; q < 15; ++q) o[4 * FKS_MAX + 1 + q] = (q < Q) ? dd[q] : 0.0;
+}
+
 // ---------------------------------------------------------------------------
 // backward, per-pair pass ([n x pair] orientation): each wave owns one row block
-// (a, B-tile) and sums over the points of its range:
+// (a, B-tile) and sums over the points of its range, reading the precomputed point
+// features and dv straight from memory (no LDS, no barriers):
 //   dBhat[d, a, b] += sum_n dv_nd psi2_nab,  LW[a, b, q'] += sum_n [1 | d_nq'] L_nab
 // ---------------------------------------------------------------------------
 template <int DB, int DS>
 __global__ __launch_bounds__(256) void fused_bwd_pairs_kernel(
-    int N, int M, int Q, int D, const double *__restrict__ mx, const double *__restrict__ vx,
-    const double *__restrict__ z, const double *__restrict__ ls, const double *__restrict__ sf,
-    const double *__restrict__ Bp, const double *__restrict__ dv, int nsplit, int nchunk,
-    double *__restrict__ dBpart, double *__restrict__ LWpart) {
-  __shared__ double zs[FM_MAX][FQ_MAX + 1];
-  __shared__ double lam[FQ_MAX], il4[FQ_MAX];
-  __shared__ double pf[16][4 * FKS_MAX + 1];  // point features
-  __shared__ double dvs[16][DB * 16 + 1];
-  __shared__ double dq1[16][FQ_MAX + 2];      // [1 | d]
+    int N, int M, int Q, int D, const double *__restrict__ z, const double *__restrict__ ls,
+    const double *__restrict__ sf, const double *__restrict__ Bp, const double *__restrict__ dv,
+    const double *__restrict__ pf, int nsplit, int nchunk, double *__restrict__ dBpart,
+    double *__restrict__ LWpart) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int T = (M + 15) / 16;
   const int rb = blockIdx.x * 4 + wave;       // row block id: tile * 16 + i
@@ -337,23 +354,25 @@ __global__ __launch_bounds__(256) void fused_bwd_pairs_kernel(
   int A, B;
   decode_tile(t, T, A, B);
   const int a = 16 * A + i;
-  const bool active = a < M;
+  if (a >= M) return;                         // whole wave; no barriers below
   const int ns = blockIdx.y;
   const int nbeg = ns * nchunk, nend = min(N, nbeg + nchunk);
-  for (int e = tid; e < M * Q; e += 256) zs[e / Q][e % Q] = z[e];
-  if (tid < Q) {
-    const double l = exp(2.0 * ls[tid]);
-    lam[tid] = l;
-    il4[tid] = 0.25 / l;
-  }
-  __syncthreads();
   const int KS = (2 * Q + 2 + 3) / 4;
   const int jb = 16 * B + (lane & 15);       // this lane's pair column b
-  const bool bvalid = active && jb < M;
+  const bool bvalid = jb < M;
+  double il4[FQ_MAX];
+#pragma unroll
+  for (int q = 0; q < FQ_MAX; ++q) il4[q] = (q < Q) ? 0.25 * exp(-2.0 * ls[q]) : 0.0;
+  double za[FQ_MAX], zb[FQ_MAX];
+#pragma unroll
+  for (int q = 0; q < FQ_MAX; ++q) {
+    za[q] = (q < Q) ? z[a * Q + q] : 0.0;
+    zb[q] = (q < Q && bvalid) ? z[jb * Q + q] : 0.0;
+  }
   double af[FKS_MAX], bs[DS];
 #pragma unroll
   for (int s = 0; s < FKS_MAX; ++s)
-    af[s] = (s < KS && bvalid) ? pair_feature((lane >> 4) + 4 * s, Q, zs[active ? a : 0], zs[bvalid ? jb : 0], il4) : 0.0;
+    af[s] = (s < KS && bvalid) ? pair_feature((lane >> 4) + 4 * s, Q, za, zb, il4) : 0.0;
 #pragma unroll
   for (int s = 0; s < DS; ++s) {
     const int d = (lane >> 4) + 4 * s;
@@ -364,46 +383,36 @@ __global__ __launch_bounds__(256) void fused_bwd_pairs_kernel(
 #pragma unroll
   for (int db = 0; db < DB; ++db) dacc[db] = gpx_d4{0.0, 0.0, 0.0, 0.0};
   for (int n0 = nbeg; n0 < nend; n0 += 16) {
-    // stage point data of 16 points
-    if (tid < 16) {
-      const int n = n0 + tid;
-      double mu[FQ_MAX], dd[FQ_MAX];
-      const double c = point_consts(n, N, Q, mx, vx, lam, mu, dd);
-      for (int k = 0; k < 4 * FKS_MAX; ++k) pf[tid][k] = (n < nend) ? point_feature(k, Q, mu, dd, c) : 0.0;
-      dq1[tid][0] = (n < nend) ? 1.0 : 0.0;
-      for (int q = 0; q < Q; ++q) dq1[tid][1 + q] = (n < nend) ? dd[q] : 0.0;
-      for (int q = Q + 1; q < FQ_MAX + 2; ++q) dq1[tid][q] = 0.0;
+    const int nA = n0 + (lane & 15);         // point of this lane as an A-operand row
+    const bool vA = nA < nend;
+    gpx_d4 e = gpx_d4{0.0, 0.0, 0.0, 0.0}, g = gpx_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < FKS_MAX; ++s)
+      if (s < KS) e = gpx_mfma16(vA ? pf[(long)nA * PF_W + (lane >> 4) + 4 * s] : 0.0, af[s], e);
+#pragma unroll
+    for (int s = 0; s < DS; ++s) {
+      const int d = (lane >> 4) + 4 * s;
+      g = gpx_mfma16((vA && d < D) ? dv[(long)nA * D + d] : 0.0, bs[s], g);
     }
-    for (int e = tid; e < 16 * DB * 16; e += 256) {
-      const int nn = e / (DB * 16), d = e % (DB * 16), n = n0 + nn;
-      dvs[nn][d] = (n < nend && d < D) ? dv[(long)n * D + d] : 0.0;
+    double p[4], L[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int nn = (lane >> 4) + 4 * r;
+      p[r] = (n0 + nn < nend && bvalid) ? sf4 * exp(e[r]) : 0.0;
+      L[r] = g[r] * p[r];
     }
-    __syncthreads();
-    if (active) {
-      gpx_d4 e = gpx_d4{0.0, 0.0, 0.0, 0.0}, g = gpx_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int s = 0; s < FKS_MAX; ++s)
-        if (s < KS) e = gpx_mfma16(pf[lane & 15][(lane >> 4) + 4 * s], af[s], e);
+    for (int r = 0; r < 4; ++r) {
+      const int nr = n0 + (lane >> 4) + 4 * r;  // point of this k-slice
+      const bool vr = nr < nend;
 #pragma unroll
-      for (int s = 0; s < DS; ++s) g = gpx_mfma16(dvs[lane & 15][(lane >> 4) + 4 * s], bs[s], g);
-      double p[4], L[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int nn = (lane >> 4) + 4 * r;
-        p[r] = (n0 + nn < nend && bvalid) ? sf4 * exp(e[r]) : 0.0;
-        L[r] = g[r] * p[r];
+      for (int db = 0; db < DB; ++db) {
+        const int d = 16 * db + (lane & 15);
+        dacc[db] = gpx_mfma16((vr && d < D) ? dv[(long)nr * D + d] : 0.0, p[r], dacc[db]);
       }
-#pragma unroll
-      for (int db = 0; db < DB; ++db)
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          dacc[db] = gpx_mfma16(dvs[(lane >> 4) + 4 * r][16 * db + (lane & 15)], p[r], dacc[db]);
-#pragma unroll
-      for (int r = 0; r < 4; ++r) lacc = gpx_mfma16(dq1[(lane >> 4) + 4 * r][lane & 15], L[r], lacc);
+      lacc = gpx_mfma16(vr ? pf[(long)nr * PF_W + 4 * FKS_MAX + (lane & 15)] : 0.0, L[r], lacc);
     }
-    __syncthreads();
   }
-  if (!active) return;
   // dacc[db]: lane holds dBhat[d = 16db + (lane>>4) + 4r][pair b = jb]
 #pragma unroll
   for (int db = 0; db < DB; ++db)
@@ -588,6 +597,7 @@ long gpx_fused_bwd_workspace(int N, int M, int Q, int D, int splits, int nsplit)
   s += (long)splits * N * M + (long)splits * N * Q;                           // Tpart, Vpart
   s += 3L * N * M + (long)N * Q + 2L * N;                                     // dpsi1, T, Lp, H, s0
   s += (long)N * (Q + 1) + 4L * N * Q + 4L * M * Q + 2;                       // Dn, Pn, En, TP, LE, cols0
+  s += (long)N * PF_W;                                                        // point features
   return s;
 }
 
@@ -609,7 +619,7 @@ int gpx_fused_mm_bwd(int N, int M, int Q, int D, const double *mx, const double 
     return GPX_EINVAL;
   hipStream_t st = (hipStream_t)stream;
   const int T = (M + 15) / 16, ntiles = T * (T + 1) / 2;
-  if (splits > ntiles) return GPX_EINVAL;
+  if (splits > T) return GPX_EINVAL;  // the per-point pass splits over column tiles
   const long MM = (long)M * M;
   double *dBpart = work;
   double *LWpart = dBpart + (long)nsplit * D * MM;
@@ -627,15 +637,18 @@ int gpx_fused_mm_bwd(int N, int M, int Q, int D, const double *mx, const double 
   double *TP = En + 2L * N * Q;
   double *LE = TP + 2L * M * Q;
   double *cols0 = LE + 2L * M * Q;
+  double *pf = cols0 + 2;
+  hipLaunchKernelGGL(fused_point_feat_kernel, dim3(gpx_cdiv(N, 256)), dim3(256), 0, st, N, Q, mx, vx, ls, pf);
+  GPX_CHECK_LAUNCH();
   // per-pair pass over point ranges
   const int nchunk = ((N + nsplit - 1) / nsplit + 15) / 16 * 16;
   dim3 gp(ntiles * 4, nsplit);
   if (D <= 16)
-    hipLaunchKernelGGL((fused_bwd_pairs_kernel<1, 4>), gp, dim3(256), 0, st, N, M, Q, D, mx, vx, z, ls, sf, Bp,
-                       dv, nsplit, nchunk, dBpart, LWpart);
+    hipLaunchKernelGGL((fused_bwd_pairs_kernel<1, 4>), gp, dim3(256), 0, st, N, M, Q, D, z, ls, sf, Bp, dv,
+                       (const double *)pf, nsplit, nchunk, dBpart, LWpart);
   else
-    hipLaunchKernelGGL((fused_bwd_pairs_kernel<2, 8>), gp, dim3(256), 0, st, N, M, Q, D, mx, vx, z, ls, sf, Bp,
-                       dv, nsplit, nchunk, dBpart, LWpart);
+    hipLaunchKernelGGL((fused_bwd_pairs_kernel<2, 8>), gp, dim3(256), 0, st, N, M, Q, D, z, ls, sf, Bp, dv,
+                       (const double *)pf, nsplit, nchunk, dBpart, LWpart);
   GPX_CHECK_LAUNCH();
   {
     const long n = MM * (D + Q + 1);

@@ -161,6 +161,76 @@ __global__ void splitk_reduce(int m, int n, int batch, int splits, double alpha,
   }
 }
 
+// Short contractions (k <= FULLK_MAX): every operand element of the block's 32-row
+// panel of op(A) and 32-column panel of op(B) is fetched in one round into LDS (one
+// memory latency instead of one per K slice), then the 2x2 waves run their MFMAs over
+// the whole K.  Same output layout and epilogue as gemm_kernel; no split-K.
+constexpr int FULLK_MAX = 128;
+
+__global__ __launch_bounds__(256) void gemm_fullk_kernel(
+    int m, int n, int k, int tA, int tB, double alpha,
+    const double *__restrict__ A, int lda, long sA,
+    const double *__restrict__ B, int ldb, long sB,
+    double beta, double *__restrict__ C, int ldc, long sC, int batch) {
+  __shared__ double As[FULLK_MAX][33];
+  __shared__ double Bs[FULLK_MAX][33];
+  const int bidx = blockIdx.z;
+  if (bidx >= batch) return;
+  A += bidx * sA;
+  B += bidx * sB;
+  const int m0 = blockIdx.y * 32, n0 = blockIdx.x * 32;
+  const int kp = (k + 3) & ~3;
+  const int tid = threadIdx.x;
+  // all loads first (up to 16 + 16 per thread in flight), then the LDS stores
+  constexpr int U = 32 * FULLK_MAX / 256;
+  const int tot = 32 * kp;
+  double ra[U], rb[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int e = tid + 256 * u;
+    ra[u] = 0.0;
+    rb[u] = 0.0;
+    if (e < tot) {
+      int mm, kk, nn, k2;
+      if (tA) { kk = e >> 5; mm = e & 31; } else { mm = e / kp; kk = e - mm * kp; }
+      if (tB) { nn = e / kp; k2 = e - nn * kp; } else { k2 = e >> 5; nn = e & 31; }
+      const int gm = m0 + mm, gn = n0 + nn;
+      if (gm < m && kk < k) ra[u] = tA ? A[(long)kk * lda + gm] : A[(long)gm * lda + kk];
+      if (gn < n && k2 < k) rb[u] = tB ? B[(long)gn * ldb + k2] : B[(long)k2 * ldb + gn];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int e = tid + 256 * u;
+    if (e < tot) {
+      int mm, kk, nn, k2;
+      if (tA) { kk = e >> 5; mm = e & 31; } else { mm = e / kp; kk = e - mm * kp; }
+      if (tB) { nn = e / kp; k2 = e - nn * kp; } else { k2 = e >> 5; nn = e & 31; }
+      As[kk][mm] = ra[u];
+      Bs[k2][nn] = rb[u];
+    }
+  }
+  __syncthreads();
+  const int lane = tid & 63, wave = tid >> 6, wm = wave >> 1, wn = wave & 1;
+  gpx_d4 acc = gpx_d4{0.0, 0.0, 0.0, 0.0};
+  for (int kq = 0; kq < kp; kq += 4) {
+    const double a = As[kq + (lane >> 4)][wm * 16 + (lane & 15)];
+    const double b = Bs[kq + (lane >> 4)][wn * 16 + (lane & 15)];
+    acc = gpx_mfma16(a, b, acc);
+  }
+  C += bidx * sC;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int gm = m0 + wm * 16 + (lane >> 4) + 4 * r;
+    const int gn = n0 + wn * 16 + (lane & 15);
+    if (gm < m && gn < n) {
+      double *c = C + (long)gm * ldc + gn;
+      const double v = alpha * acc[r];
+      *c = (beta == 0.0) ? v : v + beta * *c;
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -183,6 +253,15 @@ int gpx_gemm_batched(int transA, int transB, int m, int n, int k, double alpha,
   if (k == 0) {  // C = beta * C
     hipLaunchKernelGGL(splitk_reduce, dim3(gpx_cdiv((long)batch * m * n, 256) > 4096 ? 4096 : gpx_cdiv((long)batch * m * n, 256)),
                        dim3(256), 0, st, m, n, batch, 0, 0.0, (const double *)nullptr, beta, C, ldc, strideC);
+    GPX_CHECK_LAUNCH();
+    return GPX_OK;
+  }
+  if (splits == 1 && k <= FULLK_MAX && tile != 64 &&
+      (tile == 32 || (long)gpx_cdiv(m, 64) * gpx_cdiv(n, 64) * batch < 256)) {
+    if (batch > 65535) return GPX_EINVAL;
+    dim3 grid(gpx_cdiv(n, 32), gpx_cdiv(m, 32), batch);
+    hipLaunchKernelGGL(gemm_fullk_kernel, grid, dim3(256), 0, st, m, n, k, transA, transB, alpha, A, lda, strideA,
+                       B, ldb, strideB, beta, C, ldc, strideC, batch);
     GPX_CHECK_LAUNCH();
     return GPX_OK;
   }

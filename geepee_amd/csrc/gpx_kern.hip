@@ -182,6 +182,25 @@ __global__ __launch_bounds__(256) void colsum_tall_kernel(int rows, int cols, co
   if (threadIdx.x == 0) out[j] = (beta == 0.0) ? alpha * s : alpha * s + beta * out[j];
 }
 
+// Stage 1 of the two-stage column sum: block (j, c) sums rows [c*chunk, (c+1)*chunk) of
+// column j into part[c * cols + j] (strided loads, fixed-order block reduction).
+__global__ __launch_bounds__(256) void colsum_chunk_kernel(int rows, int cols, int chunk,
+                                                           const double *__restrict__ P,
+                                                           double *__restrict__ part) {
+  __shared__ double red[16];
+  const int j = blockIdx.x, c = blockIdx.y;
+  const long r0 = (long)c * chunk, r1 = min((long)rows, r0 + chunk);
+  double s0 = 0.0, s1 = 0.0;
+  long i = r0 + threadIdx.x;
+  for (; i + 256 < r1; i += 512) {
+    s0 += P[i * cols + j];
+    s1 += P[(i + 256) * cols + j];
+  }
+  if (i < r1) s0 += P[i * cols + j];
+  const double s = gpx_block_sum(s0 + s1, red);
+  if (threadIdx.x == 0) part[(long)c * cols + j] = s;
+}
+
 __global__ void colsum_kernel(int rows, int cols, const double *__restrict__ P, double alpha,
                               double beta, double *__restrict__ out) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -683,6 +702,34 @@ int gpx_colsum(int rows, int cols, const double *P, double alpha, double beta, d
   }
   hipLaunchKernelGGL(colsum_kernel, dim3(gpx_cdiv(cols, 256)), dim3(256), 0, (hipStream_t)stream, rows, cols,
                      P, alpha, beta, out);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+// Chunks of the two-stage column sum and its workspace (doubles).
+static int colsum_chunks(int rows) {
+  int c = (rows + 2047) / 2048;
+  return c > 256 ? 256 : c;
+}
+
+long gpx_colsum_workspace(int rows, int cols) {
+  if (rows < 0 || cols < 0) return -1;
+  return (long)colsum_chunks(rows) * cols;
+}
+
+// out[j] = alpha sum_i P[i, j] + beta out[j] for tall P: chunked partial sums into
+// `work`, then their fixed-order sum; the same result for the same input and shape.
+int gpx_colsum_tall(int rows, int cols, const double *P, double alpha, double beta, double *out, double *work,
+                    void *stream) {
+  if (rows < 0 || cols < 0 || cols > 65535 || !P || !out || !work) return GPX_EINVAL;
+  if (cols == 0) return GPX_OK;
+  const int nc = colsum_chunks(rows > 0 ? rows : 1);
+  const int chunk = (rows + nc - 1) / nc;
+  hipLaunchKernelGGL(colsum_chunk_kernel, dim3(cols, nc), dim3(256), 0, (hipStream_t)stream, rows, cols, chunk, P,
+                     work);
+  GPX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(colsum_kernel, dim3(gpx_cdiv(cols, 256)), dim3(256), 0, (hipStream_t)stream, nc, cols,
+                     (const double *)work, alpha, beta, out);
   GPX_CHECK_LAUNCH();
   return GPX_OK;
 }

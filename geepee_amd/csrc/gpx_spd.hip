@@ -184,12 +184,13 @@ __global__ __launch_bounds__(G *G) void spd_gj_kernel(int M, int batch, const do
     }
 }
 
-// NB = 4 pivots per step where the registers allow it (G = 16); the 1024-thread
-// variants for M > 128 have 128 registers per lane and take one pivot per step.
+// NB = 4 pivots per step where the registers allow it; the 1024-thread variants for
+// M > 128 hold 49-64 matrix entries in their 128 registers per lane and take one pivot
+// per step.
 template <int G, int TB>
 int launch_gj(int M, int batch, const double *A, int lda, long sA, double *out, int ldo, long sO,
               double *logdet, int *info, hipStream_t st) {
-  constexpr int NB = (G == 16) ? 4 : 1;
+  constexpr int NB = (G == 16 || TB <= 4) ? 4 : 1;
   hipLaunchKernelGGL((spd_gj_kernel<G, TB, NB>), dim3(batch), dim3(G * G), 0, st, M, batch, A, lda, sA,
                      out, ldo, sO, logdet, info);
   GPX_CHECK_LAUNCH();
@@ -212,10 +213,11 @@ int gpx_spd_inv_batched(int M, int batch, const double *A, int lda, long strideA
   if (M == 0 || batch == 0) return GPX_OK;
   if (!A || lda < M || (out && ldo < M)) return GPX_EINVAL;
   hipStream_t st = (hipStream_t)stream;
+  // 1024 threads from M > 32 on: more waves per matrix hide the per-step latency
   if (M <= 32) return launch_gj<16, 2>(M, batch, A, lda, strideA, out, ldo, strideOut, logdet, info, st);
-  if (M <= 64) return launch_gj<16, 4>(M, batch, A, lda, strideA, out, ldo, strideOut, logdet, info, st);
-  if (M <= 112) return launch_gj<16, 7>(M, batch, A, lda, strideA, out, ldo, strideOut, logdet, info, st);
-  if (M <= 128) return launch_gj<16, 8>(M, batch, A, lda, strideA, out, ldo, strideOut, logdet, info, st);
+  if (M <= 64) return launch_gj<32, 2>(M, batch, A, lda, strideA, out, ldo, strideOut, logdet, info, st);
+  if (M <= 96) return launch_gj<32, 3>(M, batch, A, lda, strideA, out, ldo, strideOut, logdet, info, st);
+  if (M <= 128) return launch_gj<32, 4>(M, batch, A, lda, strideA, out, ldo, strideOut, logdet, info, st);
   if (M <= 224) return launch_gj<32, 7>(M, batch, A, lda, strideA, out, ldo, strideOut, logdet, info, st);
   return launch_gj<32, 8>(M, batch, A, lda, strideA, out, ldo, strideOut, logdet, info, st);
 }

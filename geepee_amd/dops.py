@@ -58,9 +58,14 @@ def lincomb(out, a, x, b=0.0, y=None, c=0.0, w=None):
 
 
 def colsum(Pm, rows, cols, out=None, alpha=1.0, beta=0.0):
+    """out = alpha * column sums of the (rows, cols) matrix Pm + beta * out."""
     if out is None:
         out = empty(cols, like=Pm)
         beta = 0.0
+    if rows >= 4096 and cols <= 65535:
+        work = empty(query('gpx_colsum_workspace', rows, cols), like=Pm)
+        call('gpx_colsum_tall', rows, cols, P(Pm), float(alpha), float(beta), P(out), P(work), S())
+        return out
     call('gpx_colsum', rows, cols, P(Pm), float(alpha), float(beta), P(out), S())
     return out
 
@@ -251,7 +256,7 @@ def fused_plan(N, M, Q=None, D=None, tuned=True):
     fwd_blocks = -(-N // (16 * nsub))
     splits_f = max(1, min(ntiles, -(-1024 // fwd_blocks)))
     pt_blocks = -(-N // 16)
-    splits_b = max(1, min(ntiles, -(-1024 // pt_blocks)))
+    splits_b = max(1, min(T, -(-1024 // pt_blocks)))
     nsplit = max(1, min(-(-N // 16), -(-1024 // (ntiles * 4))))
     return {'nsub': nsub, 'splits_f': splits_f, 'splits_b': splits_b, 'nsplit': nsplit}
 

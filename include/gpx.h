@@ -111,6 +111,13 @@ int gpx_sym(int batch, int M, double a, const double *X, double *out, void *stre
 int gpx_colsum(int rows, int cols, const double *P, double alpha, double beta, double *out,
                void *stream);
 
+/* Two-stage column sum for tall P (many rows): chunked partial sums into `work`
+ * (gpx_colsum_workspace doubles) and their fixed-order sum; same contract as
+ * gpx_colsum (cols <= 65535). */
+long gpx_colsum_workspace(int rows, int cols);
+int gpx_colsum_tall(int rows, int cols, const double *P, double alpha, double beta, double *out, double *work,
+                    void *stream);
+
 /* v[n,d] = exp(2 sf) + sum_a T[d,n,a] kfu[n,a] with T[d] = kfu Bhat_det[d]
  * (geepee/aep_models.py:153-157, base_models.py:280-282). */
 int gpx_det_var(int N, int D, int M, const double *T, const double *kfu, const double *sf,

@@ -68,7 +68,7 @@ def _alt_plan(orig):
         T = (M + 15) // 16
         nt = T * (T + 1) // 2
         p['splits_f'] = p['splits_f'] + 1 if p['splits_f'] < nt else p['splits_f'] - 1
-        p['splits_b'] = p['splits_b'] + 1 if p['splits_b'] < nt else p['splits_b'] - 1
+        p['splits_b'] = p['splits_b'] + 1 if p['splits_b'] < T else max(1, p['splits_b'] - 1)
         p['nsplit'] = p['nsplit'] + 1 if p['nsplit'] < -(-N // 16) else max(1, p['nsplit'] - 1)
         return p
     return plan

@@ -22,7 +22,7 @@ def test_fused_plan_within_kernel_limits(N, M, monkeypatch, tmp_path):
     p = dops.fused_plan(N, M, 4, 8)
     assert p['nsub'] in (1, 2, 4)
     assert 1 <= p['splits_f'] <= _ntiles(M)
-    assert 1 <= p['splits_b'] <= _ntiles(M)
+    assert 1 <= p['splits_b'] <= (M + 15) // 16
     assert 1 <= p['nsplit'] <= -(-N // 16)
 
 
@@ -32,7 +32,7 @@ def test_fused_candidates_within_kernel_limits(N, M):
     assert fwd and bpt and bpr
     for c in fwd:
         assert c['nsub'] in (1, 2, 4) and 1 <= c['splits_f'] <= _ntiles(M)
-    assert all(1 <= s <= _ntiles(M) for s in bpt)
+    assert all(1 <= s <= (M + 15) // 16 for s in bpt)
     assert all(1 <= s <= -(-N // 16) for s in bpr)
 
 
