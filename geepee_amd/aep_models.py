@@ -388,17 +388,27 @@ class SDGPR(Base_Model):
         dmi, dvi, dma, scal = self.lik_layer.compute_log_Z(ms[-1], vs[-1], yb, alpha, scale_logZ,
                                                            want_dmall=last_mm)
         grads = {}
+        # the hyper-parameter chain of each moment-matched layer forks onto that layer's
+        # stream while the input gradients carry on down the layers on this one
+        streams = getattr(self, '_streams', None) if self.L > 1 else None
+        keep = []
         for i in range(self.L - 1, -1, -1):
             layer = self.sgp_layers[i]
             if i == 0:
                 g = layer.backprop_grads_reg(ms[0], vs[0], dmi, dvi, p1[0], xb, alpha)
             else:
                 g, gin = layer.backprop_grads_lvm_mm(ms[i], vs[i], dmi, dvi, p1[i], p2[i],
-                                                     ms[i - 1], vs[i - 1], alpha, dm_all=dma)
+                                                     ms[i - 1], vs[i - 1], alpha, dm_all=dma,
+                                                     side=streams[i] if streams else None, keep=keep)
                 dmi, dvi, dma = gin['mx'], gin['vx'], None
                 p2[i] = None
             for k, val in g.items():
                 grads[k + '_%d' % i] = val
+        if streams:
+            cur = torch.cuda.current_stream()
+            for st in streams:
+                cur.wait_stream(st)
+        del keep
         grads['sn'] = scal[1:2]
         energy = lincomb(torch.empty(1, dtype=F64, device='cuda'), scale_logZ, scal[0:1])
         for phi in phis:

@@ -27,6 +27,9 @@ __global__ __launch_bounds__(G *G) void spd_gj_kernel(int M, int batch, const do
   static_assert(G % NB == 0, "pivot blocks must not straddle thread rows");
   __shared__ double rowk[2][NB][W];
   __shared__ double colk[2][NB][W];
+  __shared__ double rvs[2][NB][W];
+  __shared__ double Pis[2][NB][NB];
+  __shared__ int failk[2];
   __shared__ double piv[W];
   __shared__ double red[16];
   const int b = blockIdx.x;
@@ -45,52 +48,79 @@ __global__ __launch_bounds__(G *G) void spd_gj_kernel(int M, int batch, const do
   const int Mp = (M + NB - 1) / NB * NB;
   for (int k = 0; k < Mp; k += NB) {
     const int buf = (k / NB) & 1, kb = k / G, km = k % G;
-    // rows K (owned by ty in [km, km+NB)) and columns K (tx in [km, km+NB)) to LDS
-    if (ty >= km && ty < km + NB) {
+    const bool rowK = ty >= km && ty < km + NB, colK = tx >= km && tx < km + NB;
+    const int tr = ty - km, uc = tx - km;
+    // (1) rows K (owned by ty in [km, km+NB)) and columns K (tx in [km, km+NB)) to LDS
+    if (rowK) {
 #pragma unroll
       for (int a = 0; a < TB; ++a)
         if (a == kb)
 #pragma unroll
-          for (int c = 0; c < TB; ++c) rowk[buf][ty - km][tx + G * c] = r[a][c];
+          for (int c = 0; c < TB; ++c) rowk[buf][tr][tx + G * c] = r[a][c];
     }
-    if (tx >= km && tx < km + NB) {
+    if (colK) {
 #pragma unroll
       for (int c = 0; c < TB; ++c)
         if (c == kb)
 #pragma unroll
-          for (int a = 0; a < TB; ++a) colk[buf][tx - km][ty + G * a] = r[a][c];
+          for (int a = 0; a < TB; ++a) colk[buf][uc][ty + G * a] = r[a][c];
     }
     __syncthreads();
-    // P^-1 by scalar Gauss-Jordan on the NB x NB pivot block (every thread, same order)
-    double Pi[NB][NB];
-#pragma unroll
-    for (int t = 0; t < NB; ++t)
-#pragma unroll
-      for (int u = 0; u < NB; ++u) Pi[t][u] = rowk[buf][t][k + u];
-    int bad = 0;
-#pragma unroll
-    for (int s = 0; s < NB; ++s) {
-      const double p = Pi[s][s];
-      if (!bad && !(p > 0.0 && p < INFINITY)) bad = s + 1;
-      const double rp = 1.0 / p;
-      if (threadIdx.x == 0) piv[k + s] = p;
-      double rs[NB], cs[NB];
-#pragma unroll
-      for (int u = 0; u < NB; ++u) rs[u] = Pi[s][u] * rp;
-#pragma unroll
-      for (int t = 0; t < NB; ++t) cs[t] = Pi[t][s];
+    // (2) wave 0: P^-1 by scalar Gauss-Jordan on the NB x NB pivot block, its pivots,
+    //     and rv = P^-1 A[K, :] for every column, into LDS
+    if (threadIdx.x < 64) {
+      double Pi[NB][NB];
 #pragma unroll
       for (int t = 0; t < NB; ++t)
 #pragma unroll
-        for (int u = 0; u < NB; ++u) Pi[t][u] = fma(-cs[t], rs[u], Pi[t][u]);
+        for (int u = 0; u < NB; ++u) Pi[t][u] = rowk[buf][t][k + u];
+      int bad = 0;
 #pragma unroll
-      for (int u = 0; u < NB; ++u) Pi[s][u] = rs[u];
+      for (int s2 = 0; s2 < NB; ++s2) {
+        const double p = Pi[s2][s2];
+        if (!bad && !(p > 0.0 && p < INFINITY)) bad = s2 + 1;
+        const double rp = 1.0 / p;
+        if (threadIdx.x == 0) piv[k + s2] = p;
+        double rs[NB], cs[NB];
 #pragma unroll
-      for (int t = 0; t < NB; ++t) Pi[t][s] = -cs[t] * rp;
-      Pi[s][s] = rp;
+        for (int u = 0; u < NB; ++u) rs[u] = Pi[s2][u] * rp;
+#pragma unroll
+        for (int t = 0; t < NB; ++t) cs[t] = Pi[t][s2];
+#pragma unroll
+        for (int t = 0; t < NB; ++t)
+#pragma unroll
+          for (int u = 0; u < NB; ++u) Pi[t][u] = fma(-cs[t], rs[u], Pi[t][u]);
+#pragma unroll
+        for (int u = 0; u < NB; ++u) Pi[s2][u] = rs[u];
+#pragma unroll
+        for (int t = 0; t < NB; ++t) Pi[t][s2] = -cs[t] * rp;
+        Pi[s2][s2] = rp;
+      }
+      if (threadIdx.x == 0) {
+#pragma unroll
+        for (int t = 0; t < NB; ++t)
+#pragma unroll
+          for (int u = 0; u < NB; ++u) Pis[buf][t][u] = Pi[t][u];
+        failk[buf] = bad ? k + bad : 0;
+      }
+      for (int j = threadIdx.x; j < W; j += 64) {
+        double rk[NB];
+#pragma unroll
+        for (int u = 0; u < NB; ++u) rk[u] = rowk[buf][u][j];
+#pragma unroll
+        for (int t = 0; t < NB; ++t) {
+          double v = 0.0;
+#pragma unroll
+          for (int u = 0; u < NB; ++u) v = fma(Pi[t][u], rk[u], v);
+          rvs[buf][t][j] = v;
+        }
+      }
     }
-    if (bad) {
-      fail = k + bad;
+    __syncthreads();
+    // (3) every thread: A[i, j] -= A[i, K] rv[:, j]; rows K <- rv, columns K <- -A[:, K] P^-1,
+    //     block K x K <- P^-1
+    if (failk[buf]) {
+      fail = failk[buf];
       break;
     }
     double cv[TB][NB];
@@ -98,21 +128,11 @@ __global__ __launch_bounds__(G *G) void spd_gj_kernel(int M, int batch, const do
     for (int a = 0; a < TB; ++a)
 #pragma unroll
       for (int t = 0; t < NB; ++t) cv[a][t] = colk[buf][t][ty + G * a];
-    const bool rowK = ty >= km && ty < km + NB, colK = tx >= km && tx < km + NB;
-    const int tr = ty - km, uc = tx - km;
-    // column by column: rv = P^-1 A[K, j], then A[i, j] -= A[i, K] rv
 #pragma unroll
     for (int c = 0; c < TB; ++c) {
-      double rk[NB], rv[NB];
+      double rv[NB];
 #pragma unroll
-      for (int u = 0; u < NB; ++u) rk[u] = rowk[buf][u][tx + G * c];
-#pragma unroll
-      for (int t = 0; t < NB; ++t) {
-        double v = 0.0;
-#pragma unroll
-        for (int u = 0; u < NB; ++u) v = fma(Pi[t][u], rk[u], v);
-        rv[t] = v;
-      }
+      for (int t = 0; t < NB; ++t) rv[t] = rvs[buf][t][tx + G * c];
 #pragma unroll
       for (int a = 0; a < TB; ++a) {
         double v = r[a][c];
@@ -120,7 +140,6 @@ __global__ __launch_bounds__(G *G) void spd_gj_kernel(int M, int batch, const do
         for (int t = 0; t < NB; ++t) v = fma(-cv[a][t], rv[t], v);
         r[a][c] = v;
       }
-      // rows K become P^-1 A[K, :]
       if (rowK) {
         double mine = rv[0];
 #pragma unroll
@@ -131,16 +150,10 @@ __global__ __launch_bounds__(G *G) void spd_gj_kernel(int M, int batch, const do
           if (a == kb) r[a][c] = mine;
       }
     }
-    // columns K become -A[:, K] P^-1, the block K x K becomes P^-1
     if (colK) {
       double pc[NB];
 #pragma unroll
-      for (int t = 0; t < NB; ++t) {
-        pc[t] = Pi[t][0];
-#pragma unroll
-        for (int u = 1; u < NB; ++u)
-          if (u == uc) pc[t] = Pi[t][u];
-      }
+      for (int t = 0; t < NB; ++t) pc[t] = Pis[buf][t][uc];
 #pragma unroll
       for (int c = 0; c < TB; ++c)
         if (c == kb) {
@@ -152,13 +165,9 @@ __global__ __launch_bounds__(G *G) void spd_gj_kernel(int M, int batch, const do
             r[a][c] = v;
           }
           if (rowK) {
-            double v = pc[0];
-#pragma unroll
-            for (int t = 1; t < NB; ++t)
-              if (t == tr) v = pc[t];
 #pragma unroll
             for (int a = 0; a < TB; ++a)
-              if (a == kb) r[a][c] = v;
+              if (a == kb) r[a][c] = Pis[buf][tr][uc];
           }
         }
     }

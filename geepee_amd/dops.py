@@ -15,7 +15,8 @@ def empty(*shape, like=None, device='cuda'):
 
 def mm(A, B, C=None, tA=False, tB=False, alpha=1.0, beta=0.0, splits=None):
     """C = alpha op(A) op(B) + beta C (2-D or batched 3-D; 2-D operands broadcast).
-    Picks a split-K factor when the output has few tiles and K is long."""
+    Picks a split-K factor when the output has few tiles and K is long (a tuned plan
+    for the shape, if the autotuner stored one, wins)."""
     if C is None:
         batch = max(t.shape[0] if t.dim() == 3 else 1 for t in (A, B))
         m = A.shape[-1] if tA else A.shape[-2]
@@ -34,8 +35,9 @@ def mm(A, B, C=None, tA=False, tB=False, alpha=1.0, beta=0.0, splits=None):
         else:
             tiles = -(-m // 64) * -(-n // 64) * batch
             splits = 1
-            if tiles < 256 and k >= 1024:
-                splits = max(1, min(-(-512 // tiles), k // 256, 64))
+            if tiles < 256 and k >= 256:
+                # long, skinny contractions: split K so that enough workgroups run
+                splits = max(1, min(-(-256 // tiles), k // 128, 64))
     return _lib.mm(A, B, C, tA=tA, tB=tB, alpha=alpha, beta=beta, splits=splits, tile=tile)
 
 

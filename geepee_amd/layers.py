@@ -299,9 +299,16 @@ class SGP_Layer(object):
         deta2 = lincomb(deta2_cav, 1.0, deta2_cav, 1.0, deta2_post)
         return {'sf': dsf, 'ls': dls, 'zu': dzu, 'eta1_R': deta1, 'eta2': deta2}
 
-    def backprop_grads_lvm_mm(self, m, v, dm, dv, psi1, psi2, mx, vx, alpha=1.0, dm_all=None):
+    def backprop_grads_lvm_mm(self, m, v, dm, dv, psi1, psi2, mx, vx, alpha=1.0, dm_all=None,
+                              side=None, keep=None):
         """backprop_grads_lvm_mm (aep_models.py:201-304): dm, dv already scaled.
-        Returns (hyper grads, {'mx', 'vx'})."""
+        Returns (hyper grads, {'mx', 'vx'}).
+
+        With a `side` stream, only the input gradients (the psi adjoint) are on the
+        current stream; the hyper-parameter chain through Kuu^-1 and q(u) is forked onto
+        `side`, so the caller can carry on with the next layer's backward and must join
+        `side` before reading the hyper grads.  Tensors of the current stream that the
+        fork reads are appended to `keep`, which the caller holds until the join."""
         N = self.N
         D, M = self.Dout, self.M
         B = psi1.shape[0]
@@ -323,6 +330,24 @@ class SGP_Layer(object):
             dsf2, dell, dzu, dmx, dvx = dops.psi_bwd(mx, vx, self.zu, self.ls, self.sf,
                                                      dpsi1, psi1, dpsi2, psi2)
             del dpsi2
+        if side is None:
+            grad = self._lvm_hyper_chain(dv, dAhat, dBhat, dsf2, dell, dzu, B, alpha)
+        else:
+            side.wait_stream(torch.cuda.current_stream())
+            keep.extend([dv, dAhat, dBhat, dsf2, dell, dzu])
+            with torch.cuda.stream(side):
+                grad = self._lvm_hyper_chain(dv, dAhat, dBhat, dsf2, dell, dzu, B, alpha)
+        return grad, {'mx': dmx, 'vx': dvx}
+
+    def _lvm_hyper_chain(self, dv, dAhat, dBhat, dsf2, dell, dzu, B, alpha):
+        """The q(u) / Kuu^-1 / hyper-parameter part of backprop_grads_lvm_mm
+        (aep_models.py:245-304) from dAhat, dBhat and the psi adjoint."""
+        N = self.N
+        D, M = self.Dout, self.M
+        Ki = self.Kuuinv
+        beta = (N - alpha) * 1.0 / N
+        scale_post = N * 1.0 / alpha - 1.0
+        scale_cav = -N * 1.0 / alpha
         sumdv = colsum(dv, B * D, 1)
         dvcav = _sandwich(Ki, dBhat, Ki)
         dmcav = mm(dvcav, _col(self.muhat), alpha=2.0).reshape(D, M)
@@ -346,6 +371,5 @@ class SGP_Layer(object):
         _sandwich(Ki, Minner, Ki, alpha=0.5, out=M_all)
         gsf, gls, gz = dops.dtrace(M_all, self.zu, self.ls, self.sf)
         dls, dsf = dops.hyper(self.ls, self.sf, dell, gls, dsf2, sumdv, gsf)
-        lincomb(dzu, 1.0, dzu, 1.0, gz)
-        grad = {'sf': dsf, 'ls': dls, 'zu': dzu, 'eta1_R': deta1, 'eta2': dtheta2}
-        return grad, {'mx': dmx, 'vx': dvx}
+        dzu = lincomb(torch.empty_like(dzu), 1.0, dzu, 1.0, gz)
+        return {'sf': dsf, 'ls': dls, 'zu': dzu, 'eta1_R': deta1, 'eta2': dtheta2}
