@@ -4,7 +4,10 @@ Default workload: the SGPLVM uncertain-input objective (N=4096 latent Gaussians,
 D=32 outputs, M=128 inducing points, alpha=0.5, full batch, FP64): one step = one
 ``objective_function`` evaluation (psi0/psi1/psi2 forward, likelihood, hand-written
 backward through the psi statistics and the q(u) / Kuu chain) with the parameters
-already resident in HBM and the gradient left on the device.
+already resident in HBM and the gradient left on the device.  Default path: the fused
+moment-matched kernels (psi2 recomputed, never stored); --path plain materialises psi2,
+--path graph replays the fused evaluation as one hipGraph.  Other workloads: sdgpr,
+sgpssm, sgpr, serving, copy (counter calibration).
 
 Prints one JSON line.  ``roofline`` reports the dominant kernel timed with HIP events
 on the launch stream over the timed region; ``cpu_baseline`` times the CPU oracle in
@@ -24,6 +27,15 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0       # MI355X HBM3E, GB/s
 FP64_PEAK_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
+
+# HBM bytes per launch of the roofline entry points, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (KiB per dispatch, summed over the entry point's kernels) on the benchmark
+# shapes.  On MI355X a plain copy of 512 MiB reads as FETCH_SIZE 262,200 KiB and
+# WRITE_SIZE 524,288 KiB, so FETCH_SIZE is scaled by 2.0 and WRITE_SIZE by 1.0.
+TRAFFIC = {
+    ('sgplvm', 'gpx_psi_fwd'): 2.0 * 2452 * 1024 + 1.0 * 528384 * 1024,
+    ('sgplvm', 'gpx_fused_mm_bwd'): 2.0 * 146077 * 1024 + 1.0 * 65880 * 1024,
+}
 
 WORKLOADS = {
     'sgplvm': dict(desc='SGPLVM uncertain-input AEP objective, N=4096 latent Gaussians '
@@ -74,25 +86,26 @@ def cpu_baseline_sgplvm(w, budget_s=40.0):
     """Faithful-cost oracle (naive einsums, scalar C psi loop, LU inverses) on the same
     synthetic SGPLVM problem at N=512 and N=1024 rows, extrapolated linearly in N to
     the full N (the per-point work is linear in N; the D*M^4 einsums are not)."""
+    from threadpoolctl import threadpool_limits
     from oracle import aep_oracle as O
     from geepee_amd import workloads as W
     O.FAITHFUL = True
     try:
         times = {}
-        for n in (512, 1024):
-            y, p = W.sgplvm_problem(N=n, Q=w['Q'], D=w['D'], M=w['M'], seed=1)
-            t0 = time.perf_counter()
-            O.sgplvm_objective(y, p, w['M'], n, alpha=0.5)
-            times[n] = time.perf_counter() - t0
+        with threadpool_limits(limits=1):
+            for n in (512, 1024):
+                y, p = W.sgplvm_problem(N=n, Q=w['Q'], D=w['D'], M=w['M'], seed=1)
+                t0 = time.perf_counter()
+                O.sgplvm_objective(y, p, w['M'], n, alpha=0.5)
+                times[n] = time.perf_counter() - t0
     finally:
         O.FAITHFUL = False
     slope = (times[1024] - times[512]) / 512.0
     t_full = times[1024] + slope * (w['N'] - 1024)
-    threads = os.environ.get('OMP_NUM_THREADS', str(os.cpu_count()))
-    return {'value': 1.0 / t_full, 'unit': 'evals/s', 'cores': int(threads) if threads.isdigit() else None,
-            'kind': 'port',
-            'sample': ('oracle faithful-cost SGPLVM eval timed at N=512 (%.2fs) and N=1024 (%.2fs), '
-                       'linear in N to N=%d (%.1fs/eval); %s; numpy %s'
+    return {'value': 1.0 / t_full, 'unit': 'evals/s', 'cores': 1, 'kind': 'port',
+            'sample': ('oracle faithful-cost SGPLVM eval (numpy einsums without path optimisation, '
+                       'LU inverses, the scalar C psi loop), one thread, timed at N=512 (%.2fs) and '
+                       'N=1024 (%.2fs) and extrapolated linearly in N to N=%d (%.1fs/eval); %s; numpy %s'
                        % (times[512], times[1024], w['N'], t_full, _cpu_info(), np.__version__))}
 
 
@@ -167,7 +180,7 @@ def run_evals(args, name, w):
         else:
             achieved, peak, unit = work / (ms * 1e-3) / 1e12, FP64_PEAK_TFLOPS, 'TFLOP/s'
         res['roofline'] = {'bound': bound, 'kernel': args.probe, 'achieved': achieved, 'peak': peak,
-                           'unit': unit, 'frac': achieved / peak, 'traffic': None,
+                           'unit': unit, 'frac': achieved / peak, 'traffic': TRAFFIC.get((name, args.probe)),
                            'avg_launch_ms': ms, 'launches': probe.count(),
                            ('bytes_per_launch' if bound == 'hbm' else 'flops_per_launch'): work}
     _ = out
@@ -237,13 +250,13 @@ def main():
     ap.add_argument('--steps', type=int, default=10)
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--workload', default='sgplvm', choices=sorted(WORKLOADS) + ['serving', 'copy'])
-    ap.add_argument('--path', default='plain', choices=['plain', 'fused', 'graph'])
+    ap.add_argument('--path', default='fused', choices=['plain', 'fused', 'graph'])
     ap.add_argument('--probe', default=None,
                     help='C-ABI entry point timed for the roofline (default: the path\'s dominant one)')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--rates', type=float, nargs='+', default=[1e2, 1e3, 1e4])
-    ap.add_argument('--requests', type=int, default=2000)
-    ap.add_argument('--plain-requests', type=int, default=500)
+    ap.add_argument('--requests', type=int, default=1000)
+    ap.add_argument('--plain-requests', type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU only: this build runs the hot path on a single MI355X '

@@ -12,7 +12,8 @@ Knobs
                   nsplit (point ranges of the per-pair pass)
   gemm            tile (32 or 64 rows/cols per workgroup), split-K factor
 
-Cache: ``$GEEPEE_AMD_TUNE_CACHE`` if set, else ``~/.cache/geepee_amd/tune.json``.
+Cache: ``$GEEPEE_AMD_TUNE_CACHE`` if set, else ``~/.cache/geepee_amd/tune.json``, over
+the plans shipped in ``tuned_plans.json`` (the benchmark shapes, tuned on MI355X).
 ``lookup`` is consulted by dops.fused_plan and dops.mm before their heuristics.
 """
 import itertools
@@ -45,17 +46,26 @@ def _device_tag():
     return _tag
 
 
+SHIPPED = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'tuned_plans.json')
+
+
+def _read(path):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
 def _load():
+    """Plans shipped with the package (tuned on MI355X for the benchmark shapes),
+    overridden by the user's cache."""
     global _cache
     if _cache is None:
-        _cache = {}
+        _cache = dict(_read(SHIPPED)) if os.path.exists(SHIPPED) else {}
         p = cache_path()
         if os.path.exists(p):
-            try:
-                with open(p) as f:
-                    _cache = json.load(f)
-            except (OSError, ValueError):
-                _cache = {}
+            _cache.update(_read(p))
     return _cache
 
 
@@ -74,9 +84,11 @@ def store(kind, shape, plan, save=True):
     if save:
         p = cache_path()
         os.makedirs(os.path.dirname(p), exist_ok=True)
+        user = _read(p) if os.path.exists(p) else {}
+        user[_key(kind, shape)] = plan
         tmp = p + '.tmp%d' % os.getpid()
         with open(tmp, 'w') as f:
-            json.dump(c, f, indent=1, sort_keys=True)
+            json.dump(user, f, indent=1, sort_keys=True)
         os.replace(tmp, p)
 
 

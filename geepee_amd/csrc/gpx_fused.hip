@@ -579,6 +579,7 @@ __global__ __launch_bounds__(256) void fused_fwd_finish(int N, int M, int Q, int
 
 }  // namespace
 
+extern "C" long gpx_psi_tail_workspace(int N, int M, int Q);
 extern "C" int gpx_gemm_batched(int transA, int transB, int m, int n, int k, double alpha,
                                 const double *A, int lda, long strideA, const double *B, int ldb,
                                 long strideB, double beta, double *C, int ldc, long strideC,
@@ -586,7 +587,7 @@ extern "C" int gpx_gemm_batched(int transA, int transB, int m, int n, int k, dou
 extern "C" int gpx_psi_bwd_tail(int N, int M, int Q, const double *z, const double *ls, const double *sf,
                                 const double *T, const double *Lp, const double *H, const double *s0,
                                 const double *Dn, const double *Pn, const double *En, const double *LW,
-                                double *TP, double *LE, double *cols0, double *dsf2, double *dell, double *dZ,
+                                double *TP, double *LE, double *cols0, double *dsf2, double *dell, double *dZ, double *kwork,
                                 void *stream);
 
 extern "C" {
@@ -598,6 +599,7 @@ long gpx_fused_bwd_workspace(int N, int M, int Q, int D, int splits, int nsplit)
   s += 3L * N * M + (long)N * Q + 2L * N;                                     // dpsi1, T, Lp, H, s0
   s += (long)N * (Q + 1) + 4L * N * Q + 4L * M * Q + 2;                       // Dn, Pn, En, TP, LE, cols0
   s += (long)N * PF_W;                                                        // point features
+  s += gpx_psi_tail_workspace(N, M, Q);                                       // tail split-K partials
   return s;
 }
 
@@ -673,7 +675,7 @@ int gpx_fused_mm_bwd(int N, int M, int Q, int D, const double *mx, const double 
                      (const double *)Tpart, (const double *)Vpart, (const double *)dpsi1, psi1, dmx, dvx, Tn, Lp,
                      H, s0, Dn, Pn, En);
   GPX_CHECK_LAUNCH();
-  return gpx_psi_bwd_tail(N, M, Q, z, ls, sf, Tn, Lp, H, s0, Dn, Pn, En, LW, TP, LE, cols0, dsf2, dell, dZ,
+  return gpx_psi_bwd_tail(N, M, Q, z, ls, sf, Tn, Lp, H, s0, Dn, Pn, En, LW, TP, LE, cols0, dsf2, dell, dZ, pf + (long)N * PF_W,
                           stream);
 }
 

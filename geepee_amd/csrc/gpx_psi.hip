@@ -325,18 +325,28 @@ extern "C" {
 // outputs T, Lp (N x M), H (N x Q), s0 (N x 2), Dn, Pn, En and the pair sums LW
 // ((M*M) x (Q+1)) finish dsf2, dell (w.r.t. ell) and dZ.  TP, LE (M x 2Q) and cols0 (2)
 // are workspace.
+// Split-K factor of the two N-long reductions TP = T^T Pn, LE = Lp^T En (M x 2Q outputs,
+// few tiles) and the doubles of partials they need.
+static int tail_splits(int N) {
+  int s = N / 128;
+  return s < 1 ? 1 : (s > 64 ? 64 : s);
+}
+
+long gpx_psi_tail_workspace(int N, int M, int Q) { return (long)tail_splits(N) * M * 2 * Q; }
+
 int gpx_psi_bwd_tail(int N, int M, int Q, const double *z, const double *ls, const double *sf,
                      const double *T, const double *Lp, const double *H, const double *s0,
                      const double *Dn, const double *Pn, const double *En, const double *LW,
                      double *TP, double *LE, double *cols0, double *dsf2, double *dell, double *dZ,
-                     void *stream) {
-  if (N < 1 || M < 1 || M > 256 || Q < 1 || Q > GPX_QMAX) return GPX_EINVAL;
+                     double *kwork, void *stream) {
+  if (N < 1 || M < 1 || M > 256 || Q < 1 || Q > GPX_QMAX || !kwork) return GPX_EINVAL;
   hipStream_t st = (hipStream_t)stream;
-  int rc = gpx_gemm_batched(1, 0, M, 2 * Q, N, 1.0, T, M, 0, Pn, 2 * Q, 0, 0.0, TP, 2 * Q, 0, 1, 1,
-                            nullptr, 0, stream);
+  const int ks = tail_splits(N);
+  int rc = gpx_gemm_batched(1, 0, M, 2 * Q, N, 1.0, T, M, 0, Pn, 2 * Q, 0, 0.0, TP, 2 * Q, 0, 1, ks,
+                            kwork, 0, stream);
   if (rc) return rc;
-  rc = gpx_gemm_batched(1, 0, M, 2 * Q, N, 1.0, Lp, M, 0, En, 2 * Q, 0, 0.0, LE, 2 * Q, 0, 1, 1,
-                        nullptr, 0, stream);
+  rc = gpx_gemm_batched(1, 0, M, 2 * Q, N, 1.0, Lp, M, 0, En, 2 * Q, 0, 0.0, LE, 2 * Q, 0, 1, ks,
+                        kwork, 0, stream);
   if (rc) return rc;
   hipLaunchKernelGGL(colsum_kernel, dim3(Q), dim3(256), 0, st, N, Q, H, dell);
   GPX_CHECK_LAUNCH();
@@ -380,6 +390,7 @@ long gpx_psi_bwd_workspace(int N, int M, int Q) {
   s += (long)N * (Q + 1) + 4L * N * Q;      // Dn, Pn, En
   s += (long)M * M * (Q + 1) + 4L * M * Q;  // LW, TP, LE
   s += Q + 2;                               // colH, cols0
+  s += gpx_psi_tail_workspace(N, M, Q);     // split-K partials of the tail
   return s;
 }
 
@@ -412,7 +423,7 @@ int gpx_psi_bwd(int N, int M, int Q, const double *mx, const double *vx, const d
   int rc = gpx_gemm_batched(1, 0, M * M, Q + 1, N, 1.0, L, M * M, 0, Dn, Q + 1, 0, 0.0, LW, Q + 1, 0, 1,
                             1, nullptr, 0, stream);
   if (rc) return rc;
-  return gpx_psi_bwd_tail(N, M, Q, z, ls, sf, T, Lp, H, s0, Dn, Pn, En, LW, TP, LE, cols0, dsf2, dell, dZ,
+  return gpx_psi_bwd_tail(N, M, Q, z, ls, sf, T, Lp, H, s0, Dn, Pn, En, LW, TP, LE, cols0, dsf2, dell, dZ, cols0 + 2,
                           stream);
 }
 

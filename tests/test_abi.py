@@ -47,3 +47,17 @@ def test_workspace_queries():
     assert L.gpx_psi_bwd_workspace(100, 10, 2) > 100 * 10 * 10
     assert L.gpx_spd_max_m() == 256
     assert L.gpx_psi_qmax() == 16
+
+
+def test_library_has_no_undefined_gpx_symbols():
+    """Every gpx_* symbol the library references is defined in it (a shared library
+    links with unresolved symbols, which would only fail at the first call)."""
+    import subprocess
+    from geepee_amd import _lib
+    path = _lib.lib_path() if hasattr(_lib, 'lib_path') else None
+    if path is None:
+        import os
+        path = os.path.join(os.path.dirname(_lib.__file__), 'libgpx.so')
+    out = subprocess.run(['nm', '-D', '--undefined-only', path], capture_output=True, text=True, check=True)
+    undefined = [l.split()[-1] for l in out.stdout.splitlines() if 'gpx_' in l]
+    assert not undefined, undefined
