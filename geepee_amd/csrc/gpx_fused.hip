@@ -241,49 +241,59 @@ __global__ __launch_bounds__(256) void fused_bwd_points_kernel(
   const int T = (M + 15) / 16;
   for (int B = split; B < T; B += S) {
     double colp[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int A = 0; A <= B; ++A) {
+    const int ja = 16 * B + (lane & 15);
+    // row blocks (A, i) of this column tile for this wave, flattened as it = 16 A + i with
+    // it = wave, wave + 4, ...; B-hat of the next row block is fetched during this one
+    const int nit = min((B + 1) * 16, M);
+    double bcur[DS], bnxt[DS];
+#pragma unroll
+    for (int s = 0; s < DS; ++s) {
+      const int d = (lane >> 4) + 4 * s;
+      bcur[s] = (wave < nit && ja < M && d < D) ? Bp[((long)wave * M + ja) * D + d] : 0.0;
+    }
+    for (int it = wave; it < nit; it += 4) {
+      const int A = it >> 4, a = it;
       const bool off = (A != B);
       const double w = off ? 2.0 : 1.0;
-      for (int i = wave; i < 16; i += 4) {
-        const int a = 16 * A + i;
-        if (a >= M) break;
-        const int ja = 16 * B + (lane & 15);
-        gpx_d4 e = gpx_d4{0.0, 0.0, 0.0, 0.0};
+      const int itn = it + 4;
 #pragma unroll
-        for (int s = 0; s < FKS_MAX; ++s) {
-          if (s < KS) {
-            const double af = (ja < M) ? pair_feature((lane >> 4) + 4 * s, Q, zs[a], zs[ja], il4) : 0.0;
-            e = gpx_mfma16(af, bf[s], e);
-          }
-        }
-        gpx_d4 g = gpx_d4{0.0, 0.0, 0.0, 0.0};
+      for (int s = 0; s < DS; ++s) {
+        const int d = (lane >> 4) + 4 * s;
+        bnxt[s] = (itn < nit && ja < M && d < D) ? Bp[((long)itn * M + ja) * D + d] : 0.0;
+      }
+      gpx_d4 e = gpx_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int s = 0; s < DS; ++s) {
-          const int d = (lane >> 4) + 4 * s;
-          const double bsv = (ja < M && d < D) ? Bp[((long)a * M + ja) * D + d] : 0.0;
-          g = gpx_mfma16(bsv, dvr[s], g);
-        }
-        double L[4];
-        double rs = 0.0;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int b = 16 * B + (lane >> 4) + 4 * r;
-          const double p = (b < M && nvalid) ? sf4 * exp(e[r]) : 0.0;
-          L[r] = g[r] * p;
-          rs += L[r];
-          if (off) colp[r] += L[r];
-        }
-        rs += __shfl_xor(rs, 16, 64);
-        rs += __shfl_xor(rs, 32, 64);
-        if ((lane >> 4) == 0) Tacc[lane & 15][a] += rs;
-        // V[q x n] += w z_aq z_bq L[pair x n]
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int q = lane & 15, b = 16 * B + (lane >> 4) + 4 * r;
-          const double zz = (q < Q && b < M) ? w * zs[a][q] * zs[b][q] : 0.0;
-          vacc = gpx_mfma16(zz, L[r], vacc);
+      for (int s = 0; s < FKS_MAX; ++s) {
+        if (s < KS) {
+          const double af = (ja < M) ? pair_feature((lane >> 4) + 4 * s, Q, zs[a], zs[ja], il4) : 0.0;
+          e = gpx_mfma16(af, bf[s], e);
         }
       }
+      gpx_d4 g = gpx_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < DS; ++s) g = gpx_mfma16(bcur[s], dvr[s], g);
+      double L[4];
+      double rs = 0.0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int b = 16 * B + (lane >> 4) + 4 * r;
+        const double p = (b < M && nvalid) ? sf4 * exp(e[r]) : 0.0;
+        L[r] = g[r] * p;
+        rs += L[r];
+        if (off) colp[r] += L[r];
+      }
+      rs += __shfl_xor(rs, 16, 64);
+      rs += __shfl_xor(rs, 32, 64);
+      if ((lane >> 4) == 0) Tacc[lane & 15][a] += rs;
+      // V[q x n] += w z_aq z_bq L[pair x n]
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int q = lane & 15, b = 16 * B + (lane >> 4) + 4 * r;
+        const double zz = (q < Q && b < M) ? w * zs[a][q] * zs[b][q] : 0.0;
+        vacc = gpx_mfma16(zz, L[r], vacc);
+      }
+#pragma unroll
+      for (int s = 0; s < DS; ++s) bcur[s] = bnxt[s];
     }
     if (B > 0) {
       // merge the column sums of tile column B: stage per wave, then one thread per

@@ -181,34 +181,27 @@ __global__ __launch_bounds__(256) void gemm_fullk_kernel(
   const int m0 = blockIdx.y * 32, n0 = blockIdx.x * 32;
   const int kp = (k + 3) & ~3;
   const int tid = threadIdx.x;
-  // all loads first (up to 16 + 16 per thread in flight), then the LDS stores
-  constexpr int U = 32 * FULLK_MAX / 256;
-  const int tot = 32 * kp;
+  // all loads first (16 + 16 per thread in flight), then the LDS stores.  Thread t covers
+  // k = k0 + 8j of one row / column (shifts only, no division), reading along the
+  // operand's contiguous dimension.
+  constexpr int U = FULLK_MAX / 8;
   double ra[U], rb[U];
+  // op(A)[mm][kk]: A stored (m x k) reads 8 consecutive k of a row; A^T stored (k x m)
+  // reads 32 consecutive m of a k-row
+  const int am = tA ? (tid & 31) : (tid >> 3), ak = tA ? (tid >> 5) : (tid & 7);
+  const int bn = tB ? (tid >> 3) : (tid & 31), bk = tB ? (tid & 7) : (tid >> 5);
+  const int gam = m0 + am, gbn = n0 + bn;
 #pragma unroll
   for (int u = 0; u < U; ++u) {
-    const int e = tid + 256 * u;
-    ra[u] = 0.0;
-    rb[u] = 0.0;
-    if (e < tot) {
-      int mm, kk, nn, k2;
-      if (tA) { kk = e >> 5; mm = e & 31; } else { mm = e / kp; kk = e - mm * kp; }
-      if (tB) { nn = e / kp; k2 = e - nn * kp; } else { k2 = e >> 5; nn = e & 31; }
-      const int gm = m0 + mm, gn = n0 + nn;
-      if (gm < m && kk < k) ra[u] = tA ? A[(long)kk * lda + gm] : A[(long)gm * lda + kk];
-      if (gn < n && k2 < k) rb[u] = tB ? B[(long)gn * ldb + k2] : B[(long)k2 * ldb + gn];
-    }
+    const int kka = ak + 8 * u, kkb = bk + 8 * u;
+    ra[u] = (gam < m && kka < k) ? (tA ? A[(long)kka * lda + gam] : A[(long)gam * lda + kka]) : 0.0;
+    rb[u] = (gbn < n && kkb < k) ? (tB ? B[(long)gbn * ldb + kkb] : B[(long)kkb * ldb + gbn]) : 0.0;
   }
 #pragma unroll
   for (int u = 0; u < U; ++u) {
-    const int e = tid + 256 * u;
-    if (e < tot) {
-      int mm, kk, nn, k2;
-      if (tA) { kk = e >> 5; mm = e & 31; } else { mm = e / kp; kk = e - mm * kp; }
-      if (tB) { nn = e / kp; k2 = e - nn * kp; } else { k2 = e >> 5; nn = e & 31; }
-      As[kk][mm] = ra[u];
-      Bs[k2][nn] = rb[u];
-    }
+    const int kka = ak + 8 * u, kkb = bk + 8 * u;
+    if (kka < kp) As[kka][am] = ra[u];
+    if (kkb < kp) Bs[kkb][bn] = rb[u];
   }
   __syncthreads();
   const int lane = tid & 63, wave = tid >> 6, wm = wave >> 1, wn = wave & 1;
