@@ -357,60 +357,89 @@ __global__ __launch_bounds__(256) void fused_bwd_pairs_kernel(
     const double *__restrict__ sf, const double *__restrict__ Bp, const double *__restrict__ dv,
     const double *__restrict__ pf, int nsplit, int nchunk, double *__restrict__ dBpart,
     double *__restrict__ LWpart) {
+  // A block holds 8 row blocks of one tile; wave w owns rows i = h*8 + w and i + 4, two
+  // independent MFMA chains that share every point operand it loads.
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int T = (M + 15) / 16;
-  const int rb = blockIdx.x * 4 + wave;       // row block id: tile * 16 + i
-  const int t = rb / 16, i = rb % 16;
+  const int t = blockIdx.x >> 1, h = blockIdx.x & 1;
   int A, B;
   decode_tile(t, T, A, B);
-  const int a = 16 * A + i;
-  if (a >= M) return;                         // whole wave; no barriers below
+  int a[2];
+  a[0] = 16 * A + h * 8 + wave;
+  a[1] = a[0] + 4;
+  const bool act[2] = {a[0] < M, a[1] < M};
+  if (!act[0]) return;                        // whole wave (a[1] > a[0]); no barriers below
   const int ns = blockIdx.y;
   const int nbeg = ns * nchunk, nend = min(N, nbeg + nchunk);
   const int KS = (2 * Q + 2 + 3) / 4;
   const int jb = 16 * B + (lane & 15);       // this lane's pair column b
   const bool bvalid = jb < M;
-  double il4[FQ_MAX];
-#pragma unroll
-  for (int q = 0; q < FQ_MAX; ++q) il4[q] = (q < Q) ? 0.25 * exp(-2.0 * ls[q]) : 0.0;
-  double za[FQ_MAX], zb[FQ_MAX];
+  double il4[FQ_MAX], zb[FQ_MAX];
 #pragma unroll
   for (int q = 0; q < FQ_MAX; ++q) {
-    za[q] = (q < Q) ? z[a * Q + q] : 0.0;
+    il4[q] = (q < Q) ? 0.25 * exp(-2.0 * ls[q]) : 0.0;
     zb[q] = (q < Q && bvalid) ? z[jb * Q + q] : 0.0;
   }
-  double af[FKS_MAX], bs[DS];
+  double af[2][FKS_MAX], bs[2][DS];
 #pragma unroll
-  for (int s = 0; s < FKS_MAX; ++s)
-    af[s] = (s < KS && bvalid) ? pair_feature((lane >> 4) + 4 * s, Q, za, zb, il4) : 0.0;
+  for (int h2 = 0; h2 < 2; ++h2) {
+    double za[FQ_MAX];
 #pragma unroll
-  for (int s = 0; s < DS; ++s) {
-    const int d = (lane >> 4) + 4 * s;
-    bs[s] = (bvalid && d < D) ? Bp[((long)a * M + jb) * D + d] : 0.0;
-  }
-  const double sf2 = exp(2.0 * sf[0]), sf4 = sf2 * sf2;
-  gpx_d4 dacc[DB], lacc = gpx_d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-  for (int db = 0; db < DB; ++db) dacc[db] = gpx_d4{0.0, 0.0, 0.0, 0.0};
-  for (int n0 = nbeg; n0 < nend; n0 += 16) {
-    const int nA = n0 + (lane & 15);         // point of this lane as an A-operand row
-    const bool vA = nA < nend;
-    gpx_d4 e = gpx_d4{0.0, 0.0, 0.0, 0.0}, g = gpx_d4{0.0, 0.0, 0.0, 0.0};
+    for (int q = 0; q < FQ_MAX; ++q) za[q] = (q < Q && act[h2]) ? z[a[h2] * Q + q] : 0.0;
 #pragma unroll
     for (int s = 0; s < FKS_MAX; ++s)
-      if (s < KS) e = gpx_mfma16(vA ? pf[(long)nA * PF_W + (lane >> 4) + 4 * s] : 0.0, af[s], e);
+      af[h2][s] = (s < KS && bvalid && act[h2]) ? pair_feature((lane >> 4) + 4 * s, Q, za, zb, il4) : 0.0;
 #pragma unroll
     for (int s = 0; s < DS; ++s) {
       const int d = (lane >> 4) + 4 * s;
-      g = gpx_mfma16((vA && d < D) ? dv[(long)nA * D + d] : 0.0, bs[s], g);
+      bs[h2][s] = (bvalid && act[h2] && d < D) ? Bp[((long)a[h2] * M + jb) * D + d] : 0.0;
     }
-    double p[4], L[4];
+  }
+  const double sf2 = exp(2.0 * sf[0]), sf4 = sf2 * sf2;
+  gpx_d4 dacc[2][DB], lacc[2];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int nn = (lane >> 4) + 4 * r;
-      p[r] = (n0 + nn < nend && bvalid) ? sf4 * exp(e[r]) : 0.0;
-      L[r] = g[r] * p[r];
+  for (int h2 = 0; h2 < 2; ++h2) {
+    lacc[h2] = gpx_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int db = 0; db < DB; ++db) dacc[h2][db] = gpx_d4{0.0, 0.0, 0.0, 0.0};
+  }
+  for (int n0 = nbeg; n0 < nend; n0 += 16) {
+    const int nA = n0 + (lane & 15);         // point of this lane as an A-operand row
+    const bool vA = nA < nend;
+    double pa[FKS_MAX], da[DS];
+#pragma unroll
+    for (int s = 0; s < FKS_MAX; ++s) pa[s] = (s < KS && vA) ? pf[(long)nA * PF_W + (lane >> 4) + 4 * s] : 0.0;
+#pragma unroll
+    for (int s = 0; s < DS; ++s) {
+      const int d = (lane >> 4) + 4 * s;
+      da[s] = (vA && d < D) ? dv[(long)nA * D + d] : 0.0;
     }
+    gpx_d4 e[2], g[2];
+#pragma unroll
+    for (int h2 = 0; h2 < 2; ++h2) {
+      e[h2] = gpx_d4{0.0, 0.0, 0.0, 0.0};
+      g[h2] = gpx_d4{0.0, 0.0, 0.0, 0.0};
+    }
+#pragma unroll
+    for (int s = 0; s < FKS_MAX; ++s)
+      if (s < KS) {
+        e[0] = gpx_mfma16(pa[s], af[0][s], e[0]);
+        e[1] = gpx_mfma16(pa[s], af[1][s], e[1]);
+      }
+#pragma unroll
+    for (int s = 0; s < DS; ++s) {
+      g[0] = gpx_mfma16(da[s], bs[0][s], g[0]);
+      g[1] = gpx_mfma16(da[s], bs[1][s], g[1]);
+    }
+    double p[2][4], L[2][4];
+#pragma unroll
+    for (int h2 = 0; h2 < 2; ++h2)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int nn = (lane >> 4) + 4 * r;
+        p[h2][r] = (n0 + nn < nend && bvalid && act[h2]) ? sf4 * exp(e[h2][r]) : 0.0;
+        L[h2][r] = g[h2][r] * p[h2][r];
+      }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int nr = n0 + (lane >> 4) + 4 * r;  // point of this k-slice
@@ -418,24 +447,32 @@ __global__ __launch_bounds__(256) void fused_bwd_pairs_kernel(
 #pragma unroll
       for (int db = 0; db < DB; ++db) {
         const int d = 16 * db + (lane & 15);
-        dacc[db] = gpx_mfma16((vr && d < D) ? dv[(long)nr * D + d] : 0.0, p[r], dacc[db]);
+        const double dvv = (vr && d < D) ? dv[(long)nr * D + d] : 0.0;
+        dacc[0][db] = gpx_mfma16(dvv, p[0][r], dacc[0][db]);
+        dacc[1][db] = gpx_mfma16(dvv, p[1][r], dacc[1][db]);
       }
-      lacc = gpx_mfma16(vr ? pf[(long)nr * PF_W + 4 * FKS_MAX + (lane & 15)] : 0.0, L[r], lacc);
+      const double dq = vr ? pf[(long)nr * PF_W + 4 * FKS_MAX + (lane & 15)] : 0.0;
+      lacc[0] = gpx_mfma16(dq, L[0][r], lacc[0]);
+      lacc[1] = gpx_mfma16(dq, L[1][r], lacc[1]);
     }
   }
-  // dacc[db]: lane holds dBhat[d = 16db + (lane>>4) + 4r][pair b = jb]
 #pragma unroll
-  for (int db = 0; db < DB; ++db)
+  for (int h2 = 0; h2 < 2; ++h2) {
+    if (!act[h2]) continue;
+    // dacc[db]: lane holds dBhat[d = 16db + (lane>>4) + 4r][pair b = jb]
+#pragma unroll
+    for (int db = 0; db < DB; ++db)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int d = 16 * db + (lane >> 4) + 4 * r;
+        if (d < D && jb < M) dBpart[(((long)ns * D + d) * M + a[h2]) * M + jb] = dacc[h2][db][r];
+      }
+    // lacc: lane holds LW[q' = (lane>>4) + 4r][pair b = jb]
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int d = 16 * db + (lane >> 4) + 4 * r;
-      if (d < D && jb < M) dBpart[(((long)ns * D + d) * M + a) * M + jb] = dacc[db][r];
+      const int qq = (lane >> 4) + 4 * r;
+      if (qq <= Q && jb < M) LWpart[(((long)ns * M + a[h2]) * M + jb) * (Q + 1) + qq] = lacc[h2][r];
     }
-  // lacc: lane holds LW[q' = (lane>>4) + 4r][pair b = jb]
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int qq = (lane >> 4) + 4 * r;
-    if (qq <= Q && jb < M) LWpart[(((long)ns * M + a) * M + jb) * (Q + 1) + qq] = lacc[r];
   }
 }
 
@@ -654,7 +691,7 @@ int gpx_fused_mm_bwd(int N, int M, int Q, int D, const double *mx, const double 
   GPX_CHECK_LAUNCH();
   // per-pair pass over point ranges
   const int nchunk = ((N + nsplit - 1) / nsplit + 15) / 16 * 16;
-  dim3 gp(ntiles * 4, nsplit);
+  dim3 gp(ntiles * 2, nsplit);
   if (D <= 16)
     hipLaunchKernelGGL((fused_bwd_pairs_kernel<1, 4>), gp, dim3(256), 0, st, N, M, Q, D, z, ls, sf, Bp, dv,
                        (const double *)pf, nsplit, nchunk, dBpart, LWpart);
