@@ -68,6 +68,8 @@ __global__ __launch_bounds__(256) void serve_kernel(const GpxServeModel *__restr
   __shared__ double red[4][SERVE_TILE];
   __shared__ double tm[SERVE_TILE], tv[SERVE_TILE];
   __shared__ double pc[SERVE_TILE], pd[SERVE_TILE][SERVE_QMAX];
+  __shared__ double red2[4][16][17];
+  __shared__ double vqs[SERVE_TILE][16];
   const GpxServeItem it = items[blockIdx.x];
   const GpxServeModel &mod = models[it.model];
   const int tid = threadIdx.x, rows = it.rows;
@@ -110,6 +112,92 @@ __global__ __launch_bounds__(256) void serve_kernel(const GpxServeModel *__restr
       pc[tid] = c;
     }
     __syncthreads();
+    if (L.type == 1) {
+      // <B_d, psi2_p> for all d at once on the matrix cores: psi2 exponents of a 16-pair x
+      // 16-point block as a K = 2Q+2 feature product, one exp per entry, then the
+      // contraction with the symmetrised B_d; pairs (a, b) visited as tiles A <= B
+      // (off-diagonal weight 2), row blocks split over the 4 waves, partial sums
+      // combined in wave order.
+      const int lane = tid & 63, wave = tid >> 6, pp = lane & 15;
+      const int KS = (2 * Q + 2 + 3) / 4;
+      double il4[SERVE_HMAX], lam[SERVE_HMAX];
+#pragma unroll
+      for (int q = 0; q < SERVE_HMAX; ++q) {
+        lam[q] = (q < Q) ? exp(2.0 * L.ls[q]) : 1.0;
+        il4[q] = 0.25 / lam[q];
+      }
+      double bf[5];
+      {
+        double c = 0.0;
+        for (int q = 0; q < Q; ++q) {
+          const double v = hv[pp][q], mu = hm[pp][q], dd = 1.0 / (2.0 * v + lam[q]);
+          c += 0.5 * log(lam[q] / (lam[q] + 2.0 * v)) - dd * mu * mu;
+        }
+#pragma unroll
+        for (int s2 = 0; s2 < 5; ++s2) {
+          const int k = (lane >> 4) + 4 * s2;
+          double f = 0.0;
+          if (s2 < KS) {
+            if (k < Q) f = 2.0 * hm[pp][k] / (2.0 * hv[pp][k] + lam[k]);
+            else if (k < 2 * Q) f = 1.0 / (2.0 * hv[pp][k - Q] + lam[k - Q]);
+            else if (k == 2 * Q) f = 1.0;
+            else if (k == 2 * Q + 1) f = c;
+          }
+          bf[s2] = f;
+        }
+      }
+      const double sf4 = sf2 * sf2;
+      const int T = (M + 15) / 16;
+      gpx_d4 acc = gpx_d4{0.0, 0.0, 0.0, 0.0};
+      for (int B = 0; B < T; ++B)
+        for (int A = 0; A <= B; ++A) {
+          const double wgt = (A == B) ? sf4 : 2.0 * sf4;
+          for (int i = wave; i < 16; i += 4) {
+            const int a = 16 * A + i;
+            if (a >= M) break;
+            const int ja = 16 * B + (lane & 15);
+            gpx_d4 e = gpx_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s2 = 0; s2 < 5; ++s2) {
+              if (s2 < KS) {
+                const int k = (lane >> 4) + 4 * s2;
+                double af = 0.0;
+                if (ja < M) {
+                  if (k < Q) af = 0.5 * (zs[a][k] + zs[ja][k]);
+                  else if (k < 2 * Q) {
+                    const double hh = 0.5 * (zs[a][k - Q] + zs[ja][k - Q]);
+                    af = -hh * hh;
+                  } else if (k == 2 * Q) {
+                    for (int q = 0; q < Q; ++q) {
+                      const double dz = zs[a][q] - zs[ja][q];
+                      af -= dz * dz * il4[q];
+                    }
+                  } else if (k == 2 * Q + 1) af = 1.0;
+                }
+                e = gpx_mfma16(af, bf[s2], e);
+              }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int b = 16 * B + (lane >> 4) + 4 * r, d = lane & 15;
+              const double pv = (b < M) ? wgt * exp(e[r]) : 0.0;
+              const double bd = (b < M && d < L.Dout)
+                                    ? 0.5 * (L.B[((long)d * M + a) * M + b] + L.B[((long)d * M + b) * M + a])
+                                    : 0.0;
+              acc = gpx_mfma16(bd, pv, acc);
+            }
+          }
+        }
+      // acc register r of lane: (d = (lane>>4) + 4r, point = lane & 15)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) red2[wave][(lane >> 4) + 4 * r][lane & 15] = acc[r];
+      __syncthreads();
+      {
+        const int d = tid >> 4, p = tid & 15;
+        vqs[p][d] = red2[0][d][p] + red2[1][d][p] + red2[2][d][p] + red2[3][d][p];
+      }
+      __syncthreads();
+    }
     for (int d = 0; d < L.Dout; ++d) {
       const double *Ad = L.A + (long)d * M, *Bd = L.B + (long)d * M * M;
       double vm[SERVE_TILE], vq[SERVE_TILE];
@@ -131,26 +219,9 @@ __global__ __launch_bounds__(256) void serve_kernel(const GpxServeModel *__restr
           }
 #pragma unroll
           for (int p = 0; p < SERVE_TILE; ++p) vq[p] = feat[p][b] * w[p];
-        } else {
-          double il4[SERVE_QMAX];
-          for (int q = 0; q < Q; ++q) il4[q] = 0.25 / exp(2.0 * L.ls[q]);
-          for (int a = 0; a < M; ++a) {
-            const double bab = Bd[(long)a * M + b];
-            double dz2 = 0.0;
-            for (int q = 0; q < Q; ++q) {
-              const double dz = zs[a][q] - zs[b][q];
-              dz2 += dz * dz * il4[q];
-            }
+        } else if (tid == 0) {
 #pragma unroll
-            for (int p = 0; p < SERVE_TILE; ++p) {
-              double e = pc[p] - dz2;
-              for (int q = 0; q < Q; ++q) {
-                const double h = hm[p][q] - 0.5 * (zs[a][q] + zs[b][q]);
-                e -= h * h * pd[p][q];
-              }
-              vq[p] = fma(bab, sf2 * sf2 * exp(e), vq[p]);
-            }
-          }
+          for (int p = 0; p < SERVE_TILE; ++p) vq[p] = vqs[p][d];
         }
       }
       tile_sum(vm, red, tm);
@@ -211,6 +282,7 @@ int gpx_serve_validate(const void *model) {
     if (L.type != (l == 0 ? 0 : 1)) return GPX_EINVAL;
     if (L.M < 1 || L.M > SERVE_MMAX || L.Din != din || L.Dout < 1) return GPX_EINVAL;
     if (l < m->nlayers - 1 && L.Dout > SERVE_HMAX) return GPX_EINVAL;
+    if (L.type == 1 && L.Dout > 16) return GPX_EINVAL;  // one 16-row MFMA tile of outputs
     if (!L.zu || !L.ls || !L.sf || !L.A || !L.B) return GPX_EINVAL;
     din = L.Dout;
   }

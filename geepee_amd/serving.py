@@ -93,56 +93,82 @@ class ModelStore(object):
         torch.cuda.synchronize()
         self.Din = [int(sp['Din']) for sp in specs]
         self.Dout = [int(descs[i].Dout) for i in range(len(specs))]
+        self._din = np.asarray(self.Din, dtype=np.int64)
+        self._dout = np.asarray(self.Dout, dtype=np.int64)
+        self._bufs = {}
+        assert self.ITEM_DTYPE.itemsize == ctypes.sizeof(_Item)
         self.state_bytes = sum(t.numel() * 8 for L in self.layers for l in L
                                for t in (l.A, l.B_det, l.B_sto, l.zu))
 
     # -- one grouped launch ------------------------------------------------------
+    ITEM_DTYPE = np.dtype([('model', '<i4'), ('rows', '<i4'), ('in_off', '<i8'), ('out_off', '<i8')])
+
     def plan(self, reqs):
-        """Work-item table and I/O offsets for requests [(model_id, X)]."""
-        items = []
-        in_off = out_off = 0
-        offs = []
-        for mid, X in reqs:
-            n = X.shape[0]
-            offs.append((in_off, out_off, n))
-            for r0 in range(0, n, self.tile):
-                rows = min(self.tile, n - r0)
-                items.append((mid, rows, in_off + r0 * self.Din[mid], out_off + r0 * self.Dout[mid]))
-            in_off += n * self.Din[mid]
-            out_off += n * self.Dout[mid]
-        arr = (_Item * max(1, len(items)))()
-        for j, (m, r, io, oo) in enumerate(items):
-            arr[j].model, arr[j].rows, arr[j].in_off, arr[j].out_off = m, r, io, oo
-        return arr, len(items), in_off, out_off, offs
+        """Work-item table (structured array in the GpxServeItem layout) and I/O offsets
+        for requests [(model_id, X)], built with vectorised numpy."""
+        mids = np.fromiter((m for m, _ in reqs), dtype=np.int64, count=len(reqs))
+        ns = np.fromiter((X.shape[0] for _, X in reqs), dtype=np.int64, count=len(reqs))
+        din = self._din[mids]
+        dout = self._dout[mids]
+        in_sz, out_sz = ns * din, ns * dout
+        in_off = np.concatenate(([0], np.cumsum(in_sz)[:-1]))
+        out_off = np.concatenate(([0], np.cumsum(out_sz)[:-1]))
+        ntile = -(-ns // self.tile)
+        ridx = np.repeat(np.arange(len(reqs)), ntile)
+        first = np.concatenate(([0], np.cumsum(ntile)[:-1]))
+        j = np.arange(int(ntile.sum())) - np.repeat(first, ntile)
+        items = np.empty(len(ridx), dtype=self.ITEM_DTYPE)
+        items['model'] = mids[ridx]
+        items['rows'] = np.minimum(self.tile, ns[ridx] - j * self.tile)
+        items['in_off'] = in_off[ridx] + j * self.tile * din[ridx]
+        items['out_off'] = out_off[ridx] + j * self.tile * dout[ridx]
+        offs = list(zip(in_off.tolist(), out_off.tolist(), ns.tolist()))
+        return items, int(in_sz.sum()), int(out_sz.sum()), offs
+
+    def _slot_buffers(self, key, nin, nout, nitems):
+        """Per-stream pinned host and device buffers, grown on demand and reused."""
+        b = self._bufs.get(key)
+        need = (max(nin, 1), max(nout, 1), max(nitems, 1))
+        if b is None or any(x > y for x, y in zip(need, b['cap'])):
+            cap = tuple(max(2 * x, 1024) for x in need)
+            b = {'cap': cap,
+                 'xh': torch.empty(cap[0], dtype=torch.float64, pin_memory=True),
+                 'ih': torch.empty(cap[2] * self.ITEM_DTYPE.itemsize, dtype=torch.uint8, pin_memory=True),
+                 'mh': torch.empty(cap[1], dtype=torch.float64, pin_memory=True),
+                 'vh': torch.empty(cap[1], dtype=torch.float64, pin_memory=True),
+                 'xd': torch.empty(cap[0], dtype=torch.float64, device='cuda'),
+                 'idev': torch.empty(cap[2] * self.ITEM_DTYPE.itemsize, dtype=torch.uint8, device='cuda'),
+                 'md': torch.empty(cap[1], dtype=torch.float64, device='cuda'),
+                 'vd': torch.empty(cap[1], dtype=torch.float64, device='cuda')}
+            self._bufs[key] = b
+        return b
 
     def launch(self, reqs, noisy=False, stream=None, bufs=None):
         """Enqueue one grouped predict for reqs on `stream`; returns a handle whose
-        results are valid after handle['done'] (an event) completes."""
-        arr, nitems, nin, nout, offs = self.plan(reqs)
+        results are valid after handle['done'] (an event) completes.  The stream's
+        buffers are reused, so a stream holds one launch in flight at a time."""
+        items, nin, nout, offs = self.plan(reqs)
         stream = stream or torch.cuda.current_stream()
-        xh = torch.empty(max(nin, 1), dtype=torch.float64, pin_memory=True)
-        xv = xh.numpy()
+        b = self._slot_buffers(stream.cuda_stream, nin, nout, len(items))
+        xv = b['xh'].numpy()
         o = 0
-        for mid, X in reqs:
+        for _, X in reqs:
             xv[o:o + X.size] = np.asarray(X, dtype=np.float64).ravel()
             o += X.size
-        ih = torch.from_numpy(np.frombuffer(bytes(arr), dtype=np.uint8).copy()).pin_memory()
+        ib = items.view(np.uint8)
+        b['ih'].numpy()[:ib.size] = ib
         with torch.cuda.stream(stream):
-            xd = xh.to('cuda', non_blocking=True)
-            idev = ih.to('cuda', non_blocking=True)
-            mean = torch.empty(max(nout, 1), dtype=torch.float64, device='cuda')
-            var = torch.empty(max(nout, 1), dtype=torch.float64, device='cuda')
-            _lib.call('gpx_predict_grouped', nitems, self.desc_dev.data_ptr(), idev.data_ptr(),
-                      xd.data_ptr(), mean.data_ptr(), var.data_ptr(), int(bool(noisy)),
+            b['xd'][:max(nin, 1)].copy_(b['xh'][:max(nin, 1)], non_blocking=True)
+            b['idev'][:ib.size].copy_(b['ih'][:ib.size], non_blocking=True)
+            _lib.call('gpx_predict_grouped', len(items), self.desc_dev.data_ptr(), b['idev'].data_ptr(),
+                      b['xd'].data_ptr(), b['md'].data_ptr(), b['vd'].data_ptr(), int(bool(noisy)),
                       stream.cuda_stream)
-            mh = torch.empty(max(nout, 1), dtype=torch.float64, pin_memory=True)
-            vh = torch.empty(max(nout, 1), dtype=torch.float64, pin_memory=True)
-            mh.copy_(mean, non_blocking=True)
-            vh.copy_(var, non_blocking=True)
+            b['mh'][:max(nout, 1)].copy_(b['md'][:max(nout, 1)], non_blocking=True)
+            b['vh'][:max(nout, 1)].copy_(b['vd'][:max(nout, 1)], non_blocking=True)
             done = torch.cuda.Event()
             done.record(stream)
-        return {'done': done, 'mh': mh, 'vh': vh, 'offs': offs, 'reqs': reqs,
-                'keep': (xh, ih, xd, idev, mean, var), 'bytes': 8 * (nin + 2 * nout)}
+        return {'done': done, 'mh': b['mh'], 'vh': b['vh'], 'offs': offs, 'reqs': reqs,
+                'bytes': 8 * (nin + 2 * nout)}
 
     def collect(self, h):
         h['done'].synchronize()
