@@ -20,6 +20,8 @@ SIGNATURES = {
     'gpx_gemm_workspace': (_l, [_i, _i, _i, _i]),
     'gpx_gemm_batched': (_i, [_i, _i, _i, _i, _i, _d, _p, _i, _l, _p, _i, _l, _d, _p, _i, _l,
                               _i, _i, _p, _i, _p]),
+    'gpx_gemm_batched_ex': (_i, [_i, _i, _i, _i, _i, _d, _p, _i, _l, _p, _i, _l, _d, _p, _i, _l,
+                                 _d, _p, _i, _l, _i, _i, _p, _i, _p]),
     'gpx_spd_max_m': (_i, []),
     'gpx_spd_inv_batched': (_i, [_i, _i, _p, _i, _l, _p, _i, _l, _p, _p, _p]),
     'gpx_se_kernel': (_i, [_i, _i, _i, _p, _i, _p, _p, _p, _d, _p, _p]),
@@ -47,6 +49,7 @@ SIGNATURES = {
     'gpx_dmall': (_i, [_l, _p, _p, _p, _p, _p]),
     'gpx_hyper_grads': (_i, [_i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
     'gpx_phi': (_i, [_i, _p, _p, _d, _p, _p, _d, _p, _d, _d, _p, _p]),
+    'gpx_phi_nat': (_i, [_i, _i, _p, _p, _p, _p, _p, _p, _d, _d, _d, _p, _p]),
     'gpx_lvm_x_cavity': (_i, [_i, _i, _p, _p, _p, _d, _d, _d, _i, _p, _p, _p]),
     'gpx_lvm_x_grads': (_i, [_i, _i, _i, _p, _p, _p, _p, _p, _d, _d, _d, _i, _p, _p, _p, _p]),
     'gpx_ssm_x_cavity': (_i, [_i, _i, _p, _p, _d, _d, _d, _p, _p, _p]),
@@ -173,9 +176,10 @@ def gemm(tA, tB, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch
          float(beta), P(C), ldc, sC, batch, splits, P(work), tile, S())
 
 
-def mm(A, B, C, tA=False, tB=False, alpha=1.0, beta=0.0, splits=1, work=None, tile=0):
-    """C = alpha op(A) op(B) + beta C for 2-D or batched 3-D contiguous tensors; a 2-D
-    operand is broadcast over the batch of a 3-D one."""
+def mm(A, B, C, tA=False, tB=False, alpha=1.0, beta=0.0, splits=1, work=None, tile=0,
+       E=None, gamma=0.0):
+    """C = alpha op(A) op(B) + beta C + gamma E for 2-D or batched 3-D contiguous
+    tensors; a 2-D operand (E included) is broadcast over the batch of a 3-D one."""
     batch = max(t.shape[0] if t.dim() == 3 else 1 for t in (A, B, C))
     ar, ac = A.shape[-2:]
     br, bc = B.shape[-2:]
@@ -190,7 +194,13 @@ def mm(A, B, C, tA=False, tB=False, alpha=1.0, beta=0.0, splits=1, work=None, ti
     sC = m * n if C.dim() == 3 else 0
     if splits > 1 and work is None:
         work = torch.empty(splits * batch * m * n, dtype=torch.float64, device=C.device)
-    gemm(tA, tB, m, n, k, alpha, A, ac, sA, B, bc, sB, beta, C, n, sC, batch, splits, work, tile)
+    if E is None:
+        gemm(tA, tB, m, n, k, alpha, A, ac, sA, B, bc, sB, beta, C, n, sC, batch, splits, work, tile)
+        return C
+    assert E.is_contiguous() and E.dtype == torch.float64 and E.shape[-2:] == (m, n), E.shape
+    sE = m * n if E.dim() == 3 else 0
+    call('gpx_gemm_batched_ex', int(tA), int(tB), m, n, k, float(alpha), P(A), ac, sA, P(B), bc, sB,
+         float(beta), P(C), n, sC, float(gamma), P(E), n, sE, batch, splits, P(work), tile, S())
     return C
 
 

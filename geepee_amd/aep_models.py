@@ -14,7 +14,7 @@ import pickle
 import numpy as np
 import torch
 
-from . import _lib, dops
+from . import _lib, config, dops
 from .config import PROP_MM, PROP_MC
 from .dops import mm, lincomb, colsum
 from . import init
@@ -390,7 +390,7 @@ class SDGPR(Base_Model):
         grads = {}
         # the hyper-parameter chain of each moment-matched layer forks onto that layer's
         # stream while the input gradients carry on down the layers on this one
-        streams = getattr(self, '_streams', None) if self.L > 1 else None
+        streams = getattr(self, '_streams', None) if self.L > 1 and config.MULTI_STREAM else None
         keep = []
         for i in range(self.L - 1, -1, -1):
             layer = self.sgp_layers[i]
@@ -422,10 +422,12 @@ class SDGPR(Base_Model):
         dp = self._device_params(params)
         self.lik_layer.update_hypers(dp)
         cur = torch.cuda.current_stream()
-        if self.L == 1:
-            L = self.sgp_layers[0]
-            L.prepare(dp, alpha, key_suffix='_0')
-            return [L.compute_phi(alpha)]
+        if self.L == 1 or not config.MULTI_STREAM:
+            phis = []
+            for i, layer in enumerate(self.sgp_layers):
+                layer.prepare(dp, alpha, key_suffix='_%d' % i)
+                phis.append(layer.compute_phi(alpha))
+            return phis
         if getattr(self, '_streams', None) is None:
             self._streams = [torch.cuda.Stream() for _ in range(self.L)]
         phis = []

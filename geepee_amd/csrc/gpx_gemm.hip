@@ -12,6 +12,22 @@ namespace {
 
 constexpr int BK = 32;
 
+// Output epilogue: C = v + beta C + gamma E (beta == 0 never reads C; E may be NULL; an
+// E stride of 0 broadcasts one matrix over the batch).
+struct GemmEpi {
+  double beta, gamma;
+  const double *E;
+  int ldE;
+  long sE;
+};
+
+__device__ __forceinline__ void epi_store(double *c, double v, const GemmEpi &ep, int b, int i, int j) {
+  double r = v;
+  if (ep.beta != 0.0) r += ep.beta * *c;
+  if (ep.E) r += ep.gamma * ep.E[b * ep.sE + (long)i * ep.ldE + j];
+  *c = r;
+}
+
 // Stage one BK-slice of op(A) (BM x BK) and op(B) (BK x BN) into registers.
 template <int BM, int BN>
 __device__ __forceinline__ void gemm_load(double *ra, double *rb, const double *__restrict__ A,
@@ -66,7 +82,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(
     int m, int n, int k, int tA, int tB, double alpha,
     const double *__restrict__ A, int lda, long sA,
     const double *__restrict__ B, int ldb, long sB,
-    double beta, double *__restrict__ C, int ldc, long sC,
+    GemmEpi ep, double *__restrict__ C, int ldc, long sC,
     int batch, int splits, int kchunk, double *__restrict__ part) {
   constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
   constexpr int LA = BM * BK / 256, LB = BN * BK / 256;
@@ -136,16 +152,12 @@ __global__ __launch_bounds__(256) void gemm_kernel(
       for (int r = 0; r < 4; ++r) {
         const int gm = m0 + wm * WM + i * 16 + (lane >> 4) + 4 * r;
         const int gn = n0 + wn * WN + j * 16 + (lane & 15);
-        if (gm < m && gn < n) {
-          double *c = C + (long)gm * ldc + gn;
-          const double v = alpha * acc[i][j][r];
-          *c = (beta == 0.0) ? v : v + beta * *c;
-        }
+        if (gm < m && gn < n) epi_store(C + (long)gm * ldc + gn, alpha * acc[i][j][r], ep, bidx, gm, gn);
       }
 }
 
 __global__ void splitk_reduce(int m, int n, int batch, int splits, double alpha,
-                              const double *__restrict__ part, double beta,
+                              const double *__restrict__ part, GemmEpi ep,
                               double *__restrict__ C, int ldc, long sC) {
   const long total = (long)batch * m * n;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
@@ -155,9 +167,7 @@ __global__ void splitk_reduce(int m, int n, int batch, int splits, double alpha,
     const int i = (int)(r / n), j = (int)(r % n);
     double s = 0.0;
     for (int q = 0; q < splits; ++q) s += part[((long)q * batch + b) * (long)m * n + r];
-    double *c = C + b * sC + (long)i * ldc + j;
-    const double v = alpha * s;
-    *c = (beta == 0.0) ? v : v + beta * *c;
+    epi_store(C + b * sC + (long)i * ldc + j, alpha * s, ep, b, i, j);
   }
 }
 
@@ -171,7 +181,7 @@ __global__ __launch_bounds__(256) void gemm_fullk_kernel(
     int m, int n, int k, int tA, int tB, double alpha,
     const double *__restrict__ A, int lda, long sA,
     const double *__restrict__ B, int ldb, long sB,
-    double beta, double *__restrict__ C, int ldc, long sC, int batch) {
+    GemmEpi ep, double *__restrict__ C, int ldc, long sC, int batch) {
   __shared__ double As[FULLK_MAX][33];
   __shared__ double Bs[FULLK_MAX][33];
   const int bidx = blockIdx.z;
@@ -216,11 +226,7 @@ __global__ __launch_bounds__(256) void gemm_fullk_kernel(
   for (int r = 0; r < 4; ++r) {
     const int gm = m0 + wm * 16 + (lane >> 4) + 4 * r;
     const int gn = n0 + wn * 16 + (lane & 15);
-    if (gm < m && gn < n) {
-      double *c = C + (long)gm * ldc + gn;
-      const double v = alpha * acc[r];
-      *c = (beta == 0.0) ? v : v + beta * *c;
-    }
+    if (gm < m && gn < n) epi_store(C + (long)gm * ldc + gn, alpha * acc[r], ep, bidx, gm, gn);
   }
 }
 
@@ -233,19 +239,22 @@ long gpx_gemm_workspace(int m, int n, int batch, int splits) {
   return splits > 1 ? (long)splits * batch * m * n : 0;
 }
 
-int gpx_gemm_batched(int transA, int transB, int m, int n, int k, double alpha,
-                     const double *A, int lda, long strideA,
-                     const double *B, int ldb, long strideB,
-                     double beta, double *C, int ldc, long strideC,
-                     int batch, int splits, double *work, int tile, void *stream) {
+int gpx_gemm_batched_ex(int transA, int transB, int m, int n, int k, double alpha,
+                        const double *A, int lda, long strideA,
+                        const double *B, int ldb, long strideB,
+                        double beta, double *C, int ldc, long strideC,
+                        double gamma, const double *E, int ldE, long strideE,
+                        int batch, int splits, double *work, int tile, void *stream) {
   if (m < 0 || n < 0 || k < 0 || batch < 0 || splits < 1) return GPX_EINVAL;
+  if (E && (ldE < n || strideE < 0)) return GPX_EINVAL;
+  GemmEpi ep{beta, gamma, E, ldE, strideE};
   if (m == 0 || n == 0 || batch == 0) return GPX_OK;
   if (!A || !B || !C) return GPX_EINVAL;
   if (splits > 1 && !work) return GPX_EINVAL;
   hipStream_t st = (hipStream_t)stream;
   if (k == 0) {  // C = beta * C
     hipLaunchKernelGGL(splitk_reduce, dim3(gpx_cdiv((long)batch * m * n, 256) > 4096 ? 4096 : gpx_cdiv((long)batch * m * n, 256)),
-                       dim3(256), 0, st, m, n, batch, 0, 0.0, (const double *)nullptr, beta, C, ldc, strideC);
+                       dim3(256), 0, st, m, n, batch, 0, 0.0, (const double *)nullptr, ep, C, ldc, strideC);
     GPX_CHECK_LAUNCH();
     return GPX_OK;
   }
@@ -254,7 +263,7 @@ int gpx_gemm_batched(int transA, int transB, int m, int n, int k, double alpha,
     if (batch > 65535) return GPX_EINVAL;
     dim3 grid(gpx_cdiv(n, 32), gpx_cdiv(m, 32), batch);
     hipLaunchKernelGGL(gemm_fullk_kernel, grid, dim3(256), 0, st, m, n, k, transA, transB, alpha, A, lda, strideA,
-                       B, ldb, strideB, beta, C, ldc, strideC, batch);
+                       B, ldb, strideB, ep, C, ldc, strideC, batch);
     GPX_CHECK_LAUNCH();
     return GPX_OK;
   }
@@ -266,11 +275,11 @@ int gpx_gemm_batched(int transA, int transB, int m, int n, int k, double alpha,
   if (tile == 32 || (tile == 0 && (long)gpx_cdiv(m, 64) * gpx_cdiv(n, 64) * batch * splits < 256)) {
     dim3 grid(gpx_cdiv(n, 32), gpx_cdiv(m, 32), batch * splits);
     hipLaunchKernelGGL((gemm_kernel<32, 32>), grid, dim3(256), 0, st, m, n, k, transA, transB, alpha,
-                       A, lda, strideA, B, ldb, strideB, beta, C, ldc, strideC, batch, splits, kchunk, part);
+                       A, lda, strideA, B, ldb, strideB, ep, C, ldc, strideC, batch, splits, kchunk, part);
   } else {
     dim3 grid(gpx_cdiv(n, 64), gpx_cdiv(m, 64), batch * splits);
     hipLaunchKernelGGL((gemm_kernel<64, 64>), grid, dim3(256), 0, st, m, n, k, transA, transB, alpha,
-                       A, lda, strideA, B, ldb, strideB, beta, C, ldc, strideC, batch, splits, kchunk, part);
+                       A, lda, strideA, B, ldb, strideB, ep, C, ldc, strideC, batch, splits, kchunk, part);
   }
   GPX_CHECK_LAUNCH();
   if (part) {
@@ -278,10 +287,19 @@ int gpx_gemm_batched(int transA, int transB, int m, int n, int k, double alpha,
     unsigned g = gpx_cdiv(total, 256);
     if (g > 4096) g = 4096;
     hipLaunchKernelGGL(splitk_reduce, dim3(g), dim3(256), 0, st, m, n, batch, splits, alpha,
-                       (const double *)part, beta, C, ldc, strideC);
+                       (const double *)part, ep, C, ldc, strideC);
     GPX_CHECK_LAUNCH();
   }
   return GPX_OK;
+}
+
+int gpx_gemm_batched(int transA, int transB, int m, int n, int k, double alpha,
+                     const double *A, int lda, long strideA,
+                     const double *B, int ldb, long strideB,
+                     double beta, double *C, int ldc, long strideC,
+                     int batch, int splits, double *work, int tile, void *stream) {
+  return gpx_gemm_batched_ex(transA, transB, m, n, k, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc,
+                             strideC, 0.0, nullptr, 0, 0, batch, splits, work, tile, stream);
 }
 
 }  // extern "C"

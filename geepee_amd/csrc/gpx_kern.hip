@@ -231,37 +231,42 @@ __global__ void eta_to_R_kernel(int D, int M, const double *__restrict__ eta, do
 }
 
 // out[d, triu(i,j)] = (sum_{k>=i} R_ik (G_kj + G_jk)) * (i == j ? R_ii : 1)
-// One workgroup per (d, 16 rows i, 64 columns j); k advances in slices of 16 staged in
-// LDS, the G_jk half read along k (rows of G) so both halves load coalesced.
+// One workgroup per (d, 16 rows i, 64 columns j); k advances in slices of 64 (at most
+// two for M <= 128).  Each slice issues all of its loads (R rows, G rows and the G^T
+// half read along rows of G, so every load is coalesced) before one barrier, which keeps
+// the kernel at one global round trip per slice.
 __global__ __launch_bounds__(256) void pack_dR_kernel(int D, int M, const double *__restrict__ R,
                                                       const double *__restrict__ G, double *__restrict__ out) {
-  __shared__ double Rs[16][17];
-  __shared__ double Ss[16][65];
+  constexpr int KC = 64;
+  __shared__ double Rs[16][KC + 1];
+  __shared__ double Ss[KC][65];
+  __shared__ double St[64][KC + 1];
   const int d = blockIdx.z, i0 = blockIdx.y * 16, j0 = blockIdx.x * 64;
   if (j0 + 63 < i0) return;  // tile entirely below the diagonal
   const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15;
   const double *Rd = R + (long)d * M * M, *Gd = G + (long)d * M * M;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int k0 = i0; k0 < M; k0 += 16) {
-    {
-      const int r = tid >> 4, kk = tid & 15, i = i0 + r, k = k0 + kk;
+  for (int k0 = i0; k0 < M; k0 += KC) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + 256 * u, r = e >> 6, kk = e & 63, i = i0 + r, k = k0 + kk;
       Rs[r][kk] = (i < M && k < M && k >= i) ? Rd[(long)i * M + k] : 0.0;
     }
-    for (int e = tid; e < 16 * 64; e += 256) {
-      const int kk = e >> 6, jj = e & 63, k = k0 + kk, j = j0 + jj;
-      Ss[kk][jj] = (k < M && j < M) ? Gd[(long)k * M + j] : 0.0;
-    }
-    __syncthreads();
-    for (int e = tid; e < 16 * 64; e += 256) {
-      const int jj = e >> 4, kk = e & 15, k = k0 + kk, j = j0 + jj;
-      if (k < M && j < M) Ss[kk][jj] += Gd[(long)j * M + k];
-    }
-    __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < 16; ++kk) {
+    for (int u = 0; u < 16; ++u) {
+      const int e = tid + 256 * u, hi = e >> 6, lo = e & 63;
+      // Ss[kk][jj] = G[k0 + kk][j0 + jj];  St[jj][kk] = G[j0 + jj][k0 + kk]
+      const int k = k0 + hi, j = j0 + lo;
+      Ss[hi][lo] = (k < M && j < M) ? Gd[(long)k * M + j] : 0.0;
+      const int jt = j0 + hi, kt = k0 + lo;
+      St[hi][lo] = (jt < M && kt < M) ? Gd[(long)jt * M + kt] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int kk = 0; kk < KC; ++kk) {
       const double r = Rs[ti][kk];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) acc[c] += r * Ss[kk][tj + 16 * c];
+      for (int c = 0; c < 4; ++c) acc[c] = fma(r, Ss[kk][tj + 16 * c] + St[tj + 16 * c][kk], acc[c]);
     }
     __syncthreads();
   }
@@ -620,6 +625,36 @@ __global__ void phi_kernel(int D, const double *__restrict__ ldK, const double *
   out[0] = D * 0.5 * ldK[0] + sp * 0.5 * p + sc * 0.5 * c;
 }
 
+// Natural-parameter phi with the quadratic terms taken from theta2: Sigma^-1 mu = theta2
+// and Sigma-hat^-1 mu-hat = beta theta2, so mu^T Sigma^-1 mu = mu . theta2 and
+// mu-hat^T Sigma-hat^-1 mu-hat = beta mu-hat . theta2.  One workgroup; the partial sums
+// and their reduction run in a fixed order.
+__global__ __launch_bounds__(256) void phi_nat_kernel(int D, int M, const double *__restrict__ ldK,
+                                                      const double *__restrict__ ldP,
+                                                      const double *__restrict__ ldC,
+                                                      const double *__restrict__ mu,
+                                                      const double *__restrict__ muhat,
+                                                      const double *__restrict__ th2, double beta,
+                                                      double sp, double sc, double *__restrict__ out) {
+  __shared__ double red[16];
+  double p = 0.0, c = 0.0;
+  const long n = (long)D * M;
+  for (long e = threadIdx.x; e < n; e += 256) {
+    p = fma(mu[e], th2[e], p);
+    c = fma(muhat[e], th2[e], c);
+  }
+  p = gpx_block_sum(p, red);
+  c = gpx_block_sum(c, red);
+  if (threadIdx.x == 0) {
+    double lp = 0.0, lc = 0.0;
+    for (int d = 0; d < D; ++d) {
+      lp -= ldP[d];
+      lc -= ldC[d];
+    }
+    out[0] = D * 0.5 * ldK[0] + sp * 0.5 * (lp + p) + sc * 0.5 * (lc + beta * c);
+  }
+}
+
 // my = C mf, vy_d = sum_c C_dc^2 vf_c + R_d (diagonal of lik_layers.py:558-571)
 __global__ void emission_predict_kernel(int Nb, int Dout, int Din, const double *__restrict__ C,
                                         const double *__restrict__ Rp, const double *__restrict__ mf,
@@ -903,6 +938,16 @@ int gpx_phi(int D, const double *ldK, const double *ldP, double sgnP, const doub
   if (D < 1 || !ldK || !ldP || !dotP || !ldC || !dotC || !out) return GPX_EINVAL;
   hipLaunchKernelGGL(phi_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, D, ldK, ldP, sgnP, dotP, ldC, sgnC,
                      dotC, sp, sc, out);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+int gpx_phi_nat(int D, int M, const double *ldK, const double *ldP, const double *ldC, const double *mu,
+                const double *muhat, const double *theta2, double beta, double sp, double sc, double *out,
+                void *stream) {
+  if (D < 1 || M < 1 || !ldK || !ldP || !ldC || !mu || !muhat || !theta2 || !out) return GPX_EINVAL;
+  hipLaunchKernelGGL(phi_nat_kernel, dim3(1), dim3(256), 0, (hipStream_t)stream, D, M, ldK, ldP, ldC, mu,
+                     muhat, theta2, beta, sp, sc, out);
   GPX_CHECK_LAUNCH();
   return GPX_OK;
 }

@@ -13,8 +13,9 @@ def empty(*shape, like=None, device='cuda'):
     return torch.empty(*shape, dtype=F64, device=like.device if like is not None else device)
 
 
-def mm(A, B, C=None, tA=False, tB=False, alpha=1.0, beta=0.0, splits=None):
-    """C = alpha op(A) op(B) + beta C (2-D or batched 3-D; 2-D operands broadcast).
+def mm(A, B, C=None, tA=False, tB=False, alpha=1.0, beta=0.0, splits=None, E=None, gamma=0.0):
+    """C = alpha op(A) op(B) + beta C + gamma E (2-D or batched 3-D; 2-D operands
+    broadcast; E is added in the product's own epilogue).
     Picks a split-K factor when the output has few tiles and K is long (a tuned plan
     for the shape, if the autotuner stored one, wins)."""
     if C is None:
@@ -38,7 +39,8 @@ def mm(A, B, C=None, tA=False, tB=False, alpha=1.0, beta=0.0, splits=None):
             if tiles < 256 and k >= 256:
                 # long, skinny contractions: split K so that enough workgroups run
                 splits = max(1, min(-(-256 // tiles), k // 128, 64))
-    return _lib.mm(A, B, C, tA=tA, tB=tB, alpha=alpha, beta=beta, splits=splits, tile=tile)
+    return _lib.mm(A, B, C, tA=tA, tB=tB, alpha=alpha, beta=beta, splits=splits, tile=tile,
+                   E=E, gamma=gamma)
 
 
 def spd_inv(A, out=None):
@@ -135,6 +137,13 @@ def hyper(ls, sf, dell, gls, dsf2, sumdv, gsf):
 def phi(D, ldK, ldP, sgnP, dotP, ldC, sgnC, dotC, sp, sc):
     out = empty(1, like=ldK)
     call('gpx_phi', D, P(ldK), P(ldP), float(sgnP), P(dotP), P(ldC), float(sgnC), P(dotC),
+         float(sp), float(sc), P(out), S())
+    return out
+
+
+def phi_nat(D, M, ldK, ldP, ldC, mu, muhat, theta2, beta, sp, sc):
+    out = empty(1, like=ldK)
+    call('gpx_phi_nat', D, M, P(ldK), P(ldP), P(ldC), P(mu), P(muhat), P(theta2), float(beta),
          float(sp), float(sc), P(out), S())
     return out
 

@@ -23,12 +23,30 @@ def _add_outer(C, u, v, alpha):
     return C
 
 
-def _sandwich(L, X, R, alpha=1.0, out=None):
-    """alpha L X R with L, R (M, M) or (D, M, M) and X (D, M, M)."""
+def _plus_outer(X, u, v, alpha=1.0, gamma=1.0, out=None):
+    """gamma X[d] + alpha u_d v_d^T into a new (or given) stack, in one launch."""
+    if out is None:
+        out = torch.empty_like(X)
+    return mm(_col(u), _col(v), out, tB=True, alpha=alpha, E=X, gamma=gamma)
+
+
+def _sandwich(L, X, R, alpha=1.0, out=None, beta=1.0, E=None, gamma=0.0):
+    """alpha L X R (+ beta out, + gamma E; both added in the second product's epilogue)
+    with L, R (M, M) or (D, M, M) and X (D, M, M)."""
     T = mm(X, R)
     if out is None:
-        return mm(L, T, alpha=alpha)
-    return mm(L, T, out, alpha=alpha, beta=1.0)
+        return mm(L, T, alpha=alpha, E=E, gamma=gamma)
+    return mm(L, T, out, alpha=alpha, beta=beta, E=E, gamma=gamma)
+
+
+_eyes = {}
+
+
+def _eye(M, like):
+    key = (M, like.device)
+    if key not in _eyes:
+        _eyes[key] = torch.eye(M, dtype=like.dtype, device=like.device)
+    return _eyes[key]
 
 
 class SGP_Layer(object):
@@ -54,7 +72,9 @@ class SGP_Layer(object):
         self.zu = params['zu' + key_suffix]
         self.theta_2 = params['eta2' + key_suffix]
         self.theta_1_R = dops.eta_to_R(params['eta1_R' + key_suffix], self.M)
-        self.theta_1 = mm(self.theta_1_R, self.theta_1_R, tA=True)
+        # theta1 = R^T R; the natural parameterisation only ever needs Kuu^-1 + c theta1,
+        # formed by the product's epilogue where it is used
+        self.theta_1 = None if self.nat_param else mm(self.theta_1_R, self.theta_1_R, tA=True)
         self.compute_kuu()
         self.update_posterior()
 
@@ -70,15 +90,15 @@ class SGP_Layer(object):
         self.sf = params['sf' + key_suffix]
         self.zu = params['zu' + key_suffix]
         self.theta_2 = params['eta2' + key_suffix]
-        self.theta_1_R = dops.eta_to_R(params['eta1_R' + key_suffix], self.M)
-        self.theta_1 = mm(self.theta_1_R, self.theta_1_R, tA=True)
+        R = self.theta_1_R = dops.eta_to_R(params['eta1_R' + key_suffix], self.M)
+        self.theta_1 = None
         self.compute_kuu()
         D, M = self.Dout, self.M
         beta = (self.N - alpha) * 1.0 / self.N
         Ki = self.Kuuinv
         prec = dops.empty(2 * D, M, M, like=Ki)
-        lincomb(prec[:D], 1.0, Ki, 1.0, self.theta_1)
-        lincomb(prec[D:], 1.0, Ki, beta, self.theta_1)
+        mm(R, R, prec[:D], tA=True, E=Ki, gamma=1.0)             # Kuu^-1 + theta1
+        mm(R, R, prec[D:], tA=True, alpha=beta, E=Ki, gamma=1.0)  # Kuu^-1 + beta theta1
         cov, ld, info = dops.spd_inv(prec)
         self.Suinv, self.Suhatinv = prec[:D], prec[D:]
         self.Su, self.Suhat = cov[:D], cov[D:]
@@ -86,10 +106,10 @@ class SGP_Layer(object):
         self.info_Su, self.info_Suhat = info[:D], info[D:]
         self.mu = mm(self.Su, _col(self.theta_2)).reshape(D, M)
         self._ld_post, self._sgn_post = self.ld_Suinv, -1.0
-        self.Splusmm = self.Su.clone()
-        _add_outer(self.Splusmm, self.mu, self.mu, 1.0)
+        self.Splusmm = _plus_outer(self.Su, self.mu, self.mu)
         self._pred_ready = False
         self.muhat = mm(self.Suhat, _col(self.theta_2), alpha=beta).reshape(D, M)
+        self._beta = beta
         self._cavity_tail()
 
     def compute_kuu(self):
@@ -99,7 +119,8 @@ class SGP_Layer(object):
     def update_posterior(self):
         D, M = self.Dout, self.M
         if self.nat_param:
-            self.Suinv = lincomb(dops.empty(D, M, M, like=self.Kuu), 1.0, self.Kuuinv, 1.0, self.theta_1)
+            R = self.theta_1_R
+            self.Suinv = mm(R, R, dops.empty(D, M, M, like=self.Kuu), tA=True, E=self.Kuuinv, gamma=1.0)
             self.Su, self.ld_Suinv, self.info_Su = dops.spd_inv(self.Suinv)
             self.mu = mm(self.Su, _col(self.theta_2)).reshape(D, M)
             self._ld_post, self._sgn_post = self.ld_Suinv, -1.0
@@ -108,8 +129,7 @@ class SGP_Layer(object):
             self.Suinv, self.ld_Su, self.info_Su = dops.spd_inv(self.Su)
             self.mu = self.theta_2
             self._ld_post, self._sgn_post = self.ld_Su, 1.0
-        self.Splusmm = self.Su.clone()
-        _add_outer(self.Splusmm, self.mu, self.mu, 1.0)
+        self.Splusmm = _plus_outer(self.Su, self.mu, self.mu)
         self._pred_ready = False
 
     def _posterior_predictive(self):
@@ -118,10 +138,8 @@ class SGP_Layer(object):
             return
         Ki = self.Kuuinv
         self.A = mm(self.mu, Ki, tB=True)
-        self.B_sto = lincomb(torch.empty_like(self.Su), -1.0, Ki)
-        _sandwich(Ki, self.Splusmm, Ki, out=self.B_sto)
-        self.B_det = lincomb(torch.empty_like(self.Su), -1.0, Ki)
-        _sandwich(Ki, self.Su, Ki, out=self.B_det)
+        self.B_sto = _sandwich(Ki, self.Splusmm, Ki, E=Ki, gamma=-1.0)
+        self.B_det = _sandwich(Ki, self.Su, Ki, E=Ki, gamma=-1.0)
         self._pred_ready = True
 
     # ------------------------------------------------------------------
@@ -131,8 +149,10 @@ class SGP_Layer(object):
         D, M = self.Dout, self.M
         beta = (self.N - alpha) * 1.0 / self.N
         Ki = self.Kuuinv
+        self._beta = beta
         if self.nat_param:
-            self.Suhatinv = lincomb(dops.empty(D, M, M, like=Ki), 1.0, Ki, beta, self.theta_1)
+            R = self.theta_1_R
+            self.Suhatinv = mm(R, R, dops.empty(D, M, M, like=Ki), tA=True, alpha=beta, E=Ki, gamma=1.0)
             self.Suhat, self.ld_Suhatinv, self.info_Suhat = dops.spd_inv(self.Suhatinv)
             self.muhat = mm(self.Suhat, _col(self.theta_2), alpha=beta).reshape(D, M)
         else:
@@ -144,19 +164,25 @@ class SGP_Layer(object):
         self._cavity_tail()
 
     def _cavity_tail(self):
-        """Ahat, Bhat_sto, Bhat_det from muhat, Suhat (aep_models.py:535-546)."""
+        """Ahat, Bhat_sto, Bhat_det from muhat, Suhat (aep_models.py:535-546):
+        Bhat_det = Ki Suhat Ki - Ki (the -Ki in the second product's epilogue) and
+        Bhat_sto = Ki (Suhat + muhat muhat^T) Ki - Ki = Bhat_det + Ahat Ahat^T, since
+        Ahat = Ki muhat."""
         Ki = self.Kuuinv
         self.Ahat = mm(self.muhat, Ki, tB=True)
-        self.Splusmmhat = self.Suhat.clone()
-        _add_outer(self.Splusmmhat, self.muhat, self.muhat, 1.0)
-        self.Bhat_sto = lincomb(torch.empty_like(self.Suhat), -1.0, Ki)
-        _sandwich(Ki, self.Splusmmhat, Ki, out=self.Bhat_sto)
-        self.Bhat_det = lincomb(torch.empty_like(self.Suhat), -1.0, Ki)
-        _sandwich(Ki, self.Suhat, Ki, out=self.Bhat_det)
+        self.Splusmmhat = _plus_outer(self.Suhat, self.muhat, self.muhat)
+        self.Bhat_det = _sandwich(Ki, self.Suhat, Ki, E=Ki, gamma=-1.0)
+        self.Bhat_sto = _plus_outer(self.Bhat_det, self.Ahat, self.Ahat)
 
     def compute_phi(self, alpha=1.0):
         """Device scalar (1,) of the weighted log-partitions (aep_models.py:97-114)."""
         N = self.N
+        if self.nat_param:
+            # Sigma^-1 mu = theta2 and Sigma-hat^-1 mu-hat = beta theta2 (aep_models.py:72-95
+            # solve for them); one launch for the whole phi
+            return dops.phi_nat(self.Dout, self.M, self.ld_Kuu, self._ld_post, self.ld_Suhatinv,
+                                self.mu, self.muhat, self.theta_2, self._beta,
+                                N * 1.0 / alpha - 1.0, -N * 1.0 / alpha)
         dotP = dops.batched_dot(self.mu, mm(self.Suinv, _col(self.mu)).reshape(self.Dout, self.M))
         dotC = dops.batched_dot(self.muhat,
                                 mm(self.Suhatinv, _col(self.muhat)).reshape(self.Dout, self.M))
@@ -254,6 +280,8 @@ class SGP_Layer(object):
 
     def backprop_grads_reg(self, m, v, dm, dv, kfu, x, alpha=1.0):
         """backprop_grads_reg (aep_models.py:412-511): dm, dv already scaled."""
+        if self.nat_param:
+            return self._backprop_grads_reg_nat(dm, dv, kfu, x, alpha)
         N = self.N
         D, M = self.Dout, self.M
         B = kfu.shape[0]
@@ -297,6 +325,56 @@ class SGP_Layer(object):
         lincomb(dzu, 1.0, dzu, 1.0, gz)
         deta1 = lincomb(deta1_cav, 1.0, deta1_cav, 1.0, deta1_post)
         deta2 = lincomb(deta2_cav, 1.0, deta2_cav, 1.0, deta2_post)
+        return {'sf': dsf, 'ls': dls, 'zu': dzu, 'eta1_R': deta1, 'eta2': deta2}
+
+    def _backprop_grads_reg_nat(self, dm, dv, kfu, x, alpha):
+        """backprop_grads_reg in the natural parameterisation, with the same terms as the
+        general path (compute_cav_grad_u, compute_posterior_grad_u; aep_models.py:412-511,
+        548-586, base_models.py:490-516) regrouped into fewer launches:
+          Sigma-hat^-1 mu-hat = beta theta2 and Sigma^-1 mu = theta2 (no mat-vecs),
+          the cavity's normalising-factor terms ride in the products' epilogues,
+          dSu (posterior) = sp/2 (Sigma^-1 + theta2 theta2^T)  (its two rank-1 terms
+            -sp/2 theta2 theta2^T and dmu theta2^T = sp theta2 theta2^T combined),
+          deta2 = beta Sigma-hat dmu_cav + sp mu  (Sigma dmu_post = sp Sigma theta2 = sp mu),
+          deta1 = pack(R, beta dSuinv_cav + dSuinv_post) (pack is linear),
+          2 KiS^T dBhat - sum_d dBhat = (2 KiS - I)^T dBhat in one product."""
+        N = self.N
+        D, M = self.Dout, self.M
+        B = kfu.shape[0]
+        Ki = self.Kuuinv
+        th = self.theta_2
+        beta = self._beta
+        scale_post = N * 1.0 / alpha - 1.0
+        scale_cav = -N * 1.0 / alpha
+        kfuKi = mm(kfu, Ki)
+        # grads wrt kfu: dm Ahat + 2 sum_d dv_nd (kfu Bhat_det[d])_n
+        dkfu = mm(dm, self.Ahat)
+        dops.wsum(dv, self._T_det, dkfu, alpha=2.0, beta=1.0)
+        dsf2, dell, dzu, _ = dops.kfu_bwd(dkfu, kfu, x, self.zu, self.ls, self.sf)
+        sumdv = colsum(dv, B * D, 1)
+        # cavity: dmu_cav, dSu_cav (+ the normalising factor), then dSuinv_cav
+        dmucav = mm(dm, kfuKi, tA=True, E=th, gamma=scale_cav * beta)
+        dSucav = mm(dops.rowscale(dv, kfuKi), kfuKi, tA=True, E=self.Suhatinv, gamma=0.5 * scale_cav)
+        _add_outer(dSucav, th, th, -0.5 * scale_cav * beta * beta)
+        _add_outer(dSucav, dmucav, th, beta)
+        dS = dops.empty(2 * D, M, M, like=Ki)
+        _sandwich(self.Suhat, dSucav, self.Suhat, alpha=-1.0, out=dS[:D], beta=0.0)
+        # posterior
+        dSu = _plus_outer(self.Suinv, th, th, alpha=0.5 * scale_post, gamma=0.5 * scale_post)
+        _sandwich(self.Su, dSu, self.Su, alpha=-1.0, out=dS[D:], beta=0.0)
+        deta1 = dops.pack_dR(self.theta_1_R, lincomb(dSu, beta, dS[:D], 1.0, dS[D:]))
+        deta2 = mm(self.Suhat, _col(dmucav), alpha=beta, E=_col(self.mu), gamma=scale_post).reshape(D, M)
+        # Kuuinv: phi prior, Ahat, Bhat_det, cavity, posterior
+        dAhat = mm(dm, kfu, tA=True)
+        dKi = mm(dAhat, self.muhat, tA=True, E=self.Kuu, gamma=-0.5 * D)
+        dBhat = mm(dops.rowscale(dv, kfu), kfu, tA=True)
+        KiS2 = mm(Ki, self.Suhat, alpha=2.0, E=_eye(M, Ki), gamma=-1.0)
+        mm(KiS2.reshape(D * M, M), dBhat.reshape(D * M, M), dKi, tA=True, beta=1.0)
+        colsum(dS, 2 * D, M * M, out=dKi.reshape(-1), beta=1.0)
+        M_inner = _sandwich(Ki, dKi, Ki, alpha=-1.0)
+        gsf, gls, gz = dops.dtrace(M_inner, self.zu, self.ls, self.sf)
+        dls, dsf = dops.hyper(self.ls, self.sf, dell, gls, dsf2, sumdv, gsf)
+        lincomb(dzu, 1.0, dzu, 1.0, gz)
         return {'sf': dsf, 'ls': dls, 'zu': dzu, 'eta1_R': deta1, 'eta2': deta2}
 
     def backprop_grads_lvm_mm(self, m, v, dm, dv, psi1, psi2, mx, vx, alpha=1.0, dm_all=None,
@@ -357,9 +435,9 @@ class SGP_Layer(object):
         dtheta2 = mm(self.Suhat, _col(dmcav), alpha=beta).reshape(D, M)
         dKi = colsum(dvcavinv, D, M * M).reshape(M, M)
         mm(dAhat, self.muhat, dKi, tA=True, beta=1.0)
-        KiS = mm(Ki, self.Splusmmhat)
-        mm(KiS.reshape(D * M, M), dBhat.reshape(D * M, M), dKi, tA=True, alpha=2.0, beta=1.0)
-        colsum(dBhat, D, M * M, out=dKi.reshape(-1), alpha=-1.0, beta=1.0)
+        # 2 sum_d KiS_d^T dBhat_d - sum_d dBhat_d as one product with (2 KiS - I)
+        KiS2 = mm(Ki, self.Splusmmhat, alpha=2.0, E=_eye(M, Ki), gamma=-1.0)
+        mm(KiS2.reshape(D * M, M), dBhat.reshape(D * M, M), dKi, tA=True, beta=1.0)
         Minner = colsum(self.Splusmm, D, M * M, alpha=scale_post).reshape(M, M)
         colsum(self.Splusmmhat, D, M * M, out=Minner.reshape(-1), alpha=scale_cav, beta=1.0)
         lincomb(Minner, 1.0, Minner, -2.0, dKi)
@@ -367,8 +445,7 @@ class SGP_Layer(object):
                           -0.5 * scale_cav * beta, self.Splusmmhat)
         lincomb(dtheta2, 1.0, dtheta2, scale_post, self.mu, scale_cav * beta, self.muhat)
         deta1 = dops.pack_dR(self.theta_1_R, dtheta1)
-        M_all = lincomb(torch.empty_like(Ki), 0.5 * D, Ki)
-        _sandwich(Ki, Minner, Ki, alpha=0.5, out=M_all)
+        M_all = _sandwich(Ki, Minner, Ki, alpha=0.5, E=Ki, gamma=0.5 * D)
         gsf, gls, gz = dops.dtrace(M_all, self.zu, self.ls, self.sf)
         dls, dsf = dops.hyper(self.ls, self.sf, dell, gls, dsf2, sumdv, gsf)
         dzu = lincomb(torch.empty_like(dzu), 1.0, dzu, 1.0, gz)

@@ -39,6 +39,16 @@ int gpx_gemm_batched(int transA, int transB, int m, int n, int k, double alpha,
                      const double *A, int lda, long strideA, const double *B, int ldb,
                      long strideB, double beta, double *C, int ldc, long strideC, int batch,
                      int splits, double *work, int tile, void *stream);
+/* gpx_gemm_batched with a bias term in the same pass:
+ * C[b] = alpha op(A[b]) op(B[b]) + beta C[b] + gamma E[b] (E row-major with leading
+ * dimension ldE; strideE = 0 broadcasts one E over the batch; E = NULL drops the term).
+ * Carries the A K^-1 B - K^-1 and A + B C forms of geepee/aep_models.py:62-95, 453-493
+ * in one launch instead of a product and a separate axpy. */
+int gpx_gemm_batched_ex(int transA, int transB, int m, int n, int k, double alpha,
+                        const double *A, int lda, long strideA, const double *B, int ldb,
+                        long strideB, double beta, double *C, int ldc, long strideC,
+                        double gamma, const double *E, int ldE, long strideE, int batch,
+                        int splits, double *work, int tile, void *stream);
 
 /* Batched SPD inverse and log-determinant, one workgroup per matrix (M <= 256):
  * out[b] = inv(A[b]) (out may alias A or be NULL), logdet[b] = log det A[b],
@@ -154,6 +164,14 @@ int gpx_hyper_grads(int Q, const double *ls, const double *sf, const double *del
 int gpx_phi(int D, const double *ldK, const double *ldP, double sgnP, const double *dotP,
             const double *ldC, double sgnC, const double *dotC, double sp, double sc, double *out,
             void *stream);
+/* gpx_phi for the natural parameterisation in one launch: ldP = log det Sigma^-1,
+ * ldC = log det Sigma-hat^-1 (D each), mu, muhat, theta2 (D x M); the quadratic terms
+ * mu^T Sigma^-1 mu and mu-hat^T Sigma-hat^-1 mu-hat are mu . theta2 and
+ * beta mu-hat . theta2 (Sigma^-1 mu = theta2, Sigma-hat^-1 mu-hat = beta theta2), in
+ * place of the np.linalg.solve calls of geepee/aep_models.py:72-95. */
+int gpx_phi_nat(int D, int M, const double *ldK, const double *ldP, const double *ldC, const double *mu,
+                const double *muhat, const double *theta2, double beta, double sp, double sc, double *out,
+                void *stream);
 
 /* ---- likelihoods ----------------------------------------------------------- */
 

@@ -82,6 +82,73 @@ def test_gemm_broadcast_operand():
     assert nrel(host(C), np.einsum('ab,dbc->dac', K, X)) < 1e-14
 
 
+@pytest.mark.parametrize('m,n,k,batch,splits,bcast', [
+    (100, 100, 100, 5, 1, True), (100, 100, 100, 5, 1, False), (100, 1, 100, 5, 1, False),
+    (64, 64, 300, 2, 1, False), (100, 100, 1000, 1, 4, True), (33, 17, 9, 3, 1, False)])
+def test_gemm_bias_epilogue(m, n, k, batch, splits, bcast):
+    """C = alpha A B + beta C + gamma E on every kernel path (full-K, K-looped, split-K),
+    with E per batch member or broadcast."""
+    L = _lib()
+    say('gemm_ex', m, n, k, batch, splits, bcast)
+    rs = np.random.RandomState(m * 7 + k)
+    A, B = rs.randn(batch, m, k), rs.randn(batch, k, n)
+    C0 = rs.randn(batch, m, n)
+    E = rs.randn(m, n) if bcast else rs.randn(batch, m, n)
+    ref = 0.7 * np.einsum('bij,bjk->bik', A, B) - 0.3 * C0 + 1.9 * E
+    C = dev(C0)
+    L.mm(dev(A), dev(B), C, alpha=0.7, beta=-0.3, splits=splits, E=dev(E), gamma=1.9)
+    assert nrel(host(C), ref) < 1e-14
+    C = dev(np.full((batch, m, n), np.nan))   # beta = 0 never reads C
+    L.mm(dev(A), dev(B), C, alpha=0.7, splits=splits, E=dev(E), gamma=-1.0)
+    assert nrel(host(C), 0.7 * np.einsum('bij,bjk->bik', A, B) - E) < 1e-14
+
+
+@pytest.mark.parametrize('D,M', [(2, 100), (1, 200), (3, 64)])
+def test_pack_dR_sizes(D, M):
+    """pack_dR over several k slices, against the oracle."""
+    L = _lib()
+    say('pack_dR', D, M)
+    rs = np.random.RandomState(M)
+    eta = rs.randn(D, M * (M + 1) // 2) * 0.1
+    Rref = O.eta_to_R(eta, M)
+    G = rs.randn(D, M, M)
+    out = dev(np.zeros((D, M * (M + 1) // 2)))
+    L.call('gpx_pack_dR', D, M, L.P(dev(Rref)), L.P(dev(G)), L.P(out), L.S())
+    assert nrel(host(out), O.pack_dR(Rref, G)) < 1e-14
+
+
+def test_phi_nat_matches_solve_form():
+    """gpx_phi_nat (quadratic terms from theta2) against the log-partitions written the
+    way aep_models.py:62-114 computes them: slogdet and solve with Sigma, Sigma-hat."""
+    L = _lib()
+    say('phi_nat')
+    rs = np.random.RandomState(3)
+    D, M, N, alpha = 3, 12, 50, 0.5
+    beta = (N - alpha) / N
+    G = rs.randn(M, M)
+    Kuu = G.dot(G.T) / M + np.eye(M)
+    Ki = np.linalg.inv(Kuu)
+    th1 = np.stack([(lambda R: R.T.dot(R))(np.triu(rs.randn(M, M)) * 0.3) for _ in range(D)])
+    th2 = rs.randn(D, M)
+    Su = np.linalg.inv(Ki + th1)
+    Suhat = np.linalg.inv(Ki + beta * th1)
+    mu = np.einsum('dab,db->da', Su, th2)
+    muhat = beta * np.einsum('dab,db->da', Suhat, th2)
+    sp, sc = N / alpha - 1.0, -N / alpha
+    want = D * 0.5 * np.linalg.slogdet(Kuu)[1]
+    want += sp * (0.5 * np.sum(np.linalg.slogdet(Su)[1]) +
+                  0.5 * np.sum(mu * np.linalg.solve(Su, mu[..., None])[..., 0]))
+    want += sc * (0.5 * np.sum(np.linalg.slogdet(Suhat)[1]) +
+                  0.5 * np.sum(muhat * np.linalg.solve(Suhat, muhat[..., None])[..., 0]))
+    ldK = dev([np.linalg.slogdet(Kuu)[1]])
+    ldP = dev([np.linalg.slogdet(Ki + t)[1] for t in th1])
+    ldC = dev([np.linalg.slogdet(Ki + beta * t)[1] for t in th1])
+    out = dev([0.0])
+    L.call('gpx_phi_nat', D, M, L.P(ldK), L.P(ldP), L.P(ldC), L.P(dev(mu)), L.P(dev(muhat)),
+           L.P(dev(th2)), beta, sp, sc, L.P(out), L.S())
+    assert abs(host(out)[0] - want) < 1e-10 * max(abs(want), 1.0)
+
+
 @pytest.mark.parametrize('M,batch', [(5, 3), (33, 2), (100, 4), (128, 2), (200, 2), (256, 1)])
 def test_spd_inverse_logdet(M, batch):
     L = _lib()
