@@ -35,6 +35,25 @@ __device__ __forceinline__ double gpx_block_sum(double v, double *red) {
   return t;
 }
 
+// Lane l's value of a double, broadcast to the whole wave (l uniform).
+__device__ __forceinline__ double gpx_readlane_d(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// 1/d by the hardware reciprocal estimate and two Newton-Raphson steps
+// r <- r + r (1 - d r): within an ulp of the quotient, in five instructions instead of
+// the correctly rounded division's sequence (for latency-bound pivot chains).
+__device__ __forceinline__ double gpx_recip(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+}
+
 typedef double gpx_d4 __attribute__((ext_vector_type(4)));
 
 // D(16x16) += A(16x4) * B(4x16) in FP64 on the matrix cores.  Operand layout:
