@@ -478,32 +478,55 @@ __global__ __launch_bounds__(256) void fused_bwd_pairs_kernel(
 
 // Sum the per-range partials and mirror the off-diagonal tiles:
 // dBhat (D x M x M) and LW ((M*M) x (Q+1)) in the layouts of the materialising path.
-__global__ void fused_pairs_finish(int M, int Q, int D, int nsplit, const double *__restrict__ dBpart,
-                                   const double *__restrict__ LWpart, double *__restrict__ dBhat,
-                                   double *__restrict__ LW) {
+// Sum of the per-pair pass's point-range partials.  One workgroup per stored 16 x 16
+// tile pair (A <= B, the half the pair kernel writes) and output slice (k < D: dBhat[k],
+// k == D: LW); rows are read coalesced and the mirrored tile (B, A) is written through an
+// LDS transpose.
+__global__ __launch_bounds__(256) void fused_pairs_finish(int M, int Q, int D, int nsplit,
+                                                          const double *__restrict__ dBpart,
+                                                          const double *__restrict__ LWpart,
+                                                          double *__restrict__ dBhat, double *__restrict__ LW) {
+  __shared__ double tile[16][17];
+  const int T = (M + 15) / 16;
+  int t = blockIdx.x, A = 0;
+  while (t >= T - A) {
+    t -= T - A;
+    ++A;
+  }
+  const int B = A + t, k = blockIdx.y, tid = threadIdx.x, i = tid >> 4, j = tid & 15;
   const long MM = (long)M * M;
-  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  const long W = (long)D + Q + 1;
-  if (e >= MM * W) return;
-  const long ab = e / W;
-  const int k = (int)(e % W);
-  int a = (int)(ab / M), b = (int)(ab % M);
-  // the tiles cover blocks A <= B; read the stored half
-  const int sa = (a / 16 <= b / 16) ? a : b, sb = (a / 16 <= b / 16) ? b : a;
-  double s = 0.0;
   if (k < D) {
-    for (int p = 0; p < nsplit; ++p) s += dBpart[(((long)p * D + k) * M + sa) * M + sb];
-    dBhat[(long)k * MM + (long)a * M + b] = s;
+    const int a = A * 16 + i, b = B * 16 + j;
+    double s = 0.0;
+    if (a < M && b < M) {
+      for (int p = 0; p < nsplit; ++p) s += dBpart[(((long)p * D + k) * M + a) * M + b];
+      dBhat[k * MM + (long)a * M + b] = s;
+    }
+    if (A != B) {
+      tile[i][j] = s;
+      __syncthreads();
+      const int a2 = B * 16 + i, b2 = A * 16 + j;
+      if (a2 < M && b2 < M) dBhat[k * MM + (long)a2 * M + b2] = tile[j][i];
+    }
   } else {
-    const int q = k - D;
-    for (int p = 0; p < nsplit; ++p) s += LWpart[(((long)p * M + sa) * M + sb) * (Q + 1) + q];
-    LW[((long)a * M + b) * (Q + 1) + q] = s;
+    const int Q1 = Q + 1;
+    for (int e = tid; e < 256 * Q1; e += 256) {
+      const int ii = e / (16 * Q1), r = e % (16 * Q1), jj = r / Q1, q = r % Q1;
+      const int a = A * 16 + ii, b = B * 16 + jj;
+      if (a >= M || b >= M) continue;
+      double s = 0.0;
+      for (int p = 0; p < nsplit; ++p) s += LWpart[(((long)p * M + a) * M + b) * Q1 + q];
+      LW[((long)a * M + b) * Q1 + q] = s;
+      if (A != B) LW[((long)b * M + a) * Q1 + q] = s;
+    }
   }
 }
 
 
 // Per-point finish of the fused adjoint: T = sum_s Tpart, V = sum_s Vpart, Lp = dpsi1 psi1,
 // then the same per-point formulas (and outputs) as the materialising path's row pass.
+// One wave per point: its lanes stride over the M columns and the sums are wave
+// reductions, so no workgroup barrier is needed.
 constexpr int FRED = 4 * FQ_MAX + 2;
 __global__ __launch_bounds__(256) void fused_rows_finish(
     int N, int M, int Q, int S, const double *__restrict__ mx, const double *__restrict__ vx,
@@ -512,52 +535,53 @@ __global__ __launch_bounds__(256) void fused_rows_finish(
     double *__restrict__ dmx, double *__restrict__ dvx, double *__restrict__ Tout,
     double *__restrict__ Lpout, double *__restrict__ Hout, double *__restrict__ s0out,
     double *__restrict__ Dn, double *__restrict__ Pn, double *__restrict__ En) {
-  __shared__ double red[4][FRED];
-  __shared__ double tot[FRED];
-  const int n = blockIdx.x, b = threadIdx.x, lane = b & 63, w = b >> 6;
+  const int lane = threadIdx.x & 63, n = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (n >= N) return;
   double mu[FQ_MAX];
 #pragma unroll
   for (int q = 0; q < FQ_MAX; ++q) mu[q] = (q < Q) ? mx[(long)n * Q + q] : 0.0;
-  double T = 0.0, Lp = 0.0;
-  if (b < M) {
-    for (int s = 0; s < S; ++s) T += Tpart[((long)s * N + n) * M + b];
-    Lp = dpsi1[(long)n * M + b] * psi1[(long)n * M + b];
-    Tout[(long)n * M + b] = T;
-    Lpout[(long)n * M + b] = Lp;
-  }
   // [0] S0, [1] sum Lp, [2+q] S1, [2+Q+q] sum T z^2, [2+2Q+q] B1, [2+3Q+q] C1
   double c[FRED];
 #pragma unroll
   for (int i = 0; i < FRED; ++i) c[i] = 0.0;
-  c[0] = T;
-  c[1] = Lp;
-  if (b < M) {
+  for (int m = lane; m < M; m += 64) {
+    double T = 0.0;
+    for (int s = 0; s < S; ++s) T += Tpart[((long)s * N + n) * M + m];
+    const double Lp = dpsi1[(long)n * M + m] * psi1[(long)n * M + m];
+    Tout[(long)n * M + m] = T;
+    Lpout[(long)n * M + m] = Lp;
+    c[0] += T;
+    c[1] += Lp;
 #pragma unroll
     for (int q = 0; q < FQ_MAX; ++q)
       if (q < Q) {
-        const double zq = z[b * Q + q], zm = zq - mu[q];
-        c[2 + q] = T * zq;
-        c[2 + FQ_MAX + q] = T * zq * zq;
-        c[2 + 2 * FQ_MAX + q] = Lp * zm;
-        c[2 + 3 * FQ_MAX + q] = Lp * zm * zm;
+        const double zq = z[m * Q + q], zm = zq - mu[q];
+        c[2 + q] += T * zq;
+        c[2 + FQ_MAX + q] += T * zq * zq;
+        c[2 + 2 * FQ_MAX + q] += Lp * zm;
+        c[2 + 3 * FQ_MAX + q] += Lp * zm * zm;
       }
   }
 #pragma unroll
   for (int i = 0; i < FRED; ++i) {
     if (i >= 2 && (i - 2) % FQ_MAX >= Q) continue;
-    const double s = gpx_wave_sum(c[i]);
-    if (lane == 0) red[w][i] = s;
+    c[i] = gpx_wave_sum(c[i]);
   }
-  __syncthreads();
-  for (int i = b; i < FRED; i += 256) tot[i] = red[0][i] + red[1][i] + red[2][i] + red[3][i];
-  __syncthreads();
-  if (b < Q) {
-    const int q = b;
+  double out_S1 = 0.0, out_TZ2 = 0.0, out_B1 = 0.0, out_C1 = 0.0;
+#pragma unroll
+  for (int q = 0; q < FQ_MAX; ++q)
+    if (q == lane) {
+      out_S1 = c[2 + q];
+      out_TZ2 = c[2 + FQ_MAX + q];
+      out_B1 = c[2 + 2 * FQ_MAX + q];
+      out_C1 = c[2 + 3 * FQ_MAX + q];
+    }
+  if (lane < Q) {
+    const int q = lane;
     double V = 0.0;
     for (int s = 0; s < S; ++s) V += Vpart[((long)s * N + n) * Q + q];
-    const double S0 = tot[0], A1 = tot[1];
-    const double S1 = tot[2 + q], TZ2 = tot[2 + FQ_MAX + q];
-    const double B1 = tot[2 + 2 * FQ_MAX + q], C1 = tot[2 + 3 * FQ_MAX + q];
+    const double S0 = c[0], A1 = c[1];
+    const double S1 = out_S1, TZ2 = out_TZ2, B1 = out_B1, C1 = out_C1;
     const double l = exp(2.0 * ls[q]);
     const double m = mx[(long)n * Q + q], vv = vx[(long)n * Q + q];
     const double ell = exp(ls[q]);
@@ -574,10 +598,10 @@ __global__ __launch_bounds__(256) void fused_rows_finish(
     En[(long)n * 2 * Q + q] = e;
     En[(long)n * 2 * Q + Q + q] = e * m;
   }
-  if (b == 0) {
+  if (lane == 0) {
     Dn[(long)n * (Q + 1)] = 1.0;
-    s0out[2 * n] = tot[0];
-    s0out[2 * n + 1] = tot[1];
+    s0out[2 * n] = c[0];
+    s0out[2 * n + 1] = c[1];
   }
 }
 
@@ -592,21 +616,37 @@ __global__ __launch_bounds__(256) void fused_fwd_finish(int N, int M, int Q, int
                                                         double *__restrict__ psi1, double *__restrict__ mout,
                                                         double *__restrict__ vout) {
   __shared__ double p1[16][FM_MAX + 1];
+  __shared__ double den[16][FQ_MAX], mus[16][FQ_MAX], lcs[16];
   const int n0 = blockIdx.x * 16, tid = threadIdx.x;
   const double sf2 = exp(2.0 * sf[0]);
+  // per point: 2 (v + l) and mx per dimension, and sum_q log(l / (l + v)) / 2 (the
+  // logs do not depend on the inducing point)
+  if (tid < 16 * Q) {
+    const int nn = tid / Q, q = tid % Q, n = n0 + nn;
+    const double l = exp(2.0 * ls[q]), v = n < N ? vx[(long)n * Q + q] : 0.0;
+    den[nn][q] = 2.0 * (v + l);
+    mus[nn][q] = n < N ? mx[(long)n * Q + q] : 0.0;
+  }
+  if (tid < 16) {
+    const int n = n0 + tid;
+    double lc = 0.0;
+    for (int q = 0; q < Q && n < N; ++q) {
+      const double l = exp(2.0 * ls[q]), v = vx[(long)n * Q + q];
+      lc += 0.5 * log(l / (l + v));
+    }
+    lcs[tid] = lc;
+  }
+  __syncthreads();
   for (int e = tid; e < 16 * M; e += 256) {
     const int nn = e / M, m = e % M, n = n0 + nn;
     double val = 0.0;
     if (n < N) {
-      double acc = 0.0, lc = 0.0;
+      double acc = 0.0;
       for (int q = 0; q < Q; ++q) {
-        const double l = exp(2.0 * ls[q]);
-        const double v = vx[(long)n * Q + q];
-        const double dlt = mx[(long)n * Q + q] - z[m * Q + q];
-        acc += -dlt * dlt / (2.0 * (v + l));
-        lc += 0.5 * log(l / (l + v));
+        const double dlt = mus[nn][q] - z[m * Q + q];
+        acc += -dlt * dlt / den[nn][q];
       }
-      val = sf2 * exp(acc + lc);
+      val = sf2 * exp(acc + lcs[nn]);
       psi1[(long)n * M + m] = val;
     }
     p1[nn][m] = val;
@@ -699,12 +739,9 @@ int gpx_fused_mm_bwd(int N, int M, int Q, int D, const double *mx, const double 
     hipLaunchKernelGGL((fused_bwd_pairs_kernel<2, 8>), gp, dim3(256), 0, st, N, M, Q, D, z, ls, sf, Bp, dv,
                        (const double *)pf, nsplit, nchunk, dBpart, LWpart);
   GPX_CHECK_LAUNCH();
-  {
-    const long n = MM * (D + Q + 1);
-    hipLaunchKernelGGL(fused_pairs_finish, dim3(gpx_cdiv(n, 256)), dim3(256), 0, st, M, Q, D, nsplit,
-                       (const double *)dBpart, (const double *)LWpart, dBhat, LW);
-    GPX_CHECK_LAUNCH();
-  }
+  hipLaunchKernelGGL(fused_pairs_finish, dim3(ntiles, D + 1), dim3(256), 0, st, M, Q, D, nsplit,
+                     (const double *)dBpart, (const double *)LWpart, dBhat, LW);
+  GPX_CHECK_LAUNCH();
   // per-point pass over tile splits
   dim3 gq(gpx_cdiv(N, 16), splits);
   if (D <= 16)
@@ -718,7 +755,7 @@ int gpx_fused_mm_bwd(int N, int M, int Q, int D, const double *mx, const double 
   int rc = gpx_gemm_batched(0, 0, N, M, D, 1.0, dm_all, D, 0, Ahat, M, 0, 0.0, dpsi1, M, 0, 1, 1, nullptr, 0,
                             stream);
   if (rc) return rc;
-  hipLaunchKernelGGL(fused_rows_finish, dim3(N), dim3(256), 0, st, N, M, Q, splits, mx, vx, z, ls,
+  hipLaunchKernelGGL(fused_rows_finish, dim3(gpx_cdiv(N, 4)), dim3(256), 0, st, N, M, Q, splits, mx, vx, z, ls,
                      (const double *)Tpart, (const double *)Vpart, (const double *)dpsi1, psi1, dmx, dvx, Tn, Lp,
                      H, s0, Dn, Pn, En);
   GPX_CHECK_LAUNCH();
