@@ -476,12 +476,11 @@ __global__ __launch_bounds__(256) void fused_bwd_pairs_kernel(
   }
 }
 
-// Sum the per-range partials and mirror the off-diagonal tiles:
-// dBhat (D x M x M) and LW ((M*M) x (Q+1)) in the layouts of the materialising path.
 // Sum of the per-pair pass's point-range partials.  One workgroup per stored 16 x 16
 // tile pair (A <= B, the half the pair kernel writes) and output slice (k < D: dBhat[k],
 // k == D: LW); rows are read coalesced and the mirrored tile (B, A) is written through an
-// LDS transpose.
+// LDS transpose.  dBhat (D x M x M) and LW ((M*M) x (Q+1)) come out in the layouts of
+// the materialising path.
 __global__ __launch_bounds__(256) void fused_pairs_finish(int M, int Q, int D, int nsplit,
                                                           const double *__restrict__ dBpart,
                                                           const double *__restrict__ LWpart,
