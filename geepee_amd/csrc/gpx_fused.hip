@@ -498,6 +498,7 @@ __global__ __launch_bounds__(256) void fused_pairs_finish(int M, int Q, int D, i
     const int a = A * 16 + i, b = B * 16 + j;
     double s = 0.0;
     if (a < M && b < M) {
+#pragma unroll 8
       for (int p = 0; p < nsplit; ++p) s += dBpart[(((long)p * D + k) * M + a) * M + b];
       dBhat[k * MM + (long)a * M + b] = s;
     }
@@ -514,6 +515,7 @@ __global__ __launch_bounds__(256) void fused_pairs_finish(int M, int Q, int D, i
       const int a = A * 16 + ii, b = B * 16 + jj;
       if (a >= M || b >= M) continue;
       double s = 0.0;
+#pragma unroll 8
       for (int p = 0; p < nsplit; ++p) s += LWpart[(((long)p * M + a) * M + b) * Q1 + q];
       LW[((long)a * M + b) * Q1 + q] = s;
       if (A != B) LW[((long)b * M + a) * Q1 + q] = s;

@@ -231,55 +231,44 @@ __global__ void eta_to_R_kernel(int D, int M, const double *__restrict__ eta, do
 }
 
 // out[d, triu(i,j)] = (sum_{k>=i} R_ik (G_kj + G_jk)) * (i == j ? R_ii : 1)
-// One workgroup per (d, 16 rows i, 64 columns j); k advances in slices of 64 (at most
-// two for M <= 128).  Each slice issues all of its loads (R rows, G rows and the G^T
-// half read along rows of G, so every load is coalesced) before one barrier, which keeps
-// the kernel at one global round trip per slice.
+// One wave per upper 16 x 16 output tile (I, J >= I) of one d, four per workgroup: the
+// product R (G + G^T) over the k-blocks K >= I (R is upper triangular) as FP64 MFMAs fed
+// straight from global memory (lane l: A = R[I*16 + l%16][k], B = G[k][J*16 + l%16] +
+// G[J*16 + l%16][k] for k = K*16 + 4s + l/16).  No LDS, no barriers.
 __global__ __launch_bounds__(256) void pack_dR_kernel(int D, int M, const double *__restrict__ R,
                                                       const double *__restrict__ G, double *__restrict__ out) {
-  constexpr int KC = 64;
-  __shared__ double Rs[16][KC + 1];
-  __shared__ double Ss[KC][65];
-  __shared__ double St[64][KC + 1];
-  const int d = blockIdx.z, i0 = blockIdx.y * 16, j0 = blockIdx.x * 64;
-  if (j0 + 63 < i0) return;  // tile entirely below the diagonal
-  const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15;
-  const double *Rd = R + (long)d * M * M, *Gd = G + (long)d * M * M;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int k0 = i0; k0 < M; k0 += KC) {
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int e = tid + 256 * u, r = e >> 6, kk = e & 63, i = i0 + r, k = k0 + kk;
-      Rs[r][kk] = (i < M && k < M && k >= i) ? Rd[(long)i * M + k] : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const int e = tid + 256 * u, hi = e >> 6, lo = e & 63;
-      // Ss[kk][jj] = G[k0 + kk][j0 + jj];  St[jj][kk] = G[j0 + jj][k0 + kk]
-      const int k = k0 + hi, j = j0 + lo;
-      Ss[hi][lo] = (k < M && j < M) ? Gd[(long)k * M + j] : 0.0;
-      const int jt = j0 + hi, kt = k0 + lo;
-      St[hi][lo] = (jt < M && kt < M) ? Gd[(long)jt * M + kt] : 0.0;
-    }
-    __syncthreads();
-#pragma unroll 8
-    for (int kk = 0; kk < KC; ++kk) {
-      const double r = Rs[ti][kk];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) acc[c] = fma(r, Ss[kk][tj + 16 * c] + St[tj + 16 * c][kk], acc[c]);
-    }
-    __syncthreads();
+  const int lane = threadIdx.x & 63, lr = lane >> 4, lc = lane & 15;
+  const int MT = (M + 15) / 16, T = MT * (MT + 1) / 2;
+  int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= T) return;
+  int I = 0;
+  while (t >= MT - I) {
+    t -= MT - I;
+    ++I;
   }
-  const int i = i0 + ti;
-  if (i >= M) return;
-  const long K = (long)M * (M + 1) / 2;
-  const long rowoff = (long)i * M - (long)i * (i - 1) / 2;
+  const int J = I + t, d = blockIdx.y;
+  const double *Rd = R + (long)d * M * M, *Gd = G + (long)d * M * M;
+  const int i = I * 16 + lc, j = J * 16 + lc;
+  gpx_d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 2
+  for (int K = I; K < MT; ++K) {
+    double av[4], bv[4];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int j = j0 + tj + 16 * c;
-    if (j < i || j >= M) continue;
-    const double v = (i == j) ? acc[c] * Rd[(long)i * M + i] : acc[c];
-    out[(long)d * K + rowoff + (j - i)] = v;
+    for (int s = 0; s < 4; ++s) {
+      const int k = K * 16 + 4 * s + lr;
+      av[s] = (i < M && k < M) ? Rd[(long)i * M + k] : 0.0;
+      bv[s] = (j < M && k < M) ? Gd[(long)k * M + j] + Gd[(long)j * M + k] : 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) acc = gpx_mfma16(av[s], bv[s], acc);
+  }
+  const long Kp = (long)M * (M + 1) / 2;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int ii = I * 16 + lr + 4 * r, jj = J * 16 + lc;
+    if (ii >= M || jj >= M || jj < ii) continue;
+    const long rowoff = (long)ii * M - (long)ii * (ii - 1) / 2;
+    out[(long)d * Kp + rowoff + (jj - ii)] = (ii == jj) ? acc[r] * Rd[(long)ii * M + ii] : acc[r];
   }
 }
 
@@ -783,8 +772,9 @@ int gpx_pack_dR(int D, int M, const double *R, const double *G, double *out, voi
   long n = (long)D * M * (M + 1) / 2;
   if (n == 0) return GPX_OK;
   if (D > 65535) return GPX_EINVAL;
-  hipLaunchKernelGGL(pack_dR_kernel, dim3(gpx_cdiv(M, 64), gpx_cdiv(M, 16), D), dim3(256), 0, (hipStream_t)stream, D,
-                     M, R, G, out);
+  const int MT = (M + 15) / 16;
+  hipLaunchKernelGGL(pack_dR_kernel, dim3(gpx_cdiv(MT * (MT + 1) / 2, 4), D), dim3(256), 0, (hipStream_t)stream,
+                     D, M, R, G, out);
   GPX_CHECK_LAUNCH();
   return GPX_OK;
 }
