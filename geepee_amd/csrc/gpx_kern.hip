@@ -24,10 +24,11 @@ __global__ void se_kernel(int N, int M, int Q, const double *__restrict__ x, int
   K[e] = k;
 }
 
-// kfucompDer (kernels.py:381-399), pass 1 over chunks of 64 rows:
-// partial[blk][m] = [colsum_m, sum_n L x_nq (Q), sum_n L (z_mq - x_nq)^2 (Q)],
+// kfucompDer (kernels.py:381-399) with L = dK * K, pass 1 over chunks of KFU_NB rows:
+// partial[blk][m] = [colsum_m, sum_n L x_nq (Q), sum_n L (z_mq - x_nq)^2 (Q)], one
+// thread per (m, q) entry, the chunk's L staged in LDS;
 // dx_nq = sum_m L_nm (z_mq - x_nq) / l_q.
-constexpr int KFU_NB = 64;
+constexpr int KFU_NB = 16;
 __global__ __launch_bounds__(256) void kfu_bwd_partial(int N, int M, int Q, const double *__restrict__ dK,
                                                        const double *__restrict__ K,
                                                        const double *__restrict__ x,
@@ -36,77 +37,82 @@ __global__ __launch_bounds__(256) void kfu_bwd_partial(int N, int M, int Q, cons
                                                        double *__restrict__ part,
                                                        double *__restrict__ dx) {
   __shared__ double xs[KFU_NB][17];
+  __shared__ double Ls[KFU_NB][257];
   const int n0 = blockIdx.x * KFU_NB, nb = min(KFU_NB, N - n0), tid = threadIdx.x;
   for (int e = tid; e < nb * Q; e += 256) xs[e / Q][e % Q] = x[(long)(n0 + e / Q) * Q + e % Q];
+  for (int e = tid; e < nb * M; e += 256) {
+    const long o = (long)(n0 + e / M) * M + e % M;
+    Ls[e / M][e % M] = dK[o] * K[o];
+  }
   __syncthreads();
   const int W = 2 * Q + 1;
-  for (int m = tid; m < M; m += 256) {
-    double cs = 0.0, lx[16], ld2[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) lx[q] = ld2[q] = 0.0;
-    for (int i = 0; i < nb; ++i) {
-      const long o = (long)(n0 + i) * M + m;
-      const double L = dK[o] * K[o];
-      cs += L;
-#pragma unroll
-      for (int q = 0; q < 16; ++q)
-        if (q < Q) {
-          const double xv = xs[i][q], d = z[m * Q + q] - xv;
-          lx[q] += L * xv;
-          ld2[q] += L * d * d;
-        }
-    }
+  for (int e = tid; e < M * (Q + 1); e += 256) {
+    const int m = e / (Q + 1), q1 = e % (Q + 1);
     double *P = part + ((long)blockIdx.x * M + m) * W;
-    P[0] = cs;
-#pragma unroll
-    for (int q = 0; q < 16; ++q)
-      if (q < Q) {
-        P[1 + q] = lx[q];
-        P[1 + Q + q] = ld2[q];
+    if (q1 == 0) {
+      double cs = 0.0;
+      for (int i = 0; i < nb; ++i) cs += Ls[i][m];
+      P[0] = cs;
+    } else {
+      const int q = q1 - 1;
+      const double zq = z[m * Q + q];
+      double lx = 0.0, ld2 = 0.0;
+      for (int i = 0; i < nb; ++i) {
+        const double L = Ls[i][m], xv = xs[i][q], d = zq - xv;
+        lx += L * xv;
+        ld2 += L * d * d;
       }
+      P[1 + q] = lx;
+      P[1 + Q + q] = ld2;
+    }
   }
   if (!dx) return;
   for (int e = tid; e < nb * Q; e += 256) {
     const int i = e / Q, q = e % Q;
-    const long row = (long)(n0 + i) * M;
     double s = 0.0;
-    for (int m = 0; m < M; ++m) s += dK[row + m] * K[row + m] * (z[m * Q + q] - xs[i][q]);
+    for (int m = 0; m < M; ++m) s += Ls[i][m] * (z[m * Q + q] - xs[i][q]);
     dx[(long)(n0 + i) * Q + q] = s / exp(2.0 * ls[q]);
   }
 }
 
-// pass 2: dZ_mq = -(z_mq colsum_m - Lx_mq)/l_q, dell_q = sum_m Ld2_mq / ell^3,
-// dsf2 = sum colsum / sf2.
+// pass 2, one thread per (m, q'): the chunk sums in chunk order; dZ_mq =
+// -(z_mq colsum_m - Lx_mq)/l_q; colsum_m and Ld2_mq go to `red` for pass 3.
 __global__ __launch_bounds__(256) void kfu_bwd_final(int nblk, int M, int Q, const double *__restrict__ part,
                                                      const double *__restrict__ z,
-                                                     const double *__restrict__ ls,
-                                                     const double *__restrict__ sf,
-                                                     double *__restrict__ dsf2, double *__restrict__ dell,
-                                                     double *__restrict__ dZ) {
-  __shared__ double red[16];
-  const int W = 2 * Q + 1, tid = threadIdx.x;
-  for (int e = tid; e < M * Q; e += 256) {
-    const int m = e / Q, q = e % Q;
-    double cs = 0.0, lx = 0.0;
-    for (int b = 0; b < nblk; ++b) {
-      const double *P = part + ((long)b * M + m) * W;
-      cs += P[0];
-      lx += P[1 + q];
-    }
-    dZ[e] = -(z[e] * cs - lx) / exp(2.0 * ls[q]);
+                                                     const double *__restrict__ ls, double *__restrict__ dZ,
+                                                     double *__restrict__ red) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= M * (Q + 1)) return;
+  const int m = e / (Q + 1), q1 = e % (Q + 1), W = 2 * Q + 1;
+  double cs = 0.0;
+  for (int b = 0; b < nblk; ++b) cs += part[((long)b * M + m) * W];
+  if (q1 == 0) {
+    red[e] = cs;
+    return;
   }
-  for (int q = 0; q < Q; ++q) {
-    double s = 0.0;
-    for (int m = tid; m < M; m += 256)
-      for (int b = 0; b < nblk; ++b) s += part[((long)b * M + m) * W + 1 + Q + q];
-    s = gpx_block_sum(s, red);
-    if (tid == 0) dell[q] = s / exp(3.0 * ls[q]);
+  const int q = q1 - 1;
+  double lx = 0.0, ld2 = 0.0;
+  for (int b = 0; b < nblk; ++b) {
+    const double *P = part + ((long)b * M + m) * W;
+    lx += P[1 + q];
+    ld2 += P[1 + Q + q];
   }
-  double c = 0.0;
-  for (int m = tid; m < M; m += 256)
-    for (int b = 0; b < nblk; ++b) c += part[((long)b * M + m) * W];
-  c = gpx_block_sum(c, red);
-  if (tid == 0) dsf2[0] = c / exp(2.0 * sf[0]);
+  dZ[m * Q + q] = -(z[m * Q + q] * cs - lx) / exp(2.0 * ls[q]);
+  red[e] = ld2;
+}
+
+// pass 3: dsf2 = sum_m colsum_m / sf2, dell_q = sum_m Ld2_mq / ell^3 (m in order).
+__global__ void kfu_bwd_scalars(int M, int Q, const double *__restrict__ red, const double *__restrict__ ls,
+                                const double *__restrict__ sf, double *__restrict__ dsf2,
+                                double *__restrict__ dell) {
+  const int q1 = threadIdx.x;
+  if (q1 > Q) return;
+  double s = 0.0;
+  for (int m = 0; m < M; ++m) s += red[m * (Q + 1) + q1];
+  if (q1 == 0)
+    dsf2[0] = s / exp(2.0 * sf[0]);
+  else
+    dell[q1 - 1] = s / exp(3.0 * ls[q1 - 1]);
 }
 
 // d_trace_MKzz_dhypers (kernels.py:447-475) with Kzz = SE(z, z) (no jitter):
@@ -679,21 +685,24 @@ int gpx_se_kernel(int N, int M, int Q, const double *x, int ldx, const double *z
 }
 
 long gpx_kfu_bwd_workspace(int N, int M, int Q) {
-  return (long)gpx_cdiv(N, KFU_NB) * M * (2 * Q + 1);
+  return (long)gpx_cdiv(N, KFU_NB) * M * (2 * Q + 1) + (long)M * (Q + 1);
 }
 
 // Adjoint of K(x, Z): dsf2 (w.r.t. sf2), dell (w.r.t. ell), dZ (M x Q), dx (N x Q, optional).
 int gpx_kfu_bwd(int N, int M, int Q, const double *dK, const double *K, const double *x,
                 const double *z, const double *ls, const double *sf, double *dsf2, double *dell,
                 double *dZ, double *dx, double *work, void *stream) {
-  if (N < 1 || M < 1 || Q < 1 || Q > 16) return GPX_EINVAL;
+  if (N < 1 || M < 1 || M > 256 || Q < 1 || Q > 16) return GPX_EINVAL;
   if (!dK || !K || !x || !z || !ls || !sf || !dsf2 || !dell || !dZ || !work) return GPX_EINVAL;
   hipStream_t st = (hipStream_t)stream;
   const unsigned nblk = gpx_cdiv(N, KFU_NB);
+  double *red = work + (long)nblk * M * (2 * Q + 1);
   hipLaunchKernelGGL(kfu_bwd_partial, dim3(nblk), dim3(256), 0, st, N, M, Q, dK, K, x, z, ls, work, dx);
   GPX_CHECK_LAUNCH();
-  hipLaunchKernelGGL(kfu_bwd_final, dim3(1), dim3(256), 0, st, (int)nblk, M, Q, work, z, ls, sf, dsf2,
-                     dell, dZ);
+  hipLaunchKernelGGL(kfu_bwd_final, dim3(gpx_cdiv((long)M * (Q + 1), 256)), dim3(256), 0, st, (int)nblk, M, Q,
+                     (const double *)work, z, ls, dZ, red);
+  GPX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(kfu_bwd_scalars, dim3(1), dim3(64), 0, st, M, Q, (const double *)red, ls, sf, dsf2, dell);
   GPX_CHECK_LAUNCH();
   return GPX_OK;
 }

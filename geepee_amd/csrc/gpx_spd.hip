@@ -26,31 +26,171 @@ namespace {
 // 16x16x4 MFMAs per tile.  The scalar pivots are the D of A = L D L^T: their logs sum to
 // log det A and the first non-positive one flags a matrix that is not positive definite.
 // The matrix is padded with the identity up to a multiple of 16.
+template <int MT>
+struct SpdLds {
+  static constexpr int MP = 16 * MT, RS = MP + 1;  // RS: LDS row stride (doubles)
+  // rowK: block row K (read by the panel only, so one buffer); colT: block column K,
+  // transposed (read by the tile updates while the next block's column is published,
+  // so two); rvK, Pi: the panel's results
+  double rowK[16 * RS];
+  double colT[2][16 * RS];
+  double rvK[16 * RS];
+  double Pi[16 * 17];
+  double piv[MP];
+  int failk[2];
+  double red[16];
+};
+
+// Phase 2 for panel wave pw of block K.  Lane c < 16 holds pivot column K*16 + c, lane
+// 16 + c identity column c (ending as column c of P^-1) and lanes 32.. hold CPL of the
+// other columns each (ending as P^-1 A[K, col]).  Every panel wave runs the same pivots
+// on the same pivot columns, so their copies agree bit for bit.
+template <int MT, int CPL = 1>
+__device__ __forceinline__ void spd_panel(SpdLds<MT> &sm, int K, int pw, int lane) {
+  constexpr int MP = SpdLds<MT>::MP, RS = SpdLds<MT>::RS;
+  int col[CPL];
+  bool live[CPL];
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    live[c] = c == 0 || lane >= 32;
+    col[c] = K * 16 + lane;
+    if (lane >= 32) {
+      const int m = 32 * CPL * pw + 32 * c + lane - 32;
+      live[c] = m < MP - 16;
+      col[c] = m < K * 16 ? m : m + 16;
+    }
+  }
+  const bool ident = lane >= 16 && lane < 32;
+  double v[CPL][16];
+#pragma unroll
+  for (int c = 0; c < CPL; ++c)
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+      v[c][i] = ident ? (c == 0 && i == lane - 16 ? 1.0 : 0.0) : (live[c] ? sm.rowK[i * RS + col[c]] : 0.0);
+  int bad = 0;
+#pragma unroll
+  for (int p = 0; p < 16; ++p) {
+    const double d = gpx_readlane_d(v[0][p], p);
+    if (!bad && !(d > 0.0 && d < INFINITY)) bad = p + 1;
+    const double rd = gpx_recip(d);
+    double colp[16];  // column p, read from the pivot lane
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+      if (i != p) colp[i] = gpx_readlane_d(v[0][i], p);
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      const double rowp = v[c][p] * rd;
+#pragma unroll
+      for (int i = 0; i < 16; ++i)
+        if (i != p) v[c][i] = fma(-colp[i], rowp, v[c][i]);
+      v[c][p] = rowp;
+    }
+    if (pw == 0 && lane == 0) sm.piv[K * 16 + p] = d;
+  }
+  if (ident) {
+    if (pw == 0) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) sm.Pi[i * 17 + lane - 16] = v[0][i];
+    }
+  } else if (lane >= 32) {
+#pragma unroll
+    for (int c = 0; c < CPL; ++c)
+      if (live[c]) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) sm.rvK[i * RS + col[c]] = v[c][i];
+      }
+  }
+  if (pw == 0 && lane == 0) sm.failk[K & 1] = bad ? K * 16 + bad : 0;
+}
+
+// Phase 1 for one tile: block row K and block column K (transposed) to LDS.
+template <int MT>
+__device__ __forceinline__ void spd_publish(SpdLds<MT> &sm, int K, int I, int J, const gpx_d4 &acc, int lr,
+                                            int lc) {
+  constexpr int RS = SpdLds<MT>::RS;
+  double *cT = sm.colT[K & 1];
+  if (I == K) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) sm.rowK[(lr + 4 * r) * RS + J * 16 + lc] = acc[r];
+  }
+  if (J == K) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) cT[lc * RS + I * 16 + lr + 4 * r] = acc[r];
+  }
+}
+
+// Phase 3 for one tile.
+template <int MT>
+__device__ __forceinline__ void spd_update(SpdLds<MT> &sm, int K, int I, int J, gpx_d4 &acc, int lr, int lc) {
+  constexpr int RS = SpdLds<MT>::RS;
+  const double *cT = sm.colT[K & 1];
+  if (I == K) {
+    if (J == K) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] = sm.Pi[(lr + 4 * r) * 17 + lc];
+    } else {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] = sm.rvK[(lr + 4 * r) * RS + J * 16 + lc];
+    }
+  } else if (J == K) {
+    gpx_d4 c = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks)
+      c = gpx_mfma16(-cT[(4 * ks + lr) * RS + I * 16 + lc], sm.Pi[(4 * ks + lr) * 17 + lc], c);
+    acc = c;
+  } else {
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks)
+      acc = gpx_mfma16(-cT[(4 * ks + lr) * RS + I * 16 + lc], sm.rvK[(4 * ks + lr) * RS + J * 16 + lc], acc);
+  }
+}
+
+template <int MT, int NW>
+__device__ __forceinline__ void spd_finish(SpdLds<MT> &sm, int M, int b, int fail, double *logdet, int *info) {
+  __syncthreads();  // pivots of the last block were written after the last barrier of the loop
+  double ld = 0.0;
+  for (int k = threadIdx.x; k < (fail ? 0 : M); k += NW * 64) ld += log(sm.piv[k]);
+  ld = gpx_block_sum(ld, sm.red);
+  if (threadIdx.x == 0) {
+    if (logdet) logdet[b] = fail ? NAN : ld;
+    if (info) info[b] = fail;
+  }
+}
+
+// Blocked Gauss-Jordan on the matrix cores, 16 x 16 pivot blocks.  Eliminating block
+// K with P = A[K,K] (rows of K: A[K,:], columns of K: c = A[:,K]):
+//   A[I,J] -= c_I P^-1 A[K,J]     (I, J != K)
+//   A[K,J] <- P^-1 A[K,J],  A[I,K] <- -c_I P^-1,  A[K,K] <- P^-1,
+// and after every block the matrix holds A^-1.  All MT x MT tiles are kept, in MFMA
+// accumulator layout, dealt round-robin to the tile waves (register r of lane l holds
+// tile entry [l/16 + 4r][l%16]): keeping only a symmetric half and mirroring it costs
+// two orders of magnitude in accuracy downstream on ill-conditioned input, because the
+// two halves of a computed inverse carry errors that cancel in later products.
+// Per block: the owners of block row and column K publish them to LDS; the panel waves
+// run Gauss-Jordan over the 16 scalar pivots on [P | I | A[K, other columns]], giving
+// P^-1 A[K,:] and P^-1 without ever multiplying by an explicit P^-1 (which loses an order
+// of magnitude on ill-conditioned blocks); the tile waves then update their tiles with
+// four 16x16x4 MFMAs per tile.  The scalar pivots are the D of A = L D L^T: their logs
+// sum to log det A and the first non-positive one flags a matrix that is not positive
+// definite.  The matrix is padded with the identity up to a multiple of 16.  Every wave
+// holds tiles and the first NPW waves also run the panel (giving the panel waves of its
+// own, two columns per lane, measured slower at every size on MI355X).
 template <int MT, int NW>
 __global__ __launch_bounds__(NW * 64) void spd_gj_kernel(int M, int batch, const double *__restrict__ A,
                                                          int lda, long sA, double *__restrict__ out, int ldo,
                                                          long sO, double *__restrict__ logdet,
                                                          int *__restrict__ info) {
   constexpr int MP = 16 * MT;
-  constexpr int RS = MP + 1;  // LDS row stride (doubles)
   constexpr int T = MT * MT;
+  constexpr int NPW = MP > 16 ? (MP - 16 + 31) / 32 : 1;  // panel waves
   constexpr int TPW = (T + NW - 1) / NW;
-  constexpr int NPW = MP > 16 ? (MP - 16 + 31) / 32 : 1;  // waves that eliminate the panel
   static_assert(NPW <= NW, "not enough waves for the panel");
-  // rowK: block row K (read by the panel only, so one buffer); colT: block column K,
-  // transposed (read by the tile updates while the next block's column is published,
-  // so two); rvK, Pi: the panel's results
-  __shared__ double rowK[16 * RS];
-  __shared__ double colT[2][16 * RS];
-  __shared__ double rvK[16 * RS];
-  __shared__ double Pi[16 * 17];
-  __shared__ double piv[MP];
-  __shared__ int failk[2];
-  __shared__ double red[16];
+  __shared__ SpdLds<MT> sm;
   const int b = blockIdx.x;
   if (b >= batch) return;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int lr = lane >> 4, lc = lane & 15;
+  int fail = 0;
   const double *Ab = A + b * sA;
   gpx_d4 acc[TPW];
 #pragma unroll
@@ -62,113 +202,29 @@ __global__ __launch_bounds__(NW * 64) void spd_gj_kernel(int M, int batch, const
       acc[u][r] = (t < T && i < M && j < M) ? Ab[(long)i * lda + j] : (i == j ? 1.0 : 0.0);
     }
   }
-  int fail = 0;
   for (int K = 0; K < MT; ++K) {
-    double *cT = colT[K & 1];
-    // (1) block row K and block column K (transposed) to LDS
+    // (1) block row and column K to LDS
 #pragma unroll
     for (int u = 0; u < TPW; ++u) {
-      const int t = w + NW * u, I = t / MT, J = t % MT;
-      if (t >= T) continue;
-      if (I == K) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) rowK[(lr + 4 * r) * RS + J * 16 + lc] = acc[u][r];
-      }
-      if (J == K) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) cT[lc * RS + I * 16 + lr + 4 * r] = acc[u][r];
-      }
+      const int t = w + NW * u;
+      if (t < T) spd_publish<MT>(sm, K, t / MT, t % MT, acc[u], lr, lc);
     }
     __syncthreads();
-    // (2) the panel.  In each panel wave lane c < 16 holds pivot column K*16 + c, lane
-    //     16 + c identity column c (ending as column c of P^-1) and lanes 32.. hold 32 of
-    //     the other columns each (ending as P^-1 A[K, col]).  Every panel wave runs the
-    //     same pivots on the same pivot columns, so their copies agree bit for bit.
-    if (w < NPW) {
-      int col = 0;
-      bool live = true;
-      if (lane < 16) {
-        col = K * 16 + lane;
-      } else if (lane >= 32) {
-        const int m = 32 * w + lane - 32;
-        live = m < MP - 16;
-        col = m < K * 16 ? m : m + 16;
-      }
-      double v[16];
-#pragma unroll
-      for (int i = 0; i < 16; ++i)
-        v[i] = (lane >= 16 && lane < 32) ? (i == lane - 16 ? 1.0 : 0.0) : (live ? rowK[i * RS + col] : 0.0);
-      int bad = 0;
-#pragma unroll
-      for (int p = 0; p < 16; ++p) {
-        const double d = gpx_readlane_d(v[p], p);
-        if (!bad && !(d > 0.0 && d < INFINITY)) bad = p + 1;
-        const double rd = gpx_recip(d);
-        double colp[16];  // column p, read from the pivot lane
-#pragma unroll
-        for (int i = 0; i < 16; ++i)
-          if (i != p) colp[i] = gpx_readlane_d(v[i], p);
-        const double rowp = v[p] * rd;
-#pragma unroll
-        for (int i = 0; i < 16; ++i)
-          if (i != p) v[i] = fma(-colp[i], rowp, v[i]);
-        v[p] = rowp;
-        if (w == 0 && lane == 0) piv[K * 16 + p] = d;
-      }
-      if (lane >= 16 && lane < 32) {
-        if (w == 0) {
-#pragma unroll
-          for (int i = 0; i < 16; ++i) Pi[i * 17 + lane - 16] = v[i];
-        }
-      } else if (lane >= 32 && live) {
-#pragma unroll
-        for (int i = 0; i < 16; ++i) rvK[i * RS + col] = v[i];
-      }
-      if (w == 0 && lane == 0) failk[K & 1] = bad ? K * 16 + bad : 0;
-    }
+    // (2) the panel
+    if (w < NPW) spd_panel<MT>(sm, K, w, lane);
     __syncthreads();
-    if (failk[K & 1]) {
-      fail = failk[K & 1];
+    if (sm.failk[K & 1]) {
+      fail = sm.failk[K & 1];
       break;
     }
-    // (3) every wave: its tiles
+    // (3) the tiles
 #pragma unroll
     for (int u = 0; u < TPW; ++u) {
-      const int t = w + NW * u, I = t / MT, J = t % MT;
-      if (t >= T) continue;
-      if (I == K) {
-        if (J == K) {
-#pragma unroll
-          for (int r = 0; r < 4; ++r) acc[u][r] = Pi[(lr + 4 * r) * 17 + lc];
-        } else {
-#pragma unroll
-          for (int r = 0; r < 4; ++r) acc[u][r] = rvK[(lr + 4 * r) * RS + J * 16 + lc];
-        }
-      } else if (J == K) {
-        gpx_d4 c = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int ks = 0; ks < 4; ++ks)
-          c = gpx_mfma16(-cT[(4 * ks + lr) * RS + I * 16 + lc], Pi[(4 * ks + lr) * 17 + lc], c);
-        acc[u] = c;
-      } else {
-#pragma unroll
-        for (int ks = 0; ks < 4; ++ks)
-          acc[u] = gpx_mfma16(-cT[(4 * ks + lr) * RS + I * 16 + lc], rvK[(4 * ks + lr) * RS + J * 16 + lc],
-                              acc[u]);
-      }
-      // keep each tile's operand loads next to its MFMAs: hoisting every tile's loads
-      // ahead of the first MFMA needs 16 registers per tile and spills for MT > 8
-      if (TPW > 6) __builtin_amdgcn_sched_barrier(0);
+      const int t = w + NW * u;
+      if (t < T) spd_update<MT>(sm, K, t / MT, t % MT, acc[u], lr, lc);
     }
   }
-  __syncthreads();  // pivots of the last block were written after the last barrier of the loop
-  double ld = 0.0;
-  for (int k = threadIdx.x; k < (fail ? 0 : M); k += NW * 64) ld += log(piv[k]);
-  ld = gpx_block_sum(ld, red);
-  if (threadIdx.x == 0) {
-    if (logdet) logdet[b] = fail ? NAN : ld;
-    if (info) info[b] = fail;
-  }
+  spd_finish<MT, NW>(sm, M, b, fail, logdet, info);
   if (!out) return;
   double *Ob = out + b * sO;
 #pragma unroll
@@ -186,10 +242,11 @@ __global__ __launch_bounds__(NW * 64) void spd_gj_kernel(int M, int batch, const
 template <int MT>
 int launch_sweep(int M, int batch, const double *A, int lda, long sA, double *out, int ldo, long sO,
                  double *logdet, int *info, hipStream_t st) {
-  // waves: enough for the panel and for at most ~10 tiles each
-  constexpr int NW = MT <= 3 ? 4 : (MT <= 9 ? 8 : 16);
-  hipLaunchKernelGGL((spd_gj_kernel<MT, NW>), dim3(batch), dim3(NW * 64), 0, st, M, batch, A, lda, sA, out,
-                     ldo, sO, logdet, info);
+  // 8 waves from M > 48 on (measured on MI355X against 16: equal at M = 200, faster
+  // everywhere else)
+  constexpr int NW = MT <= 3 ? 4 : 8;
+  hipLaunchKernelGGL((spd_gj_kernel<MT, NW>), dim3(batch), dim3(NW * 64), 0, st, M, batch, A, lda, sA, out, ldo,
+                     sO, logdet, info);
   GPX_CHECK_LAUNCH();
   return GPX_OK;
 }

@@ -85,7 +85,7 @@ int gpx_psi_bwd(int N, int M, int Q, const double *mx, const double *vx, const d
                 const double *dpsi2, const double *psi2, double *dsf2, double *dell, double *dZ,
                 double *dmx, double *dvx, double *work, void *stream);
 
-/* Adjoint of K(x, Z): dsf2, dell, dZ and optionally dx (N x Q).
+/* Adjoint of K(x, Z): dsf2, dell, dZ and optionally dx (N x Q); M <= 256, Q <= 16.
  * Replaces compute_kfu_derivatives -> kfucompDer (geepee/kernels.py:312-313, 381-399). */
 long gpx_kfu_bwd_workspace(int N, int M, int Q);
 int gpx_kfu_bwd(int N, int M, int Q, const double *dK, const double *K, const double *x,
