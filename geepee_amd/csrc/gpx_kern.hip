@@ -118,27 +118,30 @@ __global__ void kfu_bwd_scalars(int M, int Q, const double *__restrict__ red, co
 // d_trace_MKzz_dhypers (kernels.py:447-475) with Kzz = SE(z, z) (no jitter):
 //   gsf = sum_ab M_ab K_ab, gls_q = sum_ab M_ab K_ab (z_aq - z_bq)^2 / (2 l_q),
 //   gz_aq = -sum_b (M_ab + M_ba) K_ab (z_aq - z_bq) / l_q.
-// One block per row a; part[a] = [gsf_a, gls_a (Q)].
+// One wave per row a (four per workgroup), its lanes over b; part[a] = [gsf_a, gls_a (Q)].
 __global__ __launch_bounds__(256) void dtrace_kernel(int M, int Q, const double *__restrict__ Mm,
                                                      const double *__restrict__ z,
                                                      const double *__restrict__ ls,
                                                      const double *__restrict__ sf,
                                                      double *__restrict__ gz, double *__restrict__ part) {
-  __shared__ double red[16];
-  const int a = blockIdx.x, tid = threadIdx.x;
+  const int lane = threadIdx.x & 63, a = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (a >= M) return;
   const double sf2 = exp(2.0 * sf[0]);
-  double il[16];
+  double il[16], za[16];
 #pragma unroll
-  for (int q = 0; q < 16; ++q) il[q] = (q < Q) ? 1.0 / exp(2.0 * ls[q]) : 0.0;
+  for (int q = 0; q < 16; ++q) {
+    il[q] = (q < Q) ? 1.0 / exp(2.0 * ls[q]) : 0.0;
+    za[q] = (q < Q) ? z[a * Q + q] : 0.0;
+  }
   double g0 = 0.0, gl[16], gzq[16];
 #pragma unroll
   for (int q = 0; q < 16; ++q) gl[q] = gzq[q] = 0.0;
-  for (int b = tid; b < M; b += 256) {
+  for (int b = lane; b < M; b += 64) {
     double r2 = 0.0;
 #pragma unroll
     for (int q = 0; q < 16; ++q)
       if (q < Q) {
-        const double d = z[a * Q + q] - z[b * Q + q];
+        const double d = za[q] - z[b * Q + q];
         r2 += d * d * il[q];
       }
     const double K = sf2 * exp(-0.5 * r2);
@@ -147,25 +150,25 @@ __global__ __launch_bounds__(256) void dtrace_kernel(int M, int Q, const double 
 #pragma unroll
     for (int q = 0; q < 16; ++q)
       if (q < Q) {
-        const double d = z[a * Q + q] - z[b * Q + q];
+        const double d = za[q] - z[b * Q + q];
         gl[q] += 0.5 * mab * K * d * d * il[q];
         gzq[q] += (mab + mba) * K * d;
       }
   }
-  g0 = gpx_block_sum(g0, red);
-  if (tid == 0) part[(long)a * (Q + 1)] = g0;
-  for (int q = 0; q < Q; ++q) {
-    double s = 0.0, t = 0.0;
+  g0 = gpx_wave_sum(g0);
 #pragma unroll
-    for (int qq = 0; qq < 16; ++qq)
-      if (qq == q) { s = gl[qq]; t = gzq[qq]; }
-    s = gpx_block_sum(s, red);
-    t = gpx_block_sum(t, red);
-    if (tid == 0) {
-      part[(long)a * (Q + 1) + 1 + q] = s;
-      gz[a * Q + q] = -t * il[q];
+  for (int q = 0; q < 16; ++q)
+    if (q < Q) {
+      gl[q] = gpx_wave_sum(gl[q]);
+      gzq[q] = gpx_wave_sum(gzq[q]);
     }
-  }
+  if (lane == 0) part[(long)a * (Q + 1)] = g0;
+#pragma unroll
+  for (int q = 0; q < 16; ++q)
+    if (q < Q && lane == q) {
+      part[(long)a * (Q + 1) + 1 + q] = gl[q];
+      gz[a * Q + q] = -gzq[q] * il[q];
+    }
 }
 
 __global__ void dtrace_final(int M, int Q, const double *__restrict__ part, double *__restrict__ gsf,
@@ -299,16 +302,19 @@ __global__ void sym_kernel(int batch, int M, double a, const double *__restrict_
 }
 
 // v[n,d] = sf2 + sum_a T[d][n][a] kfu[n][a]   (aep_models.py:155-157)
-__global__ void det_var_kernel(int N, int D, int M, const double *__restrict__ T,
-                               const double *__restrict__ kfu, const double *__restrict__ sf,
-                               double *__restrict__ v) {
-  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+// One wave per (n, d), its lanes over a (coalesced rows of T and kfu).
+__global__ __launch_bounds__(256) void det_var_kernel(int N, int D, int M, const double *__restrict__ T,
+                                                      const double *__restrict__ kfu, const double *__restrict__ sf,
+                                                      double *__restrict__ v) {
+  const long e = blockIdx.x * 4L + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (e >= (long)N * D) return;
   const int n = (int)(e / D), d = (int)(e % D);
   const double *t = T + ((long)d * N + n) * M, *k = kfu + (long)n * M;
   double s = 0.0;
-  for (int a = 0; a < M; ++a) s += t[a] * k[a];
-  v[e] = exp(2.0 * sf[0]) + s;
+  for (int a = lane; a < M; a += 64) s += t[a] * k[a];
+  s = gpx_wave_sum(s);
+  if (lane == 0) v[e] = exp(2.0 * sf[0]) + s;
 }
 
 // v = sf2 + Bpsi2 - m^2   (aep_models.py:196-198)
@@ -716,7 +722,7 @@ int gpx_dtrace_kzz(int M, int Q, const double *Mm, const double *z, const double
   if (M < 1 || Q < 1 || Q > 16 || !Mm || !z || !ls || !sf || !gsf || !gls || !gz || !work)
     return GPX_EINVAL;
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(dtrace_kernel, dim3(M), dim3(256), 0, st, M, Q, Mm, z, ls, sf, gz, work);
+  hipLaunchKernelGGL(dtrace_kernel, dim3(gpx_cdiv(M, 4)), dim3(256), 0, st, M, Q, Mm, z, ls, sf, gz, work);
   GPX_CHECK_LAUNCH();
   hipLaunchKernelGGL(dtrace_final, dim3(1), dim3(64), 0, st, M, Q, (const double *)work, gsf, gls);
   GPX_CHECK_LAUNCH();
@@ -816,7 +822,7 @@ int gpx_det_var(int N, int D, int M, const double *T, const double *kfu, const d
   if (N < 0 || D < 0 || M < 1 || !T || !kfu || !sf || !v) return GPX_EINVAL;
   long n = (long)N * D;
   if (n == 0) return GPX_OK;
-  hipLaunchKernelGGL(det_var_kernel, dim3(gpx_cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream, N, D, M, T, kfu,
+  hipLaunchKernelGGL(det_var_kernel, dim3(gpx_cdiv(n, 4)), dim3(256), 0, (hipStream_t)stream, N, D, M, T, kfu,
                      sf, v);
   GPX_CHECK_LAUNCH();
   return GPX_OK;
