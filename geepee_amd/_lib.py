@@ -67,6 +67,9 @@ SIGNATURES = {
     'gpx_fused_bwd_workspace': (_l, [_i, _i, _i, _i, _i, _i]),
     'gpx_fused_mm_bwd': (_i, [_i, _i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p,
                               _p, _p, _p, _p, _p]),
+    'gpx_fused_mm_bwd_main': (_i, [_i, _i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p, _p,
+                                   _p, _p, _p]),
+    'gpx_fused_mm_bwd_tail': (_i, [_i, _i, _i, _i, _p, _p, _p, _i, _i, _p, _p, _p, _p, _p]),
     'gpx_psi_bwd_tail': (_i, [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p,
                               _p, _p]),
     'gpx_psi_qmax': (_i, []),

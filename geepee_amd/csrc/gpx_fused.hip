@@ -691,77 +691,118 @@ long gpx_fused_bwd_workspace(int N, int M, int Q, int D, int splits, int nsplit)
   return s;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Workspace of the fused adjoint (gpx_fused_bwd_workspace doubles), carved in order.
+struct FusedBwdWork {
+  double *dBpart, *LWpart, *LW, *Tpart, *Vpart, *dpsi1, *Tn, *Lp, *H, *s0, *Dn, *Pn, *En, *TP, *LE, *cols0, *pf;
+  FusedBwdWork(double *work, int N, int M, int Q, int D, int splits, int nsplit) {
+    const long MM = (long)M * M;
+    dBpart = work;
+    LWpart = dBpart + (long)nsplit * D * MM;
+    LW = LWpart + (long)nsplit * MM * (Q + 1);
+    Tpart = LW + MM * (Q + 1);
+    Vpart = Tpart + (long)splits * N * M;
+    dpsi1 = Vpart + (long)splits * N * Q;
+    Tn = dpsi1 + (long)N * M;
+    Lp = Tn + (long)N * M;
+    H = Lp + (long)N * M;
+    s0 = H + (long)N * Q;
+    Dn = s0 + 2L * N;
+    Pn = Dn + (long)N * (Q + 1);
+    En = Pn + 2L * N * Q;
+    TP = En + 2L * N * Q;
+    LE = TP + 2L * M * Q;
+    cols0 = LE + 2L * M * Q;
+    pf = cols0 + 2;
+  }
+};
+
+bool fused_bwd_args_ok(int N, int M, int Q, int D, int splits, int nsplit) {
+  // Q <= FQ_MAX - 1: the [1 | d_n] contraction fills the Q+1 rows of one 16-row MFMA tile
+  return N >= 1 && M >= 1 && M <= FM_MAX && Q >= 1 && Q <= FQ_MAX - 1 && D >= 1 && D <= 32 && splits >= 1 &&
+         nsplit >= 1 && splits <= (M + 15) / 16;  // the per-point pass splits over column tiles
+}
+
+}  // namespace
+
+extern "C" {
+
 // Adjoint of the moment-matched layer without materialising psi2 or dL/dpsi2:
 // given dm_all = dm - 2 dv m and dv (N x D), Ahat (D x M), Bp (pair-major symmetrised
 // Bhat, M x M x D) and psi1 from the forward: dBhat (D x M x M), dsf2, dell, dZ, dmx, dvx.
 // dL/dpsi2 = sum_d dv_nd Bsym[d] is symmetric by construction (kernels.py:415-418).
-int gpx_fused_mm_bwd(int N, int M, int Q, int D, const double *mx, const double *vx, const double *z,
-                     const double *ls, const double *sf, const double *Ahat, const double *Bp,
-                     const double *psi1, const double *dm_all, const double *dv, int splits, int nsplit,
-                     double *dBhat, double *dsf2, double *dell, double *dZ, double *dmx, double *dvx,
-                     double *work, void *stream) {
-  // Q <= FQ_MAX - 1: the [1 | d_n] contraction fills the Q+1 rows of one 16-row MFMA tile
-  if (N < 1 || M < 1 || M > FM_MAX || Q < 1 || Q > FQ_MAX - 1 || D < 1 || D > 32 || splits < 1 ||
-      nsplit < 1)
-    return GPX_EINVAL;
-  if (!mx || !vx || !z || !ls || !sf || !Ahat || !Bp || !psi1 || !dm_all || !dv || !dBhat || !dsf2 ||
-      !dell || !dZ || !dmx || !dvx || !work)
+// _main produces dBhat, dmx and dvx (what the layer below needs) and leaves the point and
+// pair statistics in `work`; _tail turns them into dsf2, dell, dZ and may run on another
+// stream once _main has completed there.
+int gpx_fused_mm_bwd_main(int N, int M, int Q, int D, const double *mx, const double *vx, const double *z,
+                          const double *ls, const double *sf, const double *Ahat, const double *Bp,
+                          const double *psi1, const double *dm_all, const double *dv, int splits, int nsplit,
+                          double *dBhat, double *dmx, double *dvx, double *work, void *stream) {
+  if (!fused_bwd_args_ok(N, M, Q, D, splits, nsplit)) return GPX_EINVAL;
+  if (!mx || !vx || !z || !ls || !sf || !Ahat || !Bp || !psi1 || !dm_all || !dv || !dBhat || !dmx || !dvx ||
+      !work)
     return GPX_EINVAL;
   hipStream_t st = (hipStream_t)stream;
   const int T = (M + 15) / 16, ntiles = T * (T + 1) / 2;
-  if (splits > T) return GPX_EINVAL;  // the per-point pass splits over column tiles
-  const long MM = (long)M * M;
-  double *dBpart = work;
-  double *LWpart = dBpart + (long)nsplit * D * MM;
-  double *LW = LWpart + (long)nsplit * MM * (Q + 1);
-  double *Tpart = LW + MM * (Q + 1);
-  double *Vpart = Tpart + (long)splits * N * M;
-  double *dpsi1 = Vpart + (long)splits * N * Q;
-  double *Tn = dpsi1 + (long)N * M;
-  double *Lp = Tn + (long)N * M;
-  double *H = Lp + (long)N * M;
-  double *s0 = H + (long)N * Q;
-  double *Dn = s0 + 2L * N;
-  double *Pn = Dn + (long)N * (Q + 1);
-  double *En = Pn + 2L * N * Q;
-  double *TP = En + 2L * N * Q;
-  double *LE = TP + 2L * M * Q;
-  double *cols0 = LE + 2L * M * Q;
-  double *pf = cols0 + 2;
-  hipLaunchKernelGGL(fused_point_feat_kernel, dim3(gpx_cdiv(N, 256)), dim3(256), 0, st, N, Q, mx, vx, ls, pf);
+  FusedBwdWork W(work, N, M, Q, D, splits, nsplit);
+  hipLaunchKernelGGL(fused_point_feat_kernel, dim3(gpx_cdiv(N, 256)), dim3(256), 0, st, N, Q, mx, vx, ls, W.pf);
   GPX_CHECK_LAUNCH();
   // per-pair pass over point ranges
   const int nchunk = ((N + nsplit - 1) / nsplit + 15) / 16 * 16;
   dim3 gp(ntiles * 2, nsplit);
   if (D <= 16)
     hipLaunchKernelGGL((fused_bwd_pairs_kernel<1, 4>), gp, dim3(256), 0, st, N, M, Q, D, z, ls, sf, Bp, dv,
-                       (const double *)pf, nsplit, nchunk, dBpart, LWpart);
+                       (const double *)W.pf, nsplit, nchunk, W.dBpart, W.LWpart);
   else
     hipLaunchKernelGGL((fused_bwd_pairs_kernel<2, 8>), gp, dim3(256), 0, st, N, M, Q, D, z, ls, sf, Bp, dv,
-                       (const double *)pf, nsplit, nchunk, dBpart, LWpart);
+                       (const double *)W.pf, nsplit, nchunk, W.dBpart, W.LWpart);
   GPX_CHECK_LAUNCH();
   hipLaunchKernelGGL(fused_pairs_finish, dim3(ntiles, D + 1), dim3(256), 0, st, M, Q, D, nsplit,
-                     (const double *)dBpart, (const double *)LWpart, dBhat, LW);
+                     (const double *)W.dBpart, (const double *)W.LWpart, dBhat, W.LW);
   GPX_CHECK_LAUNCH();
   // per-point pass over tile splits
   dim3 gq(gpx_cdiv(N, 16), splits);
   if (D <= 16)
     hipLaunchKernelGGL((fused_bwd_points_kernel<4>), gq, dim3(256), 0, st, N, M, Q, D, mx, vx, z, ls, sf, Bp, dv,
-                       splits, Tpart, Vpart);
+                       splits, W.Tpart, W.Vpart);
   else
     hipLaunchKernelGGL((fused_bwd_points_kernel<8>), gq, dim3(256), 0, st, N, M, Q, D, mx, vx, z, ls, sf, Bp, dv,
-                       splits, Tpart, Vpart);
+                       splits, W.Tpart, W.Vpart);
   GPX_CHECK_LAUNCH();
   // dpsi1 = dm_all Ahat
-  int rc = gpx_gemm_batched(0, 0, N, M, D, 1.0, dm_all, D, 0, Ahat, M, 0, 0.0, dpsi1, M, 0, 1, 1, nullptr, 0,
+  int rc = gpx_gemm_batched(0, 0, N, M, D, 1.0, dm_all, D, 0, Ahat, M, 0, 0.0, W.dpsi1, M, 0, 1, 1, nullptr, 0,
                             stream);
   if (rc) return rc;
   hipLaunchKernelGGL(fused_rows_finish, dim3(gpx_cdiv(N, 4)), dim3(256), 0, st, N, M, Q, splits, mx, vx, z, ls,
-                     (const double *)Tpart, (const double *)Vpart, (const double *)dpsi1, psi1, dmx, dvx, Tn, Lp,
-                     H, s0, Dn, Pn, En);
+                     (const double *)W.Tpart, (const double *)W.Vpart, (const double *)W.dpsi1, psi1, dmx, dvx,
+                     W.Tn, W.Lp, W.H, W.s0, W.Dn, W.Pn, W.En);
   GPX_CHECK_LAUNCH();
-  return gpx_psi_bwd_tail(N, M, Q, z, ls, sf, Tn, Lp, H, s0, Dn, Pn, En, LW, TP, LE, cols0, dsf2, dell, dZ, pf + (long)N * PF_W,
-                          stream);
+  return GPX_OK;
+}
+
+int gpx_fused_mm_bwd_tail(int N, int M, int Q, int D, const double *z, const double *ls, const double *sf,
+                          int splits, int nsplit, double *dsf2, double *dell, double *dZ, double *work,
+                          void *stream) {
+  if (!fused_bwd_args_ok(N, M, Q, D, splits, nsplit)) return GPX_EINVAL;
+  if (!z || !ls || !sf || !dsf2 || !dell || !dZ || !work) return GPX_EINVAL;
+  FusedBwdWork W(work, N, M, Q, D, splits, nsplit);
+  return gpx_psi_bwd_tail(N, M, Q, z, ls, sf, W.Tn, W.Lp, W.H, W.s0, W.Dn, W.Pn, W.En, W.LW, W.TP, W.LE, W.cols0,
+                          dsf2, dell, dZ, W.pf + (long)N * PF_W, stream);
+}
+
+int gpx_fused_mm_bwd(int N, int M, int Q, int D, const double *mx, const double *vx, const double *z,
+                     const double *ls, const double *sf, const double *Ahat, const double *Bp,
+                     const double *psi1, const double *dm_all, const double *dv, int splits, int nsplit,
+                     double *dBhat, double *dsf2, double *dell, double *dZ, double *dmx, double *dvx,
+                     double *work, void *stream) {
+  if (!dsf2 || !dell || !dZ) return GPX_EINVAL;
+  int rc = gpx_fused_mm_bwd_main(N, M, Q, D, mx, vx, z, ls, sf, Ahat, Bp, psi1, dm_all, dv, splits, nsplit, dBhat,
+                                 dmx, dvx, work, stream);
+  if (rc) return rc;
+  return gpx_fused_mm_bwd_tail(N, M, Q, D, z, ls, sf, splits, nsplit, dsf2, dell, dZ, work, stream);
 }
 
 }  // extern "C"

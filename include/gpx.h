@@ -233,6 +233,18 @@ int gpx_fused_mm_bwd(int N, int M, int Q, int D, const double *mx, const double 
                      const double *psi1, const double *dm_all, const double *dv, int splits, int nsplit,
                      double *dBhat, double *dsf2, double *dell, double *dZ, double *dmx, double *dvx,
                      double *work, void *stream);
+/* gpx_fused_mm_bwd in two parts over the same workspace: _main gives dBhat, dmx, dvx (the
+ * input gradients the layer below needs first); _tail gives dsf2, dell, dZ and may be
+ * queued on another stream that waits for _main, so a deep GP's backward can continue
+ * down the layers while the hyper-parameter adjoint finishes.  Same reference code as
+ * gpx_fused_mm_bwd. */
+int gpx_fused_mm_bwd_main(int N, int M, int Q, int D, const double *mx, const double *vx, const double *z,
+                          const double *ls, const double *sf, const double *Ahat, const double *Bp,
+                          const double *psi1, const double *dm_all, const double *dv, int splits, int nsplit,
+                          double *dBhat, double *dmx, double *dvx, double *work, void *stream);
+int gpx_fused_mm_bwd_tail(int N, int M, int Q, int D, const double *z, const double *ls, const double *sf,
+                          int splits, int nsplit, double *dsf2, double *dell, double *dZ, double *work,
+                          void *stream);
 
 /* Shared final stage of both psi adjoints (pair sums LW and per-point statistics in, dsf2,
  * dell, dZ out); the tail of compute_psi_derivatives (geepee/kernels.py:430-442, 371-376). */

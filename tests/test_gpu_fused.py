@@ -69,6 +69,15 @@ def test_fused_layer_matches_plain(N, M, Q, D):
     plain = [dBhat] + list(dops.psi_bwd(mxd, vxd, L.zu, L.ls, L.sf, dops.mm(dm_all, L.Ahat), p10, dpsi2, p20))
     for a, b, name in zip(fused, plain, ('dBhat', 'dsf2', 'dell', 'dZ', 'dmx', 'dvx')):
         assert nrel(a.cpu().numpy(), b.cpu().numpy()) < 1e-12, name
+    # the split entry (main part here, tail on another stream) gives the same bits
+    side = torch.cuda.Stream()
+    dB2, dmx2, dvx2, tail = dops.fused_mm_bwd_split(mxd, vxd, L.zu, L.ls, L.sf, L.Ahat, L._Bp, p11, dm_all, dv)
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        t3 = tail()
+    torch.cuda.current_stream().wait_stream(side)
+    for a, b, name in zip([dB2] + list(t3) + [dmx2, dvx2], fused, ('dBhat', 'dsf2', 'dell', 'dZ', 'dmx', 'dvx')):
+        assert torch.equal(a, b), name
     # layer level: the rest of the chain is shared code through Kuu^-1; the problems are
     # chosen so that a 1e-15 relative change of psi2 moves no layer gradient by more than
     # ~1e-12 (inducing points spread out against the lengthscale)
