@@ -395,6 +395,8 @@ class SDGPR(Base_Model):
         for i in range(self.L - 1, -1, -1):
             layer = self.sgp_layers[i]
             if i == 0:
+                if streams:
+                    torch.cuda.current_stream().wait_stream(streams[0])
                 g = layer.backprop_grads_reg(ms[0], vs[0], dmi, dvi, p1[0], xb, alpha)
             else:
                 g, gin = layer.backprop_grads_lvm_mm(ms[i], vs[i], dmi, dvi, p1[i], p2[i],
@@ -430,14 +432,20 @@ class SDGPR(Base_Model):
             return phis
         if getattr(self, '_streams', None) is None:
             self._streams = [torch.cuda.Stream() for _ in range(self.L)]
-        phis = []
+        phis, ready = [], []
         for i, (layer, st) in enumerate(zip(self.sgp_layers, self._streams)):
             st.wait_stream(cur)
             with torch.cuda.stream(st):
                 layer.prepare(dp, alpha, key_suffix='_%d' % i)
                 phis.append(layer.compute_phi(alpha))
-        for st in self._streams:
-            cur.wait_stream(st)
+                ev = torch.cuda.Event()
+                ev.record(st)
+                ready.append(ev)
+                # the adjoint's q(u)-only terms follow on the same stream; the forward
+                # waits only for the q(u) set-up above, the layer's backward for these
+                layer.adjoint_constants('reg' if i == 0 else 'mm', alpha)
+        for ev in ready:
+            cur.wait_event(ev)
         return phis
 
     def predict_f(self, inputs):
@@ -502,6 +510,9 @@ class SGPLVM(Base_Model):
         dp = self._device_params(params)
         layer = self.sgp_layer
         layer.prepare(dp, alpha)
+        # (a second stream for phi and the adjoint's q(u)-only terms beside the forward
+        # measured slower here: the fused forward already fills the GPU)
+        phi = layer.compute_phi(alpha)
         self.lik_layer.update_hypers(dp)
         xs = self._xf.lvm_x_state(dp['x1'], dp['x2'], idx, alpha, self.prior_mean,
                                   self.prior_var, self.nat_param)
@@ -518,7 +529,7 @@ class SGPLVM(Base_Model):
         grads['x1'] = gx1
         grads['x2'] = gx2
         energy = lincomb(torch.empty(1, dtype=F64, device='cuda'), scale_logZ, scal[0:1], 1.0,
-                         xcontrib, 1.0, layer.compute_phi(alpha))
+                         xcontrib, 1.0, phi)
         return energy, grads, None
 
     def predict_f(self, inputs):

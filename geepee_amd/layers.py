@@ -75,6 +75,7 @@ class SGP_Layer(object):
         # theta1 = R^T R; the natural parameterisation only ever needs Kuu^-1 + c theta1,
         # formed by the product's epilogue where it is used
         self.theta_1 = None if self.nat_param else mm(self.theta_1_R, self.theta_1_R, tA=True)
+        self._adj = None
         self.compute_kuu()
         self.update_posterior()
 
@@ -92,6 +93,7 @@ class SGP_Layer(object):
         self.theta_2 = params['eta2' + key_suffix]
         R = self.theta_1_R = dops.eta_to_R(params['eta1_R' + key_suffix], self.M)
         self.theta_1 = None
+        self._adj = None
         self.compute_kuu()
         D, M = self.Dout, self.M
         beta = (self.N - alpha) * 1.0 / self.N
@@ -109,7 +111,7 @@ class SGP_Layer(object):
         self.Splusmm = _plus_outer(self.Su, self.mu, self.mu)
         self._pred_ready = False
         self.muhat = mm(self.Suhat, _col(self.theta_2), alpha=beta).reshape(D, M)
-        self._beta = beta
+        self._beta, self._alpha = beta, alpha
         self._cavity_tail()
 
     def compute_kuu(self):
@@ -149,7 +151,7 @@ class SGP_Layer(object):
         D, M = self.Dout, self.M
         beta = (self.N - alpha) * 1.0 / self.N
         Ki = self.Kuuinv
-        self._beta = beta
+        self._beta, self._alpha = beta, alpha
         if self.nat_param:
             R = self.theta_1_R
             self.Suhatinv = mm(R, R, dops.empty(D, M, M, like=Ki), tA=True, alpha=beta, E=Ki, gamma=1.0)
@@ -162,6 +164,47 @@ class SGP_Layer(object):
             f2 = mm(self.Suinv, _col(self.mu))
             self.muhat = mm(self.Suhat, f2, alpha=beta).reshape(D, M)
         self._cavity_tail()
+
+    def adjoint_constants(self, kind, alpha=None):
+        """Parts of the adjoint that depend on q(u) and Kuu only, formed once per q(u)
+        (after prepare / compute_cavity, so they can run beside the forward pass):
+          'reg' (natural parameters): the posterior chain dSuinv_post = -Su dSu Su with
+            dSu = sp/2 (Sigma^-1 + theta2 theta2^T), Kc = sum_d dSuinv_post - (D/2) Kuu,
+            and KiS2 = 2 Ki Suhat - I;
+          'mm': Mc = sp sum_d Splusmm + sc sum_d Splusmmhat, T1 = -sp/2 Splusmm
+            - sc beta/2 Splusmmhat, c2 = sp mu + sc beta muhat, KiS2 = 2 Ki Splusmmhat - I
+        (sp = N/alpha - 1, sc = -N/alpha; aep_models.py:245-304, 412-511).  The backward
+        passes use them when present."""
+        N, D, M = self.N, self.Dout, self.M
+        alpha = self._alpha if alpha is None else alpha
+        beta = (N - alpha) * 1.0 / N
+        sp, sc = N * 1.0 / alpha - 1.0, -N * 1.0 / alpha
+        Ki = self.Kuuinv
+        c = {'kind': kind, 'alpha': alpha}
+        if kind == 'reg':
+            if not self.nat_param:
+                return
+            th = self.theta_2
+            dSu = _plus_outer(self.Suinv, th, th, alpha=0.5 * sp, gamma=0.5 * sp)
+            c['dS_post'] = _sandwich(self.Su, dSu, self.Su, alpha=-1.0)
+            c['Kc'] = lincomb(torch.empty_like(Ki), -0.5 * D, self.Kuu)
+            colsum(c['dS_post'], D, M * M, out=c['Kc'].reshape(-1), beta=1.0)
+            c['KiS2'] = mm(Ki, self.Suhat, alpha=2.0, E=_eye(M, Ki), gamma=-1.0)
+        else:
+            c['Mc'] = colsum(self.Splusmm, D, M * M, alpha=sp).reshape(M, M)
+            colsum(self.Splusmmhat, D, M * M, out=c['Mc'].reshape(-1), alpha=sc, beta=1.0)
+            c['T1'] = lincomb(torch.empty_like(self.Splusmm), -0.5 * sp, self.Splusmm, -0.5 * sc * beta,
+                              self.Splusmmhat)
+            c['c2'] = lincomb(torch.empty_like(self.mu), sp, self.mu, sc * beta, self.muhat)
+            c['KiS2'] = mm(Ki, self.Splusmmhat, alpha=2.0, E=_eye(M, Ki), gamma=-1.0)
+        self._adj = c
+
+    def _adjoint(self, kind, alpha):
+        c = getattr(self, '_adj', None)
+        if c is None or c['kind'] != kind or c['alpha'] != alpha:
+            self.adjoint_constants(kind, alpha)
+            c = self._adj
+        return c
 
     def _cavity_tail(self):
         """Ahat, Bhat_sto, Bhat_det from muhat, Suhat (aep_models.py:535-546):
@@ -357,20 +400,17 @@ class SGP_Layer(object):
         dSucav = mm(dops.rowscale(dv, kfuKi), kfuKi, tA=True, E=self.Suhatinv, gamma=0.5 * scale_cav)
         _add_outer(dSucav, th, th, -0.5 * scale_cav * beta * beta)
         _add_outer(dSucav, dmucav, th, beta)
-        dS = dops.empty(2 * D, M, M, like=Ki)
-        _sandwich(self.Suhat, dSucav, self.Suhat, alpha=-1.0, out=dS[:D], beta=0.0)
-        # posterior
-        dSu = _plus_outer(self.Suinv, th, th, alpha=0.5 * scale_post, gamma=0.5 * scale_post)
-        _sandwich(self.Su, dSu, self.Su, alpha=-1.0, out=dS[D:], beta=0.0)
-        deta1 = dops.pack_dR(self.theta_1_R, lincomb(dSu, beta, dS[:D], 1.0, dS[D:]))
+        c = self._adjoint('reg', alpha)
+        dS_cav = _sandwich(self.Suhat, dSucav, self.Suhat, alpha=-1.0)
+        # posterior chain (data independent, from adjoint_constants)
+        deta1 = dops.pack_dR(self.theta_1_R, lincomb(dSucav, beta, dS_cav, 1.0, c['dS_post']))
         deta2 = mm(self.Suhat, _col(dmucav), alpha=beta, E=_col(self.mu), gamma=scale_post).reshape(D, M)
         # Kuuinv: phi prior, Ahat, Bhat_det, cavity, posterior
         dAhat = mm(dm, kfu, tA=True)
-        dKi = mm(dAhat, self.muhat, tA=True, E=self.Kuu, gamma=-0.5 * D)
+        dKi = mm(dAhat, self.muhat, tA=True, E=c['Kc'], gamma=1.0)
         dBhat = mm(dops.rowscale(dv, kfu), kfu, tA=True)
-        KiS2 = mm(Ki, self.Suhat, alpha=2.0, E=_eye(M, Ki), gamma=-1.0)
-        mm(KiS2.reshape(D * M, M), dBhat.reshape(D * M, M), dKi, tA=True, beta=1.0)
-        colsum(dS, 2 * D, M * M, out=dKi.reshape(-1), beta=1.0)
+        mm(c['KiS2'].reshape(D * M, M), dBhat.reshape(D * M, M), dKi, tA=True, beta=1.0)
+        colsum(dS_cav, D, M * M, out=dKi.reshape(-1), beta=1.0)
         M_inner = _sandwich(Ki, dKi, Ki, alpha=-1.0)
         gsf, gls, gz = dops.dtrace(M_inner, self.zu, self.ls, self.sf)
         dls, dsf = dops.hyper(self.ls, self.sf, dell, gls, dsf2, sumdv, gsf)
@@ -442,18 +482,14 @@ class SGP_Layer(object):
         mm(dAhat, Ki, dmcav, tB=True, beta=1.0)
         _add_outer(dvcav, dmcav, self.theta_2, beta)
         dvcavinv = _sandwich(self.Suhat, dvcav, self.Suhat, alpha=-1.0)
-        dtheta2 = mm(self.Suhat, _col(dmcav), alpha=beta).reshape(D, M)
+        c = self._adjoint('mm', alpha)
+        dtheta2 = mm(self.Suhat, _col(dmcav), alpha=beta, E=_col(c['c2']), gamma=1.0).reshape(D, M)
         dKi = colsum(dvcavinv, D, M * M).reshape(M, M)
         mm(dAhat, self.muhat, dKi, tA=True, beta=1.0)
-        # 2 sum_d KiS_d^T dBhat_d - sum_d dBhat_d as one product with (2 KiS - I)
-        KiS2 = mm(Ki, self.Splusmmhat, alpha=2.0, E=_eye(M, Ki), gamma=-1.0)
-        mm(KiS2.reshape(D * M, M), dBhat.reshape(D * M, M), dKi, tA=True, beta=1.0)
-        Minner = colsum(self.Splusmm, D, M * M, alpha=scale_post).reshape(M, M)
-        colsum(self.Splusmmhat, D, M * M, out=Minner.reshape(-1), alpha=scale_cav, beta=1.0)
-        lincomb(Minner, 1.0, Minner, -2.0, dKi)
-        dtheta1 = lincomb(torch.empty_like(dvcavinv), beta, dvcavinv, -0.5 * scale_post, self.Splusmm,
-                          -0.5 * scale_cav * beta, self.Splusmmhat)
-        lincomb(dtheta2, 1.0, dtheta2, scale_post, self.mu, scale_cav * beta, self.muhat)
+        # 2 sum_d KiS_d^T dBhat_d - sum_d dBhat_d as one product with KiS2 = 2 KiS - I
+        mm(c['KiS2'].reshape(D * M, M), dBhat.reshape(D * M, M), dKi, tA=True, beta=1.0)
+        Minner = lincomb(torch.empty_like(dKi), 1.0, c['Mc'], -2.0, dKi)
+        dtheta1 = lincomb(torch.empty_like(dvcavinv), beta, dvcavinv, 1.0, c['T1'])
         deta1 = dops.pack_dR(self.theta_1_R, dtheta1)
         M_all = _sandwich(Ki, Minner, Ki, alpha=0.5, E=Ki, gamma=0.5 * D)
         gsf, gls, gz = dops.dtrace(M_all, self.zu, self.ls, self.sf)
