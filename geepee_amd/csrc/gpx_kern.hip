@@ -390,7 +390,8 @@ __global__ void gauss_tilted_finish(int nblk, int B, int D, const double *__rest
 
 // SGPSSM.compute_transition_tilted, 2-D branch (aep_models.py:1317-1343).
 // Outputs are already scaled; dmall = dmprop - 2 dvprop mprop for the MM adjoint.
-__global__ __launch_bounds__(1024) void transition_kernel(int B, int D, const double *__restrict__ mp,
+// Up to 64 blocks; part[2b], part[2b+1] = block b's (logZ, sum dv) partials.
+__global__ __launch_bounds__(256) void transition_kernel(int B, int D, const double *__restrict__ mp,
                                                           const double *__restrict__ vp,
                                                           const double *__restrict__ mt,
                                                           const double *__restrict__ vt,
@@ -399,13 +400,13 @@ __global__ __launch_bounds__(1024) void transition_kernel(int B, int D, const do
                                                           double *__restrict__ dvp,
                                                           double *__restrict__ dmt,
                                                           double *__restrict__ dvt,
-                                                          double *__restrict__ scal) {
+                                                          double *__restrict__ part) {
   __shared__ double red[16];
   const double sn2 = exp(2.0 * sn[0]);
   const double two_pi = 6.283185307179586;
   double lz = 0.0, sdv = 0.0;
   const long n = (long)B * D;
-  for (long e = threadIdx.x; e < n; e += blockDim.x) {
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
     const double vs = vt[e] + vp[e] + sn2 / alpha;
     const double md = mt[e] - mp[e];
     lz += -0.5 * md * md / vs - 0.5 * log(1.0 + alpha * (vt[e] + vp[e]) / sn2) -
@@ -421,9 +422,23 @@ __global__ __launch_bounds__(1024) void transition_kernel(int B, int D, const do
   lz = gpx_block_sum(lz, red);
   sdv = gpx_block_sum(sdv, red);
   if (threadIdx.x == 0) {
-    scal[0] = scale * lz;
-    scal[1] = sdv * 2.0 * sn2 / alpha + scale * (double)B * D * (1.0 - alpha);
+    part[2 * blockIdx.x] = lz;
+    part[2 * blockIdx.x + 1] = sdv;
   }
+}
+
+__global__ void transition_finish(int nblk, int B, int D, const double *__restrict__ part,
+                                  const double *__restrict__ sn, double alpha, double scale,
+                                  double *__restrict__ scal) {
+  if (threadIdx.x != 0) return;
+  const double sn2 = exp(2.0 * sn[0]);
+  double lz = 0.0, sdv = 0.0;
+  for (int b = 0; b < nblk; ++b) {
+    lz += part[2 * b];
+    sdv += part[2 * b + 1];
+  }
+  scal[0] = scale * lz;
+  scal[1] = sdv * 2.0 * sn2 / alpha + scale * (double)B * D * (1.0 - alpha);
 }
 
 // Gauss_Emis.compute_emission_tilted (lik_layers.py:573-627) for Dout <= 16, one thread
@@ -553,6 +568,7 @@ __global__ void emission_final(int nblk, int Nb, int Dout, int Din, const double
   const int W = Dout * Din + Dout + 1;
   for (int i = threadIdx.x; i < W; i += blockDim.x) {
     double s = 0.0;
+#pragma unroll 8
     for (int b = 0; b < nblk; ++b) s += part[(long)b * W + i];
     if (i == 0) {
       scal[0] = s * scale;
@@ -869,8 +885,15 @@ int gpx_transition_tilted(int B, int D, const double *mp, const double *vp, cons
   if (B < 1 || D < 1 || !(alpha > 0) || !mp || !vp || !mt || !vt || !sn || !dmall || !dvp || !dmt ||
       !dvt || !scal)
     return GPX_EINVAL;
-  hipLaunchKernelGGL(transition_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, B, D, mp, vp, mt, vt, sn,
-                     alpha, scale, dmall, dvp, dmt, dvt, scal);
+  hipStream_t st = (hipStream_t)stream;
+  long n = (long)B * D;
+  int nblk = (int)gpx_cdiv(n, 1024);
+  if (nblk > 64) nblk = 64;
+  hipLaunchKernelGGL(transition_kernel, dim3(nblk), dim3(256), 0, st, B, D, mp, vp, mt, vt, sn, alpha, scale, dmall,
+                     dvp, dmt, dvt, scal + 2);
+  GPX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(transition_finish, dim3(1), dim3(64), 0, st, nblk, B, D, (const double *)(scal + 2), sn,
+                     alpha, scale, scal);
   GPX_CHECK_LAUNCH();
   return GPX_OK;
 }

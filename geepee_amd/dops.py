@@ -220,7 +220,7 @@ def gauss_tilted(m, v, y, sn, alpha, scale, want_dmall=False):
 def transition(mp, vp, mt, vt, sn, alpha, scale):
     B, D = mp.shape
     outs = [torch.empty_like(mp) for _ in range(4)]
-    scal = empty(2, like=mp)
+    scal = empty(2 + 2 * 64, like=mp)
     call('gpx_transition_tilted', B, D, P(mp), P(vp), P(mt), P(vt), P(sn), float(alpha),
          float(scale), *[P(o) for o in outs], P(scal), S())
     return outs + [scal]

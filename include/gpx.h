@@ -184,7 +184,8 @@ int gpx_gauss_tilted(int B, int D, const double *m, const double *v, const doubl
                      const double *sn, double alpha, double scale, double *dm, double *dv,
                      double *dmall, double *scal, void *stream);
 
-/* GPSSM transition tilted normaliser (scaled): scal[0] = scale * logZ, scal[1] = dsn.
+/* GPSSM transition tilted normaliser (scaled): scal[0] = scale * logZ, scal[1] = dsn (scal
+ * holds 130 doubles: the rest are block partials).
  * Replaces SGPSSM.compute_transition_tilted (geepee/aep_models.py:1317-1343). */
 int gpx_transition_tilted(int B, int D, const double *mp, const double *vp, const double *mt,
                           const double *vt, const double *sn, double alpha, double scale,

@@ -310,15 +310,17 @@ def test_gauss_tilted():
     assert abs(sc[1] - dsn[0]) < 1e-12 * abs(dsn[0])
 
 
-def test_transition_and_emission():
+@pytest.mark.parametrize('B', [99, 5000])
+def test_transition_and_emission(B):
     L = _lib()
-    say('transition/emission')
+    say('transition/emission', B)
     rs = np.random.RandomState(9)
-    B, D, alpha, scale = 99, 4, 0.5, -3.0
+    D, alpha, scale = 4, 0.5, -3.0
     mp, vp, mt, vt = rs.randn(B, D), rs.uniform(0.1, 1, (B, D)), rs.randn(B, D), rs.uniform(0.1, 1, (B, D))
     sn = np.array([np.log(0.2)])
     lz, dmp, dvp, dmt, dvt, dsn = O.transition_tilted(sn, mp, vp, mt, vt, alpha, scale, D)
-    o = [dev(np.zeros((B, D))) for _ in range(4)] + [dev(np.zeros(2))]
+    # scal: [logZ, dsn] and up to 64 blocks' partials
+    o = [dev(np.zeros((B, D))) for _ in range(4)] + [dev(np.zeros(2 + 2 * 64))]
     L.call('gpx_transition_tilted', B, D, L.P(dev(mp)), L.P(dev(vp)), L.P(dev(mt)), L.P(dev(vt)),
            L.P(dev(sn)), alpha, scale, *[L.P(t) for t in o], L.S())
     assert nrel(host(o[0]), dmp - 2 * dvp * mp) < 1e-14
