@@ -85,6 +85,7 @@ __global__ __launch_bounds__(256) void kfu_bwd_final(int nblk, int M, int Q, con
   if (e >= M * (Q + 1)) return;
   const int m = e / (Q + 1), q1 = e % (Q + 1), W = 2 * Q + 1;
   double cs = 0.0;
+#pragma unroll 8
   for (int b = 0; b < nblk; ++b) cs += part[((long)b * M + m) * W];
   if (q1 == 0) {
     red[e] = cs;
@@ -92,6 +93,7 @@ __global__ __launch_bounds__(256) void kfu_bwd_final(int nblk, int M, int Q, con
   }
   const int q = q1 - 1;
   double lx = 0.0, ld2 = 0.0;
+#pragma unroll 8
   for (int b = 0; b < nblk; ++b) {
     const double *P = part + ((long)b * M + m) * W;
     lx += P[1 + q];
