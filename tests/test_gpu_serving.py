@@ -67,6 +67,21 @@ def test_grouped_predict_parity(noisy):
         assert _close(store.predict_plain(mid, X, noisy), _oracle(specs[mid], X, noisy))
 
 
+def test_grouped_predict_parity_served_sizes():
+    """The served model set's sizes (M in {32, 64, 128, 256}, Din 1-10, SGPR and two-layer
+    SDGPR), requests up to 600 points, against the oracle."""
+    from geepee_amd.serving import ModelStore
+    say('grouped predict parity, served sizes')
+    specs = W.serving_models(n_models=8, seed=4)
+    store = ModelStore(specs)
+    rs = np.random.RandomState(2)
+    reqs = [(i, rs.randn(int(rs.choice([1, 37, 600])), specs[i]['Din'])) for i in range(8)]
+    outs = store.predict_many(reqs, noisy=True)
+    for (mid, X), (m, v) in zip(reqs, outs):
+        ref = _oracle(specs[mid], X, True)
+        assert _close((m, v), ref), (mid, specs[mid]['Ms'], nrel(m, ref[0][0]), nrel(v, ref[1][0]))
+
+
 def test_batch_invariance_bitwise():
     from geepee_amd.serving import ModelStore
     say('batch invariance')
