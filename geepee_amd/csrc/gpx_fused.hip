@@ -156,7 +156,14 @@ __global__ __launch_bounds__(256) void fused_fwd_kernel(int N, int M, int Q, int
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int b = 16 * B + (lane >> 4) + 4 * r;
-          p[r] = (b < M && nvalid[u]) ? w * exp(e[r]) : 0.0;
+          // exp on every lane and a 0/1 weight instead of a branch around it (padding
+          // rows/columns have finite exponents: zero pair features, or the features of a
+          // point at 0 with zero variance); at NSUB = 4 the branch-free form needs more
+          // registers than two waves per SIMD leave
+          if constexpr (NSUB <= 2)
+            p[r] = w * exp(e[r]) * ((b < M && nvalid[u]) ? 1.0 : 0.0);
+          else
+            p[r] = (b < M && nvalid[u]) ? w * exp(e[r]) : 0.0;
         }
 #pragma unroll
         for (int db = 0; db < DB; ++db)
@@ -277,7 +284,7 @@ __global__ __launch_bounds__(256) void fused_bwd_points_kernel(
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int b = 16 * B + (lane >> 4) + 4 * r;
-        const double p = (b < M && nvalid) ? sf4 * exp(e[r]) : 0.0;
+        const double p = sf4 * exp(e[r]) * ((b < M && nvalid) ? 1.0 : 0.0);  // branch-free, as above
         L[r] = g[r] * p;
         rs += L[r];
         if (off) colp[r] += L[r];
@@ -437,7 +444,8 @@ __global__ __launch_bounds__(256) void fused_bwd_pairs_kernel(
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int nn = (lane >> 4) + 4 * r;
-        p[h2][r] = (n0 + nn < nend && bvalid && act[h2]) ? sf4 * exp(e[h2][r]) : 0.0;
+        // branch-free: padding entries have zero features, so exp(0) times a 0 weight
+        p[h2][r] = sf4 * exp(e[h2][r]) * ((n0 + nn < nend && bvalid && act[h2]) ? 1.0 : 0.0);
         L[h2][r] = g[h2][r] * p[h2][r];
       }
 #pragma unroll
