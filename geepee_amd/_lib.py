@@ -62,7 +62,7 @@ SIGNATURES = {
     'gpx_predict_grouped': (_i, [_i, _p, _p, _p, _p, _p, _i, _p]),
     'gpx_sym_pairmajor': (_i, [_i, _i, _p, _p, _p]),
     'gpx_adam_step': (_i, [_l, _p, _p, _p, _p, _d, _d, _d, _d, _i, _p, _p, _p]),
-    'gpx_fused_fwd_workspace': (_l, [_i, _i, _i]),
+    'gpx_fused_fwd_workspace': (_l, [_i, _i, _i, _i]),
     'gpx_fused_mm_fwd': (_i, [_i, _i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p, _p, _p]),
     'gpx_fused_bwd_workspace': (_l, [_i, _i, _i, _i, _i, _i]),
     'gpx_fused_mm_bwd': (_i, [_i, _i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p,

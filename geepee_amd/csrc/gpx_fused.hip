@@ -64,6 +64,39 @@ __device__ __forceinline__ double pair_feature(int k, int Q, const double *za, c
   return 0.0;
 }
 
+// pair_feature with the k = 2Q entry P_ab read from Pab (pair_p_kernel) instead of
+// summed over q: the hot loops of the forward and per-point passes evaluate features
+// per row block, and that sum was most of their feature work.
+__device__ __forceinline__ double pair_feature_p(int k, int Q, const double *za, const double *zb,
+                                                 const double *pab) {
+  if (k == 2 * Q) return *pab;
+  if (k < Q) return 0.5 * (za[k] + zb[k]);
+  if (k < 2 * Q) {
+    const double h = 0.5 * (za[k - Q] + zb[k - Q]);
+    return -h * h;
+  }
+  if (k == 2 * Q + 1) return 1.0;
+  return 0.0;
+}
+
+// P[a][b] = -sum_q (z_aq - z_bq)^2 / (4 l_q), the pair feature k = 2Q, same operations in
+// the same order as pair_feature.
+__global__ __launch_bounds__(256) void pair_p_kernel(int M, int Q, const double *__restrict__ z,
+                                                     const double *__restrict__ ls, double *__restrict__ P) {
+  __shared__ double il4[FQ_MAX];
+  if (threadIdx.x < Q) il4[threadIdx.x] = 0.25 / exp(2.0 * ls[threadIdx.x]);
+  __syncthreads();
+  const long e = blockIdx.x * 256L + threadIdx.x;
+  if (e >= (long)M * M) return;
+  const int a = (int)(e / M), b = (int)(e % M);
+  double p = 0.0;
+  for (int q = 0; q < Q; ++q) {
+    const double dz = z[a * Q + q] - z[b * Q + q];
+    p -= dz * dz * il4[q];
+  }
+  P[e] = p;
+}
+
 // Per-point constants: mu, d = 1/(2v+l), c = sum [log(l/(l+2v))/2 - d mu^2].
 __device__ __forceinline__ double point_consts(int n, int N, int Q, const double *mx, const double *vx,
                                                const double *lam, double *mu, double *d) {
@@ -92,7 +125,8 @@ __global__ __launch_bounds__(256) void fused_fwd_kernel(int N, int M, int Q, int
                                                         const double *__restrict__ z,
                                                         const double *__restrict__ ls,
                                                         const double *__restrict__ sf,
-                                                        const double *__restrict__ Bp, int S,
+                                                        const double *__restrict__ Bp,
+                                                        const double *__restrict__ Pab, int S,
                                                         double *__restrict__ part) {
   __shared__ double zs[FM_MAX][FQ_MAX + 1];
   __shared__ double lam[FQ_MAX], il4[FQ_MAX];
@@ -137,7 +171,8 @@ __global__ __launch_bounds__(256) void fused_fwd_kernel(int N, int M, int Q, int
       double af[FKS_MAX];
 #pragma unroll
       for (int s = 0; s < FKS_MAX; ++s)
-        af[s] = (s < KS && ja < M) ? pair_feature((lane >> 4) + 4 * s, Q, zs[a], zs[ja], il4) : 0.0;
+        af[s] = (s < KS && ja < M) ? pair_feature_p((lane >> 4) + 4 * s, Q, zs[a], zs[ja], Pab + (long)a * M + ja)
+                                   : 0.0;
       double bv[DB][4];
 #pragma unroll
       for (int db = 0; db < DB; ++db)
@@ -210,7 +245,7 @@ template <int DS>
 __global__ __launch_bounds__(256) void fused_bwd_points_kernel(
     int N, int M, int Q, int D, const double *__restrict__ mx, const double *__restrict__ vx,
     const double *__restrict__ z, const double *__restrict__ ls, const double *__restrict__ sf,
-    const double *__restrict__ Bp, const double *__restrict__ dv, int S,
+    const double *__restrict__ Bp, const double *__restrict__ dv, const double *__restrict__ Pab, int S,
     double *__restrict__ Tpart, double *__restrict__ Vpart) {
   __shared__ double zs[FM_MAX][FQ_MAX + 1];
   __shared__ double lam[FQ_MAX], il4[FQ_MAX];
@@ -272,7 +307,8 @@ __global__ __launch_bounds__(256) void fused_bwd_points_kernel(
 #pragma unroll
       for (int s = 0; s < FKS_MAX; ++s) {
         if (s < KS) {
-          const double af = (ja < M) ? pair_feature((lane >> 4) + 4 * s, Q, zs[a], zs[ja], il4) : 0.0;
+          const double af =
+              (ja < M) ? pair_feature_p((lane >> 4) + 4 * s, Q, zs[a], zs[ja], Pab + (long)a * M + ja) : 0.0;
           e = gpx_mfma16(af, bf[s], e);
         }
       }
@@ -696,6 +732,7 @@ long gpx_fused_bwd_workspace(int N, int M, int Q, int D, int splits, int nsplit)
   s += (long)N * (Q + 1) + 4L * N * Q + 4L * M * Q + 2;                       // Dn, Pn, En, TP, LE, cols0
   s += (long)N * PF_W;                                                        // point features
   s += gpx_psi_tail_workspace(N, M, Q);                                       // tail split-K partials
+  s += MM;                                                                    // pair feature P
   return s;
 }
 
@@ -705,7 +742,8 @@ namespace {
 
 // Workspace of the fused adjoint (gpx_fused_bwd_workspace doubles), carved in order.
 struct FusedBwdWork {
-  double *dBpart, *LWpart, *LW, *Tpart, *Vpart, *dpsi1, *Tn, *Lp, *H, *s0, *Dn, *Pn, *En, *TP, *LE, *cols0, *pf;
+  double *dBpart, *LWpart, *LW, *Tpart, *Vpart, *dpsi1, *Tn, *Lp, *H, *s0, *Dn, *Pn, *En, *TP, *LE, *cols0, *pf,
+      *Pab;
   FusedBwdWork(double *work, int N, int M, int Q, int D, int splits, int nsplit) {
     const long MM = (long)M * M;
     dBpart = work;
@@ -725,6 +763,7 @@ struct FusedBwdWork {
     LE = TP + 2L * M * Q;
     cols0 = LE + 2L * M * Q;
     pf = cols0 + 2;
+    Pab = pf + (long)N * PF_W + gpx_psi_tail_workspace(N, M, Q);  // after the tail's partials
   }
 };
 
@@ -758,6 +797,8 @@ int gpx_fused_mm_bwd_main(int N, int M, int Q, int D, const double *mx, const do
   FusedBwdWork W(work, N, M, Q, D, splits, nsplit);
   hipLaunchKernelGGL(fused_point_feat_kernel, dim3(gpx_cdiv(N, 256)), dim3(256), 0, st, N, Q, mx, vx, ls, W.pf);
   GPX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(pair_p_kernel, dim3(gpx_cdiv((long)M * M, 256)), dim3(256), 0, st, M, Q, z, ls, W.Pab);
+  GPX_CHECK_LAUNCH();
   // per-pair pass over point ranges
   const int nchunk = ((N + nsplit - 1) / nsplit + 15) / 16 * 16;
   dim3 gp(ntiles * 2, nsplit);
@@ -775,10 +816,10 @@ int gpx_fused_mm_bwd_main(int N, int M, int Q, int D, const double *mx, const do
   dim3 gq(gpx_cdiv(N, 16), splits);
   if (D <= 16)
     hipLaunchKernelGGL((fused_bwd_points_kernel<4>), gq, dim3(256), 0, st, N, M, Q, D, mx, vx, z, ls, sf, Bp, dv,
-                       splits, W.Tpart, W.Vpart);
+                       (const double *)W.Pab, splits, W.Tpart, W.Vpart);
   else
     hipLaunchKernelGGL((fused_bwd_points_kernel<8>), gq, dim3(256), 0, st, N, M, Q, D, mx, vx, z, ls, sf, Bp, dv,
-                       splits, W.Tpart, W.Vpart);
+                       (const double *)W.Pab, splits, W.Tpart, W.Vpart);
   GPX_CHECK_LAUNCH();
   // dpsi1 = dm_all Ahat
   int rc = gpx_gemm_batched(0, 0, N, M, D, 1.0, dm_all, D, 0, Ahat, M, 0, 0.0, W.dpsi1, M, 0, 1, 1, nullptr, 0,
@@ -827,7 +868,7 @@ int gpx_sym_pairmajor(int D, int M, const double *B, double *out, void *stream) 
   return GPX_OK;
 }
 
-long gpx_fused_fwd_workspace(int N, int D, int splits) { return (long)splits * N * D; }
+long gpx_fused_fwd_workspace(int N, int M, int D, int splits) { return (long)splits * N * D + (long)M * M; }
 
 // Moment-matched layer forward without materialising psi2:
 // psi1 (N x M), mout = psi1 Ahat^T, vout = sf2 + <Bsym[d], psi2_n> - mout^2 (N x D).
@@ -845,9 +886,12 @@ int gpx_fused_mm_fwd(int N, int M, int Q, int D, const double *mx, const double 
   hipStream_t st = (hipStream_t)stream;
   const int T = (M + 15) / 16, ntiles = T * (T + 1) / 2;
   if (splits > ntiles) return GPX_EINVAL;
+  double *Pab = work + (long)splits * N * D;
+  hipLaunchKernelGGL(pair_p_kernel, dim3(gpx_cdiv((long)M * M, 256)), dim3(256), 0, st, M, Q, z, ls, Pab);
+  GPX_CHECK_LAUNCH();
   dim3 grid(gpx_cdiv(N, 16 * nsub), splits);
 #define GPX_FWD(DBV, NS) hipLaunchKernelGGL((fused_fwd_kernel<DBV, NS>), grid, dim3(256), 0, st, N, M, Q, D, mx, \
-                                            vx, z, ls, sf, Bp, splits, work)
+                                            vx, z, ls, sf, Bp, (const double *)Pab, splits, work)
   if (D <= 16) {
     if (nsub == 1) GPX_FWD(1, 1); else if (nsub == 2) GPX_FWD(1, 2); else GPX_FWD(1, 4);
   } else {

@@ -287,7 +287,7 @@ def fused_mm_fwd(mx, vx, z, ls, sf, Ahat, Bp, plan=None):
     psi1 = empty(N, M, like=mx)
     mout = empty(N, D, like=mx)
     vout = empty(N, D, like=mx)
-    work = empty(query('gpx_fused_fwd_workspace', N, D, plan['splits_f']), like=mx)
+    work = empty(query('gpx_fused_fwd_workspace', N, M, D, plan['splits_f']), like=mx)
     call('gpx_fused_mm_fwd', N, M, Q, D, P(mx), P(vx), P(z), P(ls), P(sf), P(Ahat), P(Bp),
          plan['splits_f'], plan['nsub'], P(psi1), P(mout), P(vout), P(work), S())
     return psi1, mout, vout

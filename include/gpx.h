@@ -216,9 +216,10 @@ int gpx_sym_pairmajor(int D, int M, const double *B, double *out, void *stream);
 /* Forward of a Gaussian-input layer: psi1 (N x M), mout = psi1 Ahat^T and
  * vout = sf2 + <Bhat[d], psi2_n> - mout^2 with psi2 recomputed per 16x16 tile (FP64 MFMA for
  * the exponent GEMM and the Bhat contraction).  splits (grid.y over pair tiles) and nsub
- * (1, 2, 4: 16*nsub points per block) are tuning knobs.  Replaces compute_psi_weave followed by
+ * (1, 2, 4: 16*nsub points per block) are tuning knobs; `work` holds
+ * gpx_fused_fwd_workspace(N, M, D, splits) doubles.  Replaces compute_psi_weave followed by
  * the einsums of geepee/aep_models.py:193-198 (and base_models.py:298-304). */
-long gpx_fused_fwd_workspace(int N, int D, int splits);
+long gpx_fused_fwd_workspace(int N, int M, int D, int splits);
 int gpx_fused_mm_fwd(int N, int M, int Q, int D, const double *mx, const double *vx, const double *z,
                      const double *ls, const double *sf, const double *Ahat, const double *Bp,
                      int splits, int nsub, double *psi1, double *mout, double *vout, double *work,
