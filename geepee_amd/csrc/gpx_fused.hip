@@ -118,7 +118,9 @@ __device__ __forceinline__ double point_consts(int n, int N, int Q, const double
 // point tiles.  Bp is the symmetrised Bhat in pair-major layout (M x M x D) so the D
 // values of a pair are contiguous.
 // ---------------------------------------------------------------------------
-template <int DB, int NSUB>
+// KSC: the number of 4-wide feature slices, (2Q+2+3)/4, fixed at compile time for the
+// common shapes (0: read from Q at run time).
+template <int DB, int NSUB, int KSC>
 __global__ __launch_bounds__(256) void fused_fwd_kernel(int N, int M, int Q, int D,
                                                         const double *__restrict__ mx,
                                                         const double *__restrict__ vx,
@@ -140,8 +142,9 @@ __global__ __launch_bounds__(256) void fused_fwd_kernel(int N, int M, int Q, int
     il4[tid] = 0.25 / l;
   }
   __syncthreads();
-  const int KS = (2 * Q + 2 + 3) / 4;
-  double bf[NSUB][FKS_MAX];
+  constexpr int KSN = KSC ? KSC : FKS_MAX;
+  const int KS = KSC ? KSC : (2 * Q + 2 + 3) / 4;
+  double bf[NSUB][KSN];
   bool nvalid[NSUB];
 #pragma unroll
   for (int u = 0; u < NSUB; ++u) {
@@ -150,7 +153,7 @@ __global__ __launch_bounds__(256) void fused_fwd_kernel(int N, int M, int Q, int
     double mu[FQ_MAX], dq[FQ_MAX];
     const double c = point_consts(n, N, Q, mx, vx, lam, mu, dq);
 #pragma unroll
-    for (int s = 0; s < FKS_MAX; ++s)
+    for (int s = 0; s < KSN; ++s)
       bf[u][s] = (s < KS) ? point_feature((lane >> 4) + 4 * s, Q, mu, dq, c) : 0.0;
   }
   const double sf2 = exp(2.0 * sf[0]), sf4 = sf2 * sf2;
@@ -168,9 +171,9 @@ __global__ __launch_bounds__(256) void fused_fwd_kernel(int N, int M, int Q, int
       const int a = 16 * A + i;
       if (a >= M) break;
       const int ja = 16 * B + (lane & 15);
-      double af[FKS_MAX];
+      double af[KSN];
 #pragma unroll
-      for (int s = 0; s < FKS_MAX; ++s)
+      for (int s = 0; s < KSN; ++s)
         af[s] = (s < KS && ja < M) ? pair_feature_p((lane >> 4) + 4 * s, Q, zs[a], zs[ja], Pab + (long)a * M + ja)
                                    : 0.0;
       double bv[DB][4];
@@ -185,7 +188,7 @@ __global__ __launch_bounds__(256) void fused_fwd_kernel(int N, int M, int Q, int
       for (int u = 0; u < NSUB; ++u) {
         gpx_d4 e = gpx_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int s = 0; s < FKS_MAX; ++s)
+        for (int s = 0; s < KSN; ++s)
           if (s < KS) e = gpx_mfma16(af[s], bf[u][s], e);
         double p[4];
 #pragma unroll
@@ -193,9 +196,9 @@ __global__ __launch_bounds__(256) void fused_fwd_kernel(int N, int M, int Q, int
           const int b = 16 * B + (lane >> 4) + 4 * r;
           // exp on every lane and a 0/1 weight instead of a branch around it (padding
           // rows/columns have finite exponents: zero pair features, or the features of a
-          // point at 0 with zero variance); at NSUB = 4 the branch-free form needs more
-          // registers than two waves per SIMD leave
-          if constexpr (NSUB <= 2)
+          // point at 0 with zero variance); at NSUB = 4 with a run-time slice count the
+          // branch-free form needs more registers than two waves per SIMD leave
+          if constexpr (NSUB <= 2 || KSC != 0)
             p[r] = w * exp(e[r]) * ((b < M && nvalid[u]) ? 1.0 : 0.0);
           else
             p[r] = (b < M && nvalid[u]) ? w * exp(e[r]) : 0.0;
@@ -241,7 +244,7 @@ __global__ void sym_pairmajor_kernel(int D, int M, const double *__restrict__ B,
 // owned by wave (a % 16) % 4, so row sums need no barrier; a column tile's sums are
 // kept in registers over its A tiles and merged once, in wave order.
 // ---------------------------------------------------------------------------
-template <int DS>
+template <int DS, int KSC>
 __global__ __launch_bounds__(256) void fused_bwd_points_kernel(
     int N, int M, int Q, int D, const double *__restrict__ mx, const double *__restrict__ vx,
     const double *__restrict__ z, const double *__restrict__ ls, const double *__restrict__ sf,
@@ -263,13 +266,14 @@ __global__ __launch_bounds__(256) void fused_bwd_points_kernel(
   __syncthreads();
   const int n = n0 + (lane & 15);
   const bool nvalid = n < N;
-  const int KS = (2 * Q + 2 + 3) / 4;
-  double bf[FKS_MAX];
+  constexpr int KSN = KSC ? KSC : FKS_MAX;
+  const int KS = KSC ? KSC : (2 * Q + 2 + 3) / 4;
+  double bf[KSN];
   {
     double mu[FQ_MAX], dq[FQ_MAX];
     const double c = point_consts(n, N, Q, mx, vx, lam, mu, dq);
 #pragma unroll
-    for (int s = 0; s < FKS_MAX; ++s) bf[s] = (s < KS) ? point_feature((lane >> 4) + 4 * s, Q, mu, dq, c) : 0.0;
+    for (int s = 0; s < KSN; ++s) bf[s] = (s < KS) ? point_feature((lane >> 4) + 4 * s, Q, mu, dq, c) : 0.0;
   }
   // dv[n][d] for d = (lane>>4) + 4s  (B operand of the G contraction)
   double dvr[DS];
@@ -305,7 +309,7 @@ __global__ __launch_bounds__(256) void fused_bwd_points_kernel(
       }
       gpx_d4 e = gpx_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int s = 0; s < FKS_MAX; ++s) {
+      for (int s = 0; s < KSN; ++s) {
         if (s < KS) {
           const double af =
               (ja < M) ? pair_feature_p((lane >> 4) + 4 * s, Q, zs[a], zs[ja], Pab + (long)a * M + ja) : 0.0;
@@ -394,7 +398,7 @@ __global__ __launch_bounds__(256) void fused_point_feat_kernel(int N, int Q, con
 // features and dv straight from memory (no LDS, no barriers):
 //   dBhat[d, a, b] += sum_n dv_nd psi2_nab,  LW[a, b, q'] += sum_n [1 | d_nq'] L_nab
 // ---------------------------------------------------------------------------
-template <int DB, int DS>
+template <int DB, int DS, int KSC>
 __global__ __launch_bounds__(256) void fused_bwd_pairs_kernel(
     int N, int M, int Q, int D, const double *__restrict__ z, const double *__restrict__ ls,
     const double *__restrict__ sf, const double *__restrict__ Bp, const double *__restrict__ dv,
@@ -414,7 +418,8 @@ __global__ __launch_bounds__(256) void fused_bwd_pairs_kernel(
   if (!act[0]) return;                        // whole wave (a[1] > a[0]); no barriers below
   const int ns = blockIdx.y;
   const int nbeg = ns * nchunk, nend = min(N, nbeg + nchunk);
-  const int KS = (2 * Q + 2 + 3) / 4;
+  constexpr int KSN = KSC ? KSC : FKS_MAX;
+  const int KS = KSC ? KSC : (2 * Q + 2 + 3) / 4;
   const int jb = 16 * B + (lane & 15);       // this lane's pair column b
   const bool bvalid = jb < M;
   double il4[FQ_MAX], zb[FQ_MAX];
@@ -423,14 +428,14 @@ __global__ __launch_bounds__(256) void fused_bwd_pairs_kernel(
     il4[q] = (q < Q) ? 0.25 * exp(-2.0 * ls[q]) : 0.0;
     zb[q] = (q < Q && bvalid) ? z[jb * Q + q] : 0.0;
   }
-  double af[2][FKS_MAX], bs[2][DS];
+  double af[2][KSN], bs[2][DS];
 #pragma unroll
   for (int h2 = 0; h2 < 2; ++h2) {
     double za[FQ_MAX];
 #pragma unroll
     for (int q = 0; q < FQ_MAX; ++q) za[q] = (q < Q && act[h2]) ? z[a[h2] * Q + q] : 0.0;
 #pragma unroll
-    for (int s = 0; s < FKS_MAX; ++s)
+    for (int s = 0; s < KSN; ++s)
       af[h2][s] = (s < KS && bvalid && act[h2]) ? pair_feature((lane >> 4) + 4 * s, Q, za, zb, il4) : 0.0;
 #pragma unroll
     for (int s = 0; s < DS; ++s) {
@@ -449,9 +454,9 @@ __global__ __launch_bounds__(256) void fused_bwd_pairs_kernel(
   for (int n0 = nbeg; n0 < nend; n0 += 16) {
     const int nA = n0 + (lane & 15);         // point of this lane as an A-operand row
     const bool vA = nA < nend;
-    double pa[FKS_MAX], da[DS];
+    double pa[KSN], da[DS];
 #pragma unroll
-    for (int s = 0; s < FKS_MAX; ++s) pa[s] = (s < KS && vA) ? pf[(long)nA * PF_W + (lane >> 4) + 4 * s] : 0.0;
+    for (int s = 0; s < KSN; ++s) pa[s] = (s < KS && vA) ? pf[(long)nA * PF_W + (lane >> 4) + 4 * s] : 0.0;
 #pragma unroll
     for (int s = 0; s < DS; ++s) {
       const int d = (lane >> 4) + 4 * s;
@@ -464,7 +469,7 @@ __global__ __launch_bounds__(256) void fused_bwd_pairs_kernel(
       g[h2] = gpx_d4{0.0, 0.0, 0.0, 0.0};
     }
 #pragma unroll
-    for (int s = 0; s < FKS_MAX; ++s)
+    for (int s = 0; s < KSN; ++s)
       if (s < KS) {
         e[0] = gpx_mfma16(pa[s], af[0][s], e[0]);
         e[1] = gpx_mfma16(pa[s], af[1][s], e[1]);
@@ -802,24 +807,31 @@ int gpx_fused_mm_bwd_main(int N, int M, int Q, int D, const double *mx, const do
   // per-pair pass over point ranges
   const int nchunk = ((N + nsplit - 1) / nsplit + 15) / 16 * 16;
   dim3 gp(ntiles * 2, nsplit);
-  if (D <= 16)
-    hipLaunchKernelGGL((fused_bwd_pairs_kernel<1, 4>), gp, dim3(256), 0, st, N, M, Q, D, z, ls, sf, Bp, dv,
-                       (const double *)W.pf, nsplit, nchunk, W.dBpart, W.LWpart);
-  else
-    hipLaunchKernelGGL((fused_bwd_pairs_kernel<2, 8>), gp, dim3(256), 0, st, N, M, Q, D, z, ls, sf, Bp, dv,
-                       (const double *)W.pf, nsplit, nchunk, W.dBpart, W.LWpart);
+  const int ks = (2 * Q + 2 + 3) / 4;  // compile-time slice counts for Q = 4, 5 (3) and 7, 8 (5)
+#define GPX_PAIRS(DBV, DSV, KSV)                                                                               \
+  hipLaunchKernelGGL((fused_bwd_pairs_kernel<DBV, DSV, KSV>), gp, dim3(256), 0, st, N, M, Q, D, z, ls, sf, Bp, dv, \
+                     (const double *)W.pf, nsplit, nchunk, W.dBpart, W.LWpart)
+  if (D <= 16) {
+    if (ks == 3) GPX_PAIRS(1, 4, 3); else if (ks == 5) GPX_PAIRS(1, 4, 5); else GPX_PAIRS(1, 4, 0);
+  } else {
+    if (ks == 3) GPX_PAIRS(2, 8, 3); else if (ks == 5) GPX_PAIRS(2, 8, 5); else GPX_PAIRS(2, 8, 0);
+  }
+#undef GPX_PAIRS
   GPX_CHECK_LAUNCH();
   hipLaunchKernelGGL(fused_pairs_finish, dim3(ntiles, D + 1), dim3(256), 0, st, M, Q, D, nsplit,
                      (const double *)W.dBpart, (const double *)W.LWpart, dBhat, W.LW);
   GPX_CHECK_LAUNCH();
   // per-point pass over tile splits
   dim3 gq(gpx_cdiv(N, 16), splits);
-  if (D <= 16)
-    hipLaunchKernelGGL((fused_bwd_points_kernel<4>), gq, dim3(256), 0, st, N, M, Q, D, mx, vx, z, ls, sf, Bp, dv,
-                       (const double *)W.Pab, splits, W.Tpart, W.Vpart);
-  else
-    hipLaunchKernelGGL((fused_bwd_points_kernel<8>), gq, dim3(256), 0, st, N, M, Q, D, mx, vx, z, ls, sf, Bp, dv,
-                       (const double *)W.Pab, splits, W.Tpart, W.Vpart);
+#define GPX_POINTS(DSV, KSV)                                                                                    \
+  hipLaunchKernelGGL((fused_bwd_points_kernel<DSV, KSV>), gq, dim3(256), 0, st, N, M, Q, D, mx, vx, z, ls, sf, Bp, \
+                     dv, (const double *)W.Pab, splits, W.Tpart, W.Vpart)
+  if (D <= 16) {
+    if (ks == 3) GPX_POINTS(4, 3); else if (ks == 5) GPX_POINTS(4, 5); else GPX_POINTS(4, 0);
+  } else {
+    if (ks == 3) GPX_POINTS(8, 3); else if (ks == 5) GPX_POINTS(8, 5); else GPX_POINTS(8, 0);
+  }
+#undef GPX_POINTS
   GPX_CHECK_LAUNCH();
   // dpsi1 = dm_all Ahat
   int rc = gpx_gemm_batched(0, 0, N, M, D, 1.0, dm_all, D, 0, Ahat, M, 0, 0.0, W.dpsi1, M, 0, 1, 1, nullptr, 0,
@@ -890,14 +902,26 @@ int gpx_fused_mm_fwd(int N, int M, int Q, int D, const double *mx, const double 
   hipLaunchKernelGGL(pair_p_kernel, dim3(gpx_cdiv((long)M * M, 256)), dim3(256), 0, st, M, Q, z, ls, Pab);
   GPX_CHECK_LAUNCH();
   dim3 grid(gpx_cdiv(N, 16 * nsub), splits);
-#define GPX_FWD(DBV, NS) hipLaunchKernelGGL((fused_fwd_kernel<DBV, NS>), grid, dim3(256), 0, st, N, M, Q, D, mx, \
-                                            vx, z, ls, sf, Bp, (const double *)Pab, splits, work)
+  const int ks = (2 * Q + 2 + 3) / 4;  // compile-time slice counts for Q = 4, 5 (3) and 7, 8 (5)
+#define GPX_FWD1(DBV, NS, KSV)                                                                                 \
+  hipLaunchKernelGGL((fused_fwd_kernel<DBV, NS, KSV>), grid, dim3(256), 0, st, N, M, Q, D, mx, vx, z, ls, sf, Bp, \
+                     (const double *)Pab, splits, work)
+#define GPX_FWD(DBV, NS)                    \
+  do {                                      \
+    if (ks == 3)                            \
+      GPX_FWD1(DBV, NS, 3);                 \
+    else if (ks == 5)                       \
+      GPX_FWD1(DBV, NS, 5);                 \
+    else                                    \
+      GPX_FWD1(DBV, NS, 0);                 \
+  } while (0)
   if (D <= 16) {
     if (nsub == 1) GPX_FWD(1, 1); else if (nsub == 2) GPX_FWD(1, 2); else GPX_FWD(1, 4);
   } else {
     if (nsub == 1) GPX_FWD(2, 1); else if (nsub == 2) GPX_FWD(2, 2); else GPX_FWD(2, 4);
   }
 #undef GPX_FWD
+#undef GPX_FWD1
   GPX_CHECK_LAUNCH();
   hipLaunchKernelGGL(fused_fwd_finish, dim3(gpx_cdiv(N, 16)), dim3(256), 0, st, N, M, Q, D, mx, vx, z, ls, sf,
                      Ahat, (const double *)work, splits, psi1, mout, vout);
