@@ -577,7 +577,8 @@ __global__ __launch_bounds__(256) void fused_pairs_finish(int M, int Q, int D, i
 // then the same per-point formulas (and outputs) as the materialising path's row pass.
 // One wave per point: its lanes stride over the M columns and the sums are wave
 // reductions, so no workgroup barrier is needed.
-constexpr int FRED = 4 * FQ_MAX + 2;
+// QC: Q fixed at compile time for the common shapes (0: read at run time).
+template <int QC>
 __global__ __launch_bounds__(256) void fused_rows_finish(
     int N, int M, int Q, int S, const double *__restrict__ mx, const double *__restrict__ vx,
     const double *__restrict__ z, const double *__restrict__ ls, const double *__restrict__ Tpart,
@@ -585,11 +586,13 @@ __global__ __launch_bounds__(256) void fused_rows_finish(
     double *__restrict__ dmx, double *__restrict__ dvx, double *__restrict__ Tout,
     double *__restrict__ Lpout, double *__restrict__ Hout, double *__restrict__ s0out,
     double *__restrict__ Dn, double *__restrict__ Pn, double *__restrict__ En) {
+  constexpr int QN = QC ? QC : FQ_MAX, FRED = 4 * QN + 2;
+  if (QC) Q = QC;
   const int lane = threadIdx.x & 63, n = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (n >= N) return;
-  double mu[FQ_MAX];
+  double mu[QN];
 #pragma unroll
-  for (int q = 0; q < FQ_MAX; ++q) mu[q] = (q < Q) ? mx[(long)n * Q + q] : 0.0;
+  for (int q = 0; q < QN; ++q) mu[q] = (q < Q) ? mx[(long)n * Q + q] : 0.0;
   // [0] S0, [1] sum Lp, [2+q] S1, [2+Q+q] sum T z^2, [2+2Q+q] B1, [2+3Q+q] C1
   double c[FRED];
 #pragma unroll
@@ -603,28 +606,28 @@ __global__ __launch_bounds__(256) void fused_rows_finish(
     c[0] += T;
     c[1] += Lp;
 #pragma unroll
-    for (int q = 0; q < FQ_MAX; ++q)
+    for (int q = 0; q < QN; ++q)
       if (q < Q) {
         const double zq = z[m * Q + q], zm = zq - mu[q];
         c[2 + q] += T * zq;
-        c[2 + FQ_MAX + q] += T * zq * zq;
-        c[2 + 2 * FQ_MAX + q] += Lp * zm;
-        c[2 + 3 * FQ_MAX + q] += Lp * zm * zm;
+        c[2 + QN + q] += T * zq * zq;
+        c[2 + 2 * QN + q] += Lp * zm;
+        c[2 + 3 * QN + q] += Lp * zm * zm;
       }
   }
 #pragma unroll
   for (int i = 0; i < FRED; ++i) {
-    if (i >= 2 && (i - 2) % FQ_MAX >= Q) continue;
+    if (i >= 2 && (i - 2) % QN >= Q) continue;
     c[i] = gpx_wave_sum(c[i]);
   }
   double out_S1 = 0.0, out_TZ2 = 0.0, out_B1 = 0.0, out_C1 = 0.0;
 #pragma unroll
-  for (int q = 0; q < FQ_MAX; ++q)
+  for (int q = 0; q < QN; ++q)
     if (q == lane) {
       out_S1 = c[2 + q];
-      out_TZ2 = c[2 + FQ_MAX + q];
-      out_B1 = c[2 + 2 * FQ_MAX + q];
-      out_C1 = c[2 + 3 * FQ_MAX + q];
+      out_TZ2 = c[2 + QN + q];
+      out_B1 = c[2 + 2 * QN + q];
+      out_C1 = c[2 + 3 * QN + q];
     }
   if (lane < Q) {
     const int q = lane;
@@ -837,9 +840,12 @@ int gpx_fused_mm_bwd_main(int N, int M, int Q, int D, const double *mx, const do
   int rc = gpx_gemm_batched(0, 0, N, M, D, 1.0, dm_all, D, 0, Ahat, M, 0, 0.0, W.dpsi1, M, 0, 1, 1, nullptr, 0,
                             stream);
   if (rc) return rc;
-  hipLaunchKernelGGL(fused_rows_finish, dim3(gpx_cdiv(N, 4)), dim3(256), 0, st, N, M, Q, splits, mx, vx, z, ls,
-                     (const double *)W.Tpart, (const double *)W.Vpart, (const double *)W.dpsi1, psi1, dmx, dvx,
-                     W.Tn, W.Lp, W.H, W.s0, W.Dn, W.Pn, W.En);
+#define GPX_ROWS(QV)                                                                                           \
+  hipLaunchKernelGGL((fused_rows_finish<QV>), dim3(gpx_cdiv(N, 4)), dim3(256), 0, st, N, M, Q, splits, mx, vx, z, ls, \
+                     (const double *)W.Tpart, (const double *)W.Vpart, (const double *)W.dpsi1, psi1, dmx, dvx,   \
+                     W.Tn, W.Lp, W.H, W.s0, W.Dn, W.Pn, W.En)
+  if (Q == 4) GPX_ROWS(4); else if (Q == 5) GPX_ROWS(5); else if (Q == 8) GPX_ROWS(8); else GPX_ROWS(0);
+#undef GPX_ROWS
   GPX_CHECK_LAUNCH();
   return GPX_OK;
 }
