@@ -216,6 +216,11 @@ class SGP_Layer(object):
         self.Splusmmhat = _plus_outer(self.Suhat, self.muhat, self.muhat)
         self.Bhat_det = _sandwich(Ki, self.Suhat, Ki, E=Ki, gamma=-1.0)
         self.Bhat_sto = _plus_outer(self.Bhat_det, self.Ahat, self.Ahat)
+        # the fused kernels' operand layout, formed with the rest of the q(u) set-up (off the
+        # forward pass's critical path in a deep GP)
+        self._Bp = None
+        if self.fused and dops.fused_fits(self.M, self.Din, self.Dout):
+            self._Bp = dops.sym_pairmajor(self.Bhat_sto)
 
     def compute_phi(self, alpha=1.0):
         """Device scalar (1,) of the weighted log-partitions (aep_models.py:97-114)."""
@@ -255,7 +260,8 @@ class SGP_Layer(object):
         N = mx.shape[0]
         M, D = self.M, self.Dout
         if self.fused and dops.fused_fits(M, mx.shape[1], D):
-            self._Bp = dops.sym_pairmajor(self.Bhat_sto)
+            if getattr(self, '_Bp', None) is None:
+                self._Bp = dops.sym_pairmajor(self.Bhat_sto)
             psi1, mout, vout = dops.fused_mm_fwd(mx, vx, self.zu, self.ls, self.sf, self.Ahat, self._Bp)
             return mout, vout, psi1, None
         psi1, psi2 = dops.psi_fwd(mx, vx, self.zu, self.ls, self.sf)
