@@ -10,22 +10,6 @@
 
 namespace {
 
-// Blocked Gauss-Jordan on the matrix cores, 16 x 16 pivot blocks.  Eliminating block
-// K with P = A[K,K] (rows of K: A[K,:], columns of K: c = A[:,K]):
-//   A[I,J] -= c_I P^-1 A[K,J]     (I, J != K)
-//   A[K,J] <- P^-1 A[K,J],  A[I,K] <- -c_I P^-1,  A[K,K] <- P^-1,
-// and after every block the matrix holds A^-1.  All MT x MT tiles are kept, in MFMA
-// accumulator layout, dealt round-robin to the NW waves (register r of lane l holds
-// tile entry [l/16 + 4r][l%16]): keeping only a symmetric half and mirroring it costs
-// two orders of magnitude in accuracy downstream on ill-conditioned input, because the
-// two halves of a computed inverse carry errors that cancel in later products.
-// Per block: the owners of block row and column K publish them to LDS; the panel waves
-// run Gauss-Jordan over the 16 scalar pivots on [P | I | A[K, other columns]], giving
-// P^-1 A[K,:] and P^-1 without ever multiplying by an explicit P^-1 (which loses an order
-// of magnitude on ill-conditioned blocks); every wave then updates its tiles with four
-// 16x16x4 MFMAs per tile.  The scalar pivots are the D of A = L D L^T: their logs sum to
-// log det A and the first non-positive one flags a matrix that is not positive definite.
-// The matrix is padded with the identity up to a multiple of 16.
 template <int MT>
 struct SpdLds {
   static constexpr int MP = 16 * MT, RS = MP + 1;  // RS: LDS row stride (doubles)
@@ -240,7 +224,7 @@ __global__ __launch_bounds__(NW * 64) void spd_gj_kernel(int M, int batch, const
 }
 
 template <int MT>
-int launch_sweep(int M, int batch, const double *A, int lda, long sA, double *out, int ldo, long sO,
+int launch_gj(int M, int batch, const double *A, int lda, long sA, double *out, int ldo, long sO,
                  double *logdet, int *info, hipStream_t st) {
   // 8 waves from M > 48 on (measured on MI355X against 16: equal at M = 200, faster
   // everywhere else)
@@ -251,11 +235,11 @@ int launch_sweep(int M, int batch, const double *A, int lda, long sA, double *ou
   return GPX_OK;
 }
 
-typedef int (*sweep_fn)(int, int, const double *, int, long, double *, int, long, double *, int *, hipStream_t);
-const sweep_fn kSweep[16] = {launch_sweep<1>,  launch_sweep<2>,  launch_sweep<3>,  launch_sweep<4>,
-                             launch_sweep<5>,  launch_sweep<6>,  launch_sweep<7>,  launch_sweep<8>,
-                             launch_sweep<9>,  launch_sweep<10>, launch_sweep<11>, launch_sweep<12>,
-                             launch_sweep<13>, launch_sweep<14>, launch_sweep<15>, launch_sweep<16>};
+typedef int (*gj_fn)(int, int, const double *, int, long, double *, int, long, double *, int *, hipStream_t);
+const gj_fn kGJ[16] = {launch_gj<1>,  launch_gj<2>,  launch_gj<3>,  launch_gj<4>,
+                             launch_gj<5>,  launch_gj<6>,  launch_gj<7>,  launch_gj<8>,
+                             launch_gj<9>,  launch_gj<10>, launch_gj<11>, launch_gj<12>,
+                             launch_gj<13>, launch_gj<14>, launch_gj<15>, launch_gj<16>};
 
 }  // namespace
 
@@ -272,7 +256,7 @@ int gpx_spd_inv_batched(int M, int batch, const double *A, int lda, long strideA
   if (M < 0 || batch < 0 || M > 256) return GPX_EINVAL;
   if (M == 0 || batch == 0) return GPX_OK;
   if (!A || lda < M || (out && ldo < M)) return GPX_EINVAL;
-  return kSweep[(M + 15) / 16 - 1](M, batch, A, lda, strideA, out, ldo, strideOut, logdet, info,
+  return kGJ[(M + 15) / 16 - 1](M, batch, A, lda, strideA, out, ldo, strideOut, logdet, info,
                                    (hipStream_t)stream);
 }
 

@@ -52,8 +52,9 @@ int gpx_gemm_batched_ex(int transA, int transB, int m, int n, int k, double alph
 
 /* Batched SPD inverse and log-determinant, one workgroup per matrix (M <= 256):
  * out[b] = inv(A[b]) (out may alias A or be NULL), logdet[b] = log det A[b],
- * info[b] = 0 or k+1 when pivot k is not positive (outputs NaN).  A[b] is taken to be
- * symmetric (only its upper triangle is read) and out[b] is exactly symmetric.
+ * info[b] = 0 or k+1 when pivot k is not positive (outputs NaN).  A[b] is read whole and
+ * taken to be symmetric; out[b] is symmetric to rounding (both halves are computed, since
+ * their rounding errors cancel in later products).
  * Replaces np.linalg.inv / slogdet / solve at geepee/base_models.py:462-476 and
  * geepee/aep_models.py:62-95, 523-533. */
 int gpx_spd_max_m(void);
