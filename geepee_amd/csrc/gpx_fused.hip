@@ -79,6 +79,35 @@ __device__ __forceinline__ double pair_feature_p(int k, int Q, const double *za,
   return 0.0;
 }
 
+// Branch-free pair feature for the hot loops.  Feature k = (lane>>4) + 4s is fixed per lane
+// and slice, so its kind is set up once (PairSel); each evaluation reads both z rows at a
+// clamped index and selects, so no lane's load sits behind a branch (a load under a
+// branch is followed by a wait on every outstanding load, prefetches included).
+struct PairSel {
+  int kq;        // z column read (k, k - Q, or 0 when unused)
+  bool lin, sq;  // k < Q: zh;  Q <= k < 2Q: -zh^2
+  bool pab;      // k == 2Q: P_ab
+  double cst;    // k == 2Q+1: 1, beyond: 0
+};
+
+__device__ __forceinline__ PairSel pair_sel(int k, int Q) {
+  PairSel p;
+  p.kq = k < Q ? k : (k < 2 * Q ? k - Q : 0);
+  p.lin = k < Q;
+  p.sq = k >= Q && k < 2 * Q;
+  p.pab = k == 2 * Q;
+  p.cst = (k == 2 * Q + 1) ? 1.0 : 0.0;
+  return p;
+}
+
+// Same value as pair_feature_p for the same (k, za, zb, P_ab).
+__device__ __forceinline__ double pair_feature_sel(const PairSel &p, const double *za, const double *zb,
+                                                   double pab) {
+  const double h = 0.5 * (za[p.kq] + zb[p.kq]);
+  const double f = p.lin ? h : -h * h;
+  return (p.lin || p.sq) ? f : (p.pab ? pab : p.cst);
+}
+
 // P[a][b] = -sum_q (z_aq - z_bq)^2 / (4 l_q), the pair feature k = 2Q, same operations in
 // the same order as pair_feature.
 __global__ __launch_bounds__(256) void pair_p_kernel(int M, int Q, const double *__restrict__ z,
@@ -156,6 +185,9 @@ __global__ __launch_bounds__(256) void fused_fwd_kernel(int N, int M, int Q, int
     for (int s = 0; s < KSN; ++s)
       bf[u][s] = (s < KS) ? point_feature((lane >> 4) + 4 * s, Q, mu, dq, c) : 0.0;
   }
+  PairSel ps[KSN];
+#pragma unroll
+  for (int s = 0; s < KSN; ++s) ps[s] = pair_sel((lane >> 4) + 4 * s, Q);
   const double sf2 = exp(2.0 * sf[0]), sf4 = sf2 * sf2;
   gpx_d4 acc[NSUB][DB];
 #pragma unroll
@@ -167,23 +199,26 @@ __global__ __launch_bounds__(256) void fused_fwd_kernel(int N, int M, int Q, int
     int A, B;
     decode_tile(t, T, A, B);
     const double w = (A == B) ? sf4 : 2.0 * sf4;
+    // column of this lane, clamped for the loads: a padding column's exponent is that of
+    // a real pair (finite) and its weight is 0 below
+    const int jl = min(16 * B + (lane & 15), M - 1);
     for (int i = wave; i < 16; i += 4) {
       const int a = 16 * A + i;
       if (a >= M) break;
-      const int ja = 16 * B + (lane & 15);
-      double af[KSN];
-#pragma unroll
-      for (int s = 0; s < KSN; ++s)
-        af[s] = (s < KS && ja < M) ? pair_feature_p((lane >> 4) + 4 * s, Q, zs[a], zs[ja], Pab + (long)a * M + ja)
-                                   : 0.0;
+      // unconditional loads (column b and row d clamped; rows d >= D of acc are never
+      // stored), issued together and waited for once
+      const double pab = Pab[(long)a * M + jl];
       double bv[DB][4];
 #pragma unroll
       for (int db = 0; db < DB; ++db)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int d = 16 * db + (lane & 15), b = 16 * B + (lane >> 4) + 4 * r;
-          bv[db][r] = (d < D && b < M) ? Bp[((long)a * M + b) * D + d] : 0.0;
+          const int d = min(16 * db + (lane & 15), D - 1), b = min(16 * B + (lane >> 4) + 4 * r, M - 1);
+          bv[db][r] = Bp[((long)a * M + b) * D + d];
         }
+      double af[KSN];
+#pragma unroll
+      for (int s = 0; s < KSN; ++s) af[s] = (s < KS) ? pair_feature_sel(ps[s], zs[a], zs[jl], pab) : 0.0;
 #pragma unroll
       for (int u = 0; u < NSUB; ++u) {
         gpx_d4 e = gpx_d4{0.0, 0.0, 0.0, 0.0};
@@ -250,72 +285,81 @@ __global__ __launch_bounds__(256) void fused_bwd_points_kernel(
     const double *__restrict__ z, const double *__restrict__ ls, const double *__restrict__ sf,
     const double *__restrict__ Bp, const double *__restrict__ dv, const double *__restrict__ Pab, int S,
     double *__restrict__ Tpart, double *__restrict__ Vpart) {
-  __shared__ double zs[FM_MAX][FQ_MAX + 1];
-  __shared__ double lam[FQ_MAX], il4[FQ_MAX];
-  __shared__ double Tacc[16][FM_MAX + 1];
+  // dynamic LDS, sized by the shape (points_lds_bytes): z rows (M x (Q+1)) and the row
+  // sums T (16 x (M+1)); at M = 128, Q = 8 that leaves room for three blocks per CU
+  extern __shared__ double dyn[];
+  const int ZS = Q + 1, TS = M + 1;
+  double *zs = dyn;
+  double *Tacc = dyn + M * ZS;
+  __shared__ double lam[FQ_MAX];
   __shared__ double cst[4][16][17];  // column-sum staging; reused for the V reduction
   const int n0 = blockIdx.x * 16, split = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int e = tid; e < M * Q; e += 256) zs[e / Q][e % Q] = z[e];
-  for (int e = tid; e < 16 * (FM_MAX + 1); e += 256) Tacc[e / (FM_MAX + 1)][e % (FM_MAX + 1)] = 0.0;
-  if (tid < Q) {
-    const double l = exp(2.0 * ls[tid]);
-    lam[tid] = l;
-    il4[tid] = 0.25 / l;
-  }
+  for (int e = tid; e < M * Q; e += 256) zs[(e / Q) * ZS + e % Q] = z[e];
+  for (int e = tid; e < 16 * TS; e += 256) Tacc[e] = 0.0;
+  if (tid < Q) lam[tid] = exp(2.0 * ls[tid]);
   __syncthreads();
   const int n = n0 + (lane & 15);
   const bool nvalid = n < N;
   constexpr int KSN = KSC ? KSC : FKS_MAX;
   const int KS = KSC ? KSC : (2 * Q + 2 + 3) / 4;
   double bf[KSN];
+  PairSel ps[KSN];
   {
     double mu[FQ_MAX], dq[FQ_MAX];
     const double c = point_consts(n, N, Q, mx, vx, lam, mu, dq);
 #pragma unroll
-    for (int s = 0; s < KSN; ++s) bf[s] = (s < KS) ? point_feature((lane >> 4) + 4 * s, Q, mu, dq, c) : 0.0;
+    for (int s = 0; s < KSN; ++s) {
+      bf[s] = (s < KS) ? point_feature((lane >> 4) + 4 * s, Q, mu, dq, c) : 0.0;
+      ps[s] = pair_sel((lane >> 4) + 4 * s, Q);
+    }
   }
-  // dv[n][d] for d = (lane>>4) + 4s  (B operand of the G contraction)
+  // dv[n][d] for d = (lane>>4) + 4s  (B operand of the G contraction); B-hat is read at
+  // d clamped to D - 1, which these zeros cancel
   double dvr[DS];
+  int dcl[DS];
 #pragma unroll
   for (int s = 0; s < DS; ++s) {
     const int d = (lane >> 4) + 4 * s;
     dvr[s] = (nvalid && d < D) ? dv[(long)n * D + d] : 0.0;
+    dcl[s] = min(d, D - 1);
   }
+  // V operand column: q clamped to Q - 1 (rows q >= Q of V are never stored)
+  const int qcl = min(lane & 15, Q - 1);
   const double sf2 = exp(2.0 * sf[0]), sf4 = sf2 * sf2;
   gpx_d4 vacc = gpx_d4{0.0, 0.0, 0.0, 0.0};
   const int T = (M + 15) / 16;
   for (int B = split; B < T; B += S) {
     double colp[4] = {0.0, 0.0, 0.0, 0.0};
     const int ja = 16 * B + (lane & 15);
+    // loads use the column clamped to M - 1: a padding column's exponent is then that of
+    // a real pair (finite), and the 0 weight below removes it
+    const int jl = min(ja, M - 1);
+    const double *zb = zs + jl * ZS;
     // row blocks (A, i) of this column tile for this wave, flattened as it = 16 A + i with
-    // it = wave, wave + 4, ...; B-hat of the next row block is fetched during this one
+    // it = wave, wave + 4, ...; B-hat and P_ab of the next row block are fetched during
+    // this one (at a row clamped to nit - 1, so every load is in bounds and unconditional)
     const int nit = min((B + 1) * 16, M);
-    double bcur[DS], bnxt[DS];
+    double bcur[DS], bnxt[DS], pcur, pnxt;
+    {
+      const int it0 = min(wave, nit - 1);
 #pragma unroll
-    for (int s = 0; s < DS; ++s) {
-      const int d = (lane >> 4) + 4 * s;
-      bcur[s] = (wave < nit && ja < M && d < D) ? Bp[((long)wave * M + ja) * D + d] : 0.0;
+      for (int s = 0; s < DS; ++s) bcur[s] = Bp[((long)it0 * M + jl) * D + dcl[s]];
+      pcur = Pab[(long)it0 * M + jl];
     }
     for (int it = wave; it < nit; it += 4) {
       const int A = it >> 4, a = it;
       const bool off = (A != B);
       const double w = off ? 2.0 : 1.0;
-      const int itn = it + 4;
+      const int itn = min(it + 4, nit - 1);
 #pragma unroll
-      for (int s = 0; s < DS; ++s) {
-        const int d = (lane >> 4) + 4 * s;
-        bnxt[s] = (itn < nit && ja < M && d < D) ? Bp[((long)itn * M + ja) * D + d] : 0.0;
-      }
+      for (int s = 0; s < DS; ++s) bnxt[s] = Bp[((long)itn * M + jl) * D + dcl[s]];
+      pnxt = Pab[(long)itn * M + jl];
+      const double *za = zs + a * ZS;
       gpx_d4 e = gpx_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int s = 0; s < KSN; ++s) {
-        if (s < KS) {
-          const double af =
-              (ja < M) ? pair_feature_p((lane >> 4) + 4 * s, Q, zs[a], zs[ja], Pab + (long)a * M + ja) : 0.0;
-          e = gpx_mfma16(af, bf[s], e);
-        }
-      }
+      for (int s = 0; s < KSN; ++s)
+        if (s < KS) e = gpx_mfma16(pair_feature_sel(ps[s], za, zb, pcur), bf[s], e);
       gpx_d4 g = gpx_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int s = 0; s < DS; ++s) g = gpx_mfma16(bcur[s], dvr[s], g);
@@ -331,16 +375,17 @@ __global__ __launch_bounds__(256) void fused_bwd_points_kernel(
       }
       rs += __shfl_xor(rs, 16, 64);
       rs += __shfl_xor(rs, 32, 64);
-      if ((lane >> 4) == 0) Tacc[lane & 15][a] += rs;
-      // V[q x n] += w z_aq z_bq L[pair x n]
+      if ((lane >> 4) == 0) Tacc[(lane & 15) * TS + a] += rs;
+      // V[q x n] += w z_aq z_bq L[pair x n]  (padding pairs have L = 0)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int q = lane & 15, b = 16 * B + (lane >> 4) + 4 * r;
-        const double zz = (q < Q && b < M) ? w * zs[a][q] * zs[b][q] : 0.0;
+        const int b = min(16 * B + (lane >> 4) + 4 * r, M - 1);
+        const double zz = w * za[qcl] * zs[b * ZS + qcl];
         vacc = gpx_mfma16(zz, L[r], vacc);
       }
 #pragma unroll
       for (int s = 0; s < DS; ++s) bcur[s] = bnxt[s];
+      pcur = pnxt;
     }
     if (B > 0) {
       // merge the column sums of tile column B: stage per wave, then one thread per
@@ -350,7 +395,7 @@ __global__ __launch_bounds__(256) void fused_bwd_points_kernel(
       __syncthreads();
       {
         const int nn = tid >> 4, bb = tid & 15, b = 16 * B + bb;
-        if (b < M) Tacc[nn][b] += cst[0][nn][bb] + cst[1][nn][bb] + cst[2][nn][bb] + cst[3][nn][bb];
+        if (b < M) Tacc[nn * TS + b] += cst[0][nn][bb] + cst[1][nn][bb] + cst[2][nn][bb] + cst[3][nn][bb];
       }
       __syncthreads();
     }
@@ -367,9 +412,12 @@ __global__ __launch_bounds__(256) void fused_bwd_points_kernel(
   }
   for (int e = tid; e < 16 * M; e += 256) {
     const int nn = e / M, a = e % M;
-    if (n0 + nn < N) Tpart[((long)split * N + n0 + nn) * M + a] = Tacc[nn][a];
+    if (n0 + nn < N) Tpart[((long)split * N + n0 + nn) * M + a] = Tacc[nn * TS + a];
   }
 }
+
+// Dynamic LDS of fused_bwd_points_kernel.
+inline size_t points_lds_bytes(int M, int Q) { return sizeof(double) * ((size_t)M * (Q + 1) + 16 * (size_t)(M + 1)); }
 
 // Per-point features of the per-pair pass, formed once per launch: for each point n,
 // pf[n][k] (k < 4*FKS_MAX, the exponent operand) and dq[n][j] = [1 | d_n | 0...] (16).
@@ -452,16 +500,16 @@ __global__ __launch_bounds__(256) void fused_bwd_pairs_kernel(
     for (int db = 0; db < DB; ++db) dacc[h2][db] = gpx_d4{0.0, 0.0, 0.0, 0.0};
   }
   for (int n0 = nbeg; n0 < nend; n0 += 16) {
-    const int nA = n0 + (lane & 15);         // point of this lane as an A-operand row
-    const bool vA = nA < nend;
+    // Every point operand is loaded unconditionally, at a point index clamped to nend - 1
+    // and a column clamped into range, so the loads issue together and are waited for
+    // once.  Padding points get weight 0 through p; rows d >= D of dacc and q' > Q of lacc
+    // are never stored.
+    const int nA = min(n0 + (lane & 15), nend - 1);  // point of this lane as an A-operand row
     double pa[KSN], da[DS];
 #pragma unroll
-    for (int s = 0; s < KSN; ++s) pa[s] = (s < KS && vA) ? pf[(long)nA * PF_W + (lane >> 4) + 4 * s] : 0.0;
+    for (int s = 0; s < KSN; ++s) pa[s] = pf[(long)nA * PF_W + (lane >> 4) + 4 * s];
 #pragma unroll
-    for (int s = 0; s < DS; ++s) {
-      const int d = (lane >> 4) + 4 * s;
-      da[s] = (vA && d < D) ? dv[(long)nA * D + d] : 0.0;
-    }
+    for (int s = 0; s < DS; ++s) da[s] = dv[(long)nA * D + min((lane >> 4) + 4 * s, D - 1)];
     gpx_d4 e[2], g[2];
 #pragma unroll
     for (int h2 = 0; h2 < 2; ++h2) {
@@ -479,30 +527,35 @@ __global__ __launch_bounds__(256) void fused_bwd_pairs_kernel(
       g[0] = gpx_mfma16(da[s], bs[0][s], g[0]);
       g[1] = gpx_mfma16(da[s], bs[1][s], g[1]);
     }
+    // operands of the contractions over the points, loaded while the exps run
+    double dvv[4][DB], dq[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int nr = min(n0 + (lane >> 4) + 4 * r, nend - 1);  // point of this k-slice
+#pragma unroll
+      for (int db = 0; db < DB; ++db) dvv[r][db] = dv[(long)nr * D + min(16 * db + (lane & 15), D - 1)];
+      dq[r] = pf[(long)nr * PF_W + 4 * FKS_MAX + (lane & 15)];
+    }
     double p[2][4], L[2][4];
 #pragma unroll
     for (int h2 = 0; h2 < 2; ++h2)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int nn = (lane >> 4) + 4 * r;
-        // branch-free: padding entries have zero features, so exp(0) times a 0 weight
+        // branch-free: padding entries have finite exponents (zero pair features, or a
+        // real point's features), times a 0 weight
         p[h2][r] = sf4 * exp(e[h2][r]) * ((n0 + nn < nend && bvalid && act[h2]) ? 1.0 : 0.0);
         L[h2][r] = g[h2][r] * p[h2][r];
       }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int nr = n0 + (lane >> 4) + 4 * r;  // point of this k-slice
-      const bool vr = nr < nend;
 #pragma unroll
       for (int db = 0; db < DB; ++db) {
-        const int d = 16 * db + (lane & 15);
-        const double dvv = (vr && d < D) ? dv[(long)nr * D + d] : 0.0;
-        dacc[0][db] = gpx_mfma16(dvv, p[0][r], dacc[0][db]);
-        dacc[1][db] = gpx_mfma16(dvv, p[1][r], dacc[1][db]);
+        dacc[0][db] = gpx_mfma16(dvv[r][db], p[0][r], dacc[0][db]);
+        dacc[1][db] = gpx_mfma16(dvv[r][db], p[1][r], dacc[1][db]);
       }
-      const double dq = vr ? pf[(long)nr * PF_W + 4 * FKS_MAX + (lane & 15)] : 0.0;
-      lacc[0] = gpx_mfma16(dq, L[0][r], lacc[0]);
-      lacc[1] = gpx_mfma16(dq, L[1][r], lacc[1]);
+      lacc[0] = gpx_mfma16(dq[r], L[0][r], lacc[0]);
+      lacc[1] = gpx_mfma16(dq[r], L[1][r], lacc[1]);
     }
   }
 #pragma unroll
@@ -827,7 +880,8 @@ int gpx_fused_mm_bwd_main(int N, int M, int Q, int D, const double *mx, const do
   // per-point pass over tile splits
   dim3 gq(gpx_cdiv(N, 16), splits);
 #define GPX_POINTS(DSV, KSV)                                                                                    \
-  hipLaunchKernelGGL((fused_bwd_points_kernel<DSV, KSV>), gq, dim3(256), 0, st, N, M, Q, D, mx, vx, z, ls, sf, Bp, \
+  hipLaunchKernelGGL((fused_bwd_points_kernel<DSV, KSV>), gq, dim3(256), points_lds_bytes(M, Q), st, N, M, Q, D, mx, \
+                     vx, z, ls, sf, Bp, \
                      dv, (const double *)W.Pab, splits, W.Tpart, W.Vpart)
   if (D <= 16) {
     if (ks == 3) GPX_POINTS(4, 3); else if (ks == 5) GPX_POINTS(4, 5); else GPX_POINTS(4, 0);
