@@ -22,6 +22,15 @@
 #define FQ_MAX 16
 #define FM_MAX 256
 #define FKS_MAX 9  // ceil((2*FQ_MAX+2)/4)
+#ifndef PAIRS_NR
+#define PAIRS_NR 1
+#endif
+#ifndef POINTS_PREFETCH
+#define POINTS_PREFETCH 1
+#endif
+#ifndef POINTS_DFULL
+#define POINTS_DFULL 1
+#endif
 
 namespace {
 
@@ -87,7 +96,7 @@ struct PairSel {
   int kq;        // z column read (k, k - Q, or 0 when unused)
   bool lin, sq;  // k < Q: zh;  Q <= k < 2Q: -zh^2
   bool pab;      // k == 2Q: P_ab
-  double cst;    // k == 2Q+1: 1, beyond: 0
+  bool one;      // k == 2Q+1: 1, beyond: 0
 };
 
 __device__ __forceinline__ PairSel pair_sel(int k, int Q) {
@@ -96,7 +105,7 @@ __device__ __forceinline__ PairSel pair_sel(int k, int Q) {
   p.lin = k < Q;
   p.sq = k >= Q && k < 2 * Q;
   p.pab = k == 2 * Q;
-  p.cst = (k == 2 * Q + 1) ? 1.0 : 0.0;
+  p.one = k == 2 * Q + 1;
   return p;
 }
 
@@ -105,7 +114,7 @@ __device__ __forceinline__ double pair_feature_sel(const PairSel &p, const doubl
                                                    double pab) {
   const double h = 0.5 * (za[p.kq] + zb[p.kq]);
   const double f = p.lin ? h : -h * h;
-  return (p.lin || p.sq) ? f : (p.pab ? pab : p.cst);
+  return (p.lin || p.sq) ? f : (p.pab ? pab : (p.one ? 1.0 : 0.0));
 }
 
 // P[a][b] = -sum_q (z_aq - z_bq)^2 / (4 l_q), the pair feature k = 2Q, same operations in
@@ -279,7 +288,9 @@ __global__ void sym_pairmajor_kernel(int D, int M, const double *__restrict__ B,
 // owned by wave (a % 16) % 4, so row sums need no barrier; a column tile's sums are
 // kept in registers over its A tiles and merged once, in wave order.
 // ---------------------------------------------------------------------------
-template <int DS, int KSC>
+// DFULL: D == 4 DS, so every slice d = (lane>>4) + 4s is a real column and the B-hat loads
+// need no clamp (one address per row block, immediate offsets per slice).
+template <int DS, int KSC, bool DFULL>
 __global__ __launch_bounds__(256) void fused_bwd_points_kernel(
     int N, int M, int Q, int D, const double *__restrict__ mx, const double *__restrict__ vx,
     const double *__restrict__ z, const double *__restrict__ ls, const double *__restrict__ sf,
@@ -322,7 +333,7 @@ __global__ __launch_bounds__(256) void fused_bwd_points_kernel(
   for (int s = 0; s < DS; ++s) {
     const int d = (lane >> 4) + 4 * s;
     dvr[s] = (nvalid && d < D) ? dv[(long)n * D + d] : 0.0;
-    dcl[s] = min(d, D - 1);
+    dcl[s] = DFULL ? d : min(d, D - 1);
   }
   // V operand column: q clamped to Q - 1 (rows q >= Q of V are never stored)
   const int qcl = min(lane & 15, Q - 1);
@@ -351,10 +362,16 @@ __global__ __launch_bounds__(256) void fused_bwd_points_kernel(
       const int A = it >> 4, a = it;
       const bool off = (A != B);
       const double w = off ? 2.0 : 1.0;
+#if POINTS_PREFETCH
       const int itn = min(it + 4, nit - 1);
 #pragma unroll
       for (int s = 0; s < DS; ++s) bnxt[s] = Bp[((long)itn * M + jl) * D + dcl[s]];
       pnxt = Pab[(long)itn * M + jl];
+#else
+#pragma unroll
+      for (int s = 0; s < DS; ++s) bcur[s] = Bp[((long)it * M + jl) * D + dcl[s]];
+      pcur = Pab[(long)it * M + jl];
+#endif
       const double *za = zs + a * ZS;
       gpx_d4 e = gpx_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -383,9 +400,11 @@ __global__ __launch_bounds__(256) void fused_bwd_points_kernel(
         const double zz = w * za[qcl] * zs[b * ZS + qcl];
         vacc = gpx_mfma16(zz, L[r], vacc);
       }
+#if POINTS_PREFETCH
 #pragma unroll
       for (int s = 0; s < DS; ++s) bcur[s] = bnxt[s];
       pcur = pnxt;
+#endif
     }
     if (B > 0) {
       // merge the column sums of tile column B: stage per wave, then one thread per
@@ -446,24 +465,30 @@ __global__ __launch_bounds__(256) void fused_point_feat_kernel(int N, int Q, con
 // features and dv straight from memory (no LDS, no barriers):
 //   dBhat[d, a, b] += sum_n dv_nd psi2_nab,  LW[a, b, q'] += sum_n [1 | d_nq'] L_nab
 // ---------------------------------------------------------------------------
-template <int DB, int DS, int KSC>
+template <int DB, int DS, int KSC, int NR>
 __global__ __launch_bounds__(256) void fused_bwd_pairs_kernel(
     int N, int M, int Q, int D, const double *__restrict__ z, const double *__restrict__ ls,
     const double *__restrict__ sf, const double *__restrict__ Bp, const double *__restrict__ dv,
     const double *__restrict__ pf, int nsplit, int nchunk, double *__restrict__ dBpart,
     double *__restrict__ LWpart) {
-  // A block holds 8 row blocks of one tile; wave w owns rows i = h*8 + w and i + 4, two
-  // independent MFMA chains that share every point operand it loads.
+  // A block holds 4 NR row blocks of one tile (16 / (4 NR) blocks per tile); wave w owns
+  // rows i = 4 NR h + w + 4 u (u < NR): NR independent MFMA chains that share every point
+  // operand the wave loads (NR = 2 halves the point loads, NR = 1 halves the registers).
+  static_assert(NR == 1 || NR == 2, "rows per wave");
+  constexpr int BPT = 4 / NR;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int T = (M + 15) / 16;
-  const int t = blockIdx.x >> 1, h = blockIdx.x & 1;
+  const int t = blockIdx.x / BPT, h = blockIdx.x % BPT;
   int A, B;
   decode_tile(t, T, A, B);
-  int a[2];
-  a[0] = 16 * A + h * 8 + wave;
-  a[1] = a[0] + 4;
-  const bool act[2] = {a[0] < M, a[1] < M};
-  if (!act[0]) return;                        // whole wave (a[1] > a[0]); no barriers below
+  int a[NR];
+  bool act[NR];
+#pragma unroll
+  for (int u = 0; u < NR; ++u) {
+    a[u] = 16 * A + 4 * NR * h + wave + 4 * u;
+    act[u] = a[u] < M;
+  }
+  if (!act[0]) return;                        // whole wave (a[u] > a[0]); no barriers below
   const int ns = blockIdx.y;
   const int nbeg = ns * nchunk, nend = min(N, nbeg + nchunk);
   constexpr int KSN = KSC ? KSC : FKS_MAX;
@@ -476,9 +501,9 @@ __global__ __launch_bounds__(256) void fused_bwd_pairs_kernel(
     il4[q] = (q < Q) ? 0.25 * exp(-2.0 * ls[q]) : 0.0;
     zb[q] = (q < Q && bvalid) ? z[jb * Q + q] : 0.0;
   }
-  double af[2][KSN], bs[2][DS];
+  double af[NR][KSN], bs[NR][DS];
 #pragma unroll
-  for (int h2 = 0; h2 < 2; ++h2) {
+  for (int h2 = 0; h2 < NR; ++h2) {
     double za[FQ_MAX];
 #pragma unroll
     for (int q = 0; q < FQ_MAX; ++q) za[q] = (q < Q && act[h2]) ? z[a[h2] * Q + q] : 0.0;
@@ -492,9 +517,9 @@ __global__ __launch_bounds__(256) void fused_bwd_pairs_kernel(
     }
   }
   const double sf2 = exp(2.0 * sf[0]), sf4 = sf2 * sf2;
-  gpx_d4 dacc[2][DB], lacc[2];
+  gpx_d4 dacc[NR][DB], lacc[NR];
 #pragma unroll
-  for (int h2 = 0; h2 < 2; ++h2) {
+  for (int h2 = 0; h2 < NR; ++h2) {
     lacc[h2] = gpx_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int db = 0; db < DB; ++db) dacc[h2][db] = gpx_d4{0.0, 0.0, 0.0, 0.0};
@@ -510,22 +535,22 @@ __global__ __launch_bounds__(256) void fused_bwd_pairs_kernel(
     for (int s = 0; s < KSN; ++s) pa[s] = pf[(long)nA * PF_W + (lane >> 4) + 4 * s];
 #pragma unroll
     for (int s = 0; s < DS; ++s) da[s] = dv[(long)nA * D + min((lane >> 4) + 4 * s, D - 1)];
-    gpx_d4 e[2], g[2];
+    gpx_d4 e[NR], g[NR];
 #pragma unroll
-    for (int h2 = 0; h2 < 2; ++h2) {
+    for (int h2 = 0; h2 < NR; ++h2) {
       e[h2] = gpx_d4{0.0, 0.0, 0.0, 0.0};
       g[h2] = gpx_d4{0.0, 0.0, 0.0, 0.0};
     }
 #pragma unroll
     for (int s = 0; s < KSN; ++s)
       if (s < KS) {
-        e[0] = gpx_mfma16(pa[s], af[0][s], e[0]);
-        e[1] = gpx_mfma16(pa[s], af[1][s], e[1]);
+#pragma unroll
+        for (int h2 = 0; h2 < NR; ++h2) e[h2] = gpx_mfma16(pa[s], af[h2][s], e[h2]);
       }
 #pragma unroll
     for (int s = 0; s < DS; ++s) {
-      g[0] = gpx_mfma16(da[s], bs[0][s], g[0]);
-      g[1] = gpx_mfma16(da[s], bs[1][s], g[1]);
+#pragma unroll
+      for (int h2 = 0; h2 < NR; ++h2) g[h2] = gpx_mfma16(da[s], bs[h2][s], g[h2]);
     }
     // operands of the contractions over the points, loaded while the exps run
     double dvv[4][DB], dq[4];
@@ -536,9 +561,9 @@ __global__ __launch_bounds__(256) void fused_bwd_pairs_kernel(
       for (int db = 0; db < DB; ++db) dvv[r][db] = dv[(long)nr * D + min(16 * db + (lane & 15), D - 1)];
       dq[r] = pf[(long)nr * PF_W + 4 * FKS_MAX + (lane & 15)];
     }
-    double p[2][4], L[2][4];
+    double p[NR][4], L[NR][4];
 #pragma unroll
-    for (int h2 = 0; h2 < 2; ++h2)
+    for (int h2 = 0; h2 < NR; ++h2)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int nn = (lane >> 4) + 4 * r;
@@ -550,16 +575,15 @@ __global__ __launch_bounds__(256) void fused_bwd_pairs_kernel(
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
 #pragma unroll
-      for (int db = 0; db < DB; ++db) {
-        dacc[0][db] = gpx_mfma16(dvv[r][db], p[0][r], dacc[0][db]);
-        dacc[1][db] = gpx_mfma16(dvv[r][db], p[1][r], dacc[1][db]);
-      }
-      lacc[0] = gpx_mfma16(dq[r], L[0][r], lacc[0]);
-      lacc[1] = gpx_mfma16(dq[r], L[1][r], lacc[1]);
+      for (int db = 0; db < DB; ++db)
+#pragma unroll
+        for (int h2 = 0; h2 < NR; ++h2) dacc[h2][db] = gpx_mfma16(dvv[r][db], p[h2][r], dacc[h2][db]);
+#pragma unroll
+      for (int h2 = 0; h2 < NR; ++h2) lacc[h2] = gpx_mfma16(dq[r], L[h2][r], lacc[h2]);
     }
   }
 #pragma unroll
-  for (int h2 = 0; h2 < 2; ++h2) {
+  for (int h2 = 0; h2 < NR; ++h2) {
     if (!act[h2]) continue;
     // dacc[db]: lane holds dBhat[d = 16db + (lane>>4) + 4r][pair b = jb]
 #pragma unroll
@@ -862,12 +886,18 @@ int gpx_fused_mm_bwd_main(int N, int M, int Q, int D, const double *mx, const do
   GPX_CHECK_LAUNCH();
   // per-pair pass over point ranges
   const int nchunk = ((N + nsplit - 1) / nsplit + 15) / 16 * 16;
-  dim3 gp(ntiles * 2, nsplit);
+  dim3 gp(ntiles * 4 / PAIRS_NR, nsplit);
   const int ks = (2 * Q + 2 + 3) / 4;  // compile-time slice counts for Q = 4, 5 (3) and 7, 8 (5)
 #define GPX_PAIRS(DBV, DSV, KSV)                                                                               \
-  hipLaunchKernelGGL((fused_bwd_pairs_kernel<DBV, DSV, KSV>), gp, dim3(256), 0, st, N, M, Q, D, z, ls, sf, Bp, dv, \
+  hipLaunchKernelGGL((fused_bwd_pairs_kernel<DBV, DSV, KSV, PAIRS_NR>), gp, dim3(256), 0, st, N, M, Q, D, z, ls, sf, \
+                     Bp, dv, \
                      (const double *)W.pf, nsplit, nchunk, W.dBpart, W.LWpart)
-  if (D <= 16) {
+  // DS = ceil(D / 4) rounded up to a power of two: slices of the D contraction
+  if (D <= 4) {
+    if (ks == 3) GPX_PAIRS(1, 1, 3); else if (ks == 5) GPX_PAIRS(1, 1, 5); else GPX_PAIRS(1, 1, 0);
+  } else if (D <= 8) {
+    if (ks == 3) GPX_PAIRS(1, 2, 3); else if (ks == 5) GPX_PAIRS(1, 2, 5); else GPX_PAIRS(1, 2, 0);
+  } else if (D <= 16) {
     if (ks == 3) GPX_PAIRS(1, 4, 3); else if (ks == 5) GPX_PAIRS(1, 4, 5); else GPX_PAIRS(1, 4, 0);
   } else {
     if (ks == 3) GPX_PAIRS(2, 8, 3); else if (ks == 5) GPX_PAIRS(2, 8, 5); else GPX_PAIRS(2, 8, 0);
@@ -879,16 +909,23 @@ int gpx_fused_mm_bwd_main(int N, int M, int Q, int D, const double *mx, const do
   GPX_CHECK_LAUNCH();
   // per-point pass over tile splits
   dim3 gq(gpx_cdiv(N, 16), splits);
-#define GPX_POINTS(DSV, KSV)                                                                                    \
-  hipLaunchKernelGGL((fused_bwd_points_kernel<DSV, KSV>), gq, dim3(256), points_lds_bytes(M, Q), st, N, M, Q, D, mx, \
-                     vx, z, ls, sf, Bp, \
-                     dv, (const double *)W.Pab, splits, W.Tpart, W.Vpart)
-  if (D <= 16) {
-    if (ks == 3) GPX_POINTS(4, 3); else if (ks == 5) GPX_POINTS(4, 5); else GPX_POINTS(4, 0);
+#define GPX_POINTS1(DSV, KSV, FULL)                                                                             \
+  hipLaunchKernelGGL((fused_bwd_points_kernel<DSV, KSV, FULL>), gq, dim3(256), points_lds_bytes(M, Q), st, N, M, Q, \
+                     D, mx, vx, z, ls, sf, Bp, dv, (const double *)W.Pab, splits, W.Tpart, W.Vpart)
+#define GPX_POINTS(DSV, KSV)                                      \
+  if (POINTS_DFULL && D == 4 * DSV) GPX_POINTS1(DSV, KSV, true); \
+  else GPX_POINTS1(DSV, KSV, false);
+  if (D <= 4) {
+    if (ks == 3) { GPX_POINTS(1, 3) } else if (ks == 5) { GPX_POINTS(1, 5) } else { GPX_POINTS(1, 0) }
+  } else if (D <= 8) {
+    if (ks == 3) { GPX_POINTS(2, 3) } else if (ks == 5) { GPX_POINTS(2, 5) } else { GPX_POINTS(2, 0) }
+  } else if (D <= 16) {
+    if (ks == 3) { GPX_POINTS(4, 3) } else if (ks == 5) { GPX_POINTS(4, 5) } else { GPX_POINTS(4, 0) }
   } else {
-    if (ks == 3) GPX_POINTS(8, 3); else if (ks == 5) GPX_POINTS(8, 5); else GPX_POINTS(8, 0);
+    if (ks == 3) { GPX_POINTS(8, 3) } else if (ks == 5) { GPX_POINTS(8, 5) } else { GPX_POINTS(8, 0) }
   }
 #undef GPX_POINTS
+#undef GPX_POINTS1
   GPX_CHECK_LAUNCH();
   // dpsi1 = dm_all Ahat
   int rc = gpx_gemm_batched(0, 0, N, M, D, 1.0, dm_all, D, 0, Ahat, M, 0, 0.0, W.dpsi1, M, 0, 1, 1, nullptr, 0,
