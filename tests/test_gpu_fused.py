@@ -31,8 +31,11 @@ def _spread(M, Q):
     return g[np.random.RandomState(1).permutation(len(g))[:M]]
 
 
+# D covers every slice count of the D contraction (1, 2, 4, 8), with D = 4 x slices and
+# not; M and N not multiples of 16 exercise the padded columns and points
 @pytest.mark.parametrize('N,M,Q,D', [(37, 20, 3, 2), (130, 40, 2, 5), (300, 100, 5, 5),
-                                     (256, 128, 8, 32), (64, 33, 1, 17)])
+                                     (256, 128, 8, 32), (64, 33, 1, 17), (100, 50, 4, 4),
+                                     (70, 24, 2, 8), (50, 16, 2, 16)])
 def test_fused_layer_matches_plain(N, M, Q, D):
     from geepee_amd import dops
     from geepee_amd.aep_models import DeviceParams, _shapes
