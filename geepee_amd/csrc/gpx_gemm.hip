@@ -230,6 +230,40 @@ __global__ __launch_bounds__(256) void gemm_fullk_kernel(
   }
 }
 
+// k == 1: an outer product.  One thread per output (a block per row segment) with the
+// product and epilogue of the MFMA path, which for k = 1 adds exact zeros to round(a b),
+// so the results are the same bits at a fraction of the latency.
+__global__ __launch_bounds__(256) void rank1_kernel(int m, int n, int tA, int tB, double alpha,
+                                                    const double *__restrict__ A, int lda, long sA,
+                                                    const double *__restrict__ B, int ldb, long sB, GemmEpi ep,
+                                                    double *__restrict__ C, int ldc, long sC, int batch) {
+  // grid: x over the rows (and columns in chunks of 256), y over the batch
+  const int b = blockIdx.y, i = blockIdx.x % m;
+  const double a = tA ? A[b * sA + i] : A[b * sA + (long)i * lda];
+  for (int j = (blockIdx.x / m) * 256 + threadIdx.x; j < n; j += 256 * ((gridDim.x + m - 1) / m)) {
+    const double bb = tB ? B[b * sB + (long)j * ldb] : B[b * sB + j];
+    epi_store(C + b * sC + (long)i * ldc + j, alpha * (a * bb), ep, b, i, j);
+  }
+}
+
+// n == 1: matrix-vector products.  One wave per output row: lanes stride over k, then a
+// fixed-order wave reduction, so a row's value depends only on its own operands.
+__global__ __launch_bounds__(256) void gemv_kernel(int m, int k, int tA, int tB, double alpha,
+                                                   const double *__restrict__ A, int lda, long sA,
+                                                   const double *__restrict__ B, int ldb, long sB, GemmEpi ep,
+                                                   double *__restrict__ C, int ldc, long sC, int batch) {
+  const long o = blockIdx.x * 4L + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (o >= (long)batch * m) return;  // whole wave; no barriers below
+  const int b = (int)(o / m), i = (int)(o - (long)b * m);
+  const double *Ab = A + b * sA, *Bb = B + b * sB;
+  double s = 0.0;
+  for (int kk = lane; kk < k; kk += 64)
+    s = fma(tA ? Ab[(long)kk * lda + i] : Ab[(long)i * lda + kk], tB ? Bb[kk] : Bb[(long)kk * ldb], s);
+  s = gpx_wave_sum(s);
+  if (lane == 0) epi_store(C + b * sC + (long)i * ldc, alpha * s, ep, b, i, 0);
+}
+
 }  // namespace
 
 extern "C" {
@@ -255,6 +289,20 @@ int gpx_gemm_batched_ex(int transA, int transB, int m, int n, int k, double alph
   if (k == 0) {  // C = beta * C
     hipLaunchKernelGGL(splitk_reduce, dim3(gpx_cdiv((long)batch * m * n, 256) > 4096 ? 4096 : gpx_cdiv((long)batch * m * n, 256)),
                        dim3(256), 0, st, m, n, batch, 0, 0.0, (const double *)nullptr, ep, C, ldc, strideC);
+    GPX_CHECK_LAUNCH();
+    return GPX_OK;
+  }
+  // outer products and matrix-vector products (automatic tile choice only)
+  if (tile == 0 && splits == 1 && k == 1) {
+    if (batch > 65535) return GPX_EINVAL;
+    hipLaunchKernelGGL(rank1_kernel, dim3(m * gpx_cdiv(n, 256), batch), dim3(256), 0, st, m, n, transA, transB,
+                       alpha, A, lda, strideA, B, ldb, strideB, ep, C, ldc, strideC, batch);
+    GPX_CHECK_LAUNCH();
+    return GPX_OK;
+  }
+  if (tile == 0 && splits == 1 && n == 1) {
+    hipLaunchKernelGGL(gemv_kernel, dim3(gpx_cdiv((long)batch * m, 4)), dim3(256), 0, st, m, k, transA, transB,
+                       alpha, A, lda, strideA, B, ldb, strideB, ep, C, ldc, strideC, batch);
     GPX_CHECK_LAUNCH();
     return GPX_OK;
   }

@@ -56,7 +56,10 @@ def say(*a):
 @pytest.mark.parametrize('m,n,k,tA,tB,batch,splits', [
     (16, 16, 4, 0, 0, 1, 1), (37, 53, 29, 0, 0, 1, 1), (64, 64, 64, 1, 0, 2, 1),
     (100, 100, 100, 0, 1, 5, 1), (130, 70, 300, 1, 1, 3, 1), (200, 33, 1000, 0, 1, 1, 4),
-    (5, 1, 7, 0, 0, 4, 1), (1, 300, 1, 0, 0, 1, 1)])
+    (5, 1, 7, 0, 0, 4, 1), (1, 300, 1, 0, 0, 1, 1),
+    # matrix-vector (n = 1) and outer-product (k = 1) kernels, every transpose
+    (128, 1, 128, 1, 0, 3, 1), (100, 1, 4096, 0, 1, 1, 1), (70, 1, 65, 1, 1, 2, 1),
+    (128, 128, 1, 0, 1, 4, 1), (33, 70, 1, 1, 0, 2, 1), (9, 5, 1, 1, 1, 3, 1)])
 def test_gemm(m, n, k, tA, tB, batch, splits):
     L = _lib()
     say('gemm', m, n, k, tA, tB, batch, splits)
@@ -101,6 +104,21 @@ def test_gemm_bias_epilogue(m, n, k, batch, splits, bcast):
     C = dev(np.full((batch, m, n), np.nan))   # beta = 0 never reads C
     L.mm(dev(A), dev(B), C, alpha=0.7, splits=splits, E=dev(E), gamma=-1.0)
     assert nrel(host(C), 0.7 * np.einsum('bij,bjk->bik', A, B) - E) < 1e-14
+
+
+def test_gemm_outer_product_matches_mfma_path():
+    """k = 1 products through the outer-product kernel give the bits of the matrix-core
+    kernel (forced with tile = 32), epilogue included."""
+    L = _lib()
+    say('gemm k=1 bits')
+    rs = np.random.RandomState(5)
+    u, v, X, E = rs.randn(6, 100, 1), rs.randn(6, 1, 100), rs.randn(6, 100, 100), rs.randn(100, 100)
+    outs = []
+    for tile in (0, 32):
+        C = dev(X)
+        L.mm(dev(u), dev(v), C, alpha=0.37, beta=1.0, E=dev(E), gamma=-0.5, tile=tile)
+        outs.append(host(C))
+    assert np.array_equal(outs[0], outs[1])
 
 
 @pytest.mark.parametrize('D,M', [(2, 100), (1, 200), (3, 64)])
