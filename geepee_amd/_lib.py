@@ -69,6 +69,9 @@ SIGNATURES = {
                               _p, _p, _p, _p, _p]),
     'gpx_fused_mm_bwd_main': (_i, [_i, _i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p, _p,
                                    _p, _p, _p]),
+    'gpx_fused_mm_bwd_points': (_i, [_i, _i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p, _p,
+                                     _p, _p]),
+    'gpx_fused_mm_bwd_pairs': (_i, [_i, _i, _i, _i, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p]),
     'gpx_fused_mm_bwd_tail': (_i, [_i, _i, _i, _i, _p, _p, _p, _i, _i, _p, _p, _p, _p, _p]),
     'gpx_psi_bwd_tail': (_i, [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p,
                               _p, _p]),

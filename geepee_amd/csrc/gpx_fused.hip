@@ -866,47 +866,25 @@ extern "C" {
 // given dm_all = dm - 2 dv m and dv (N x D), Ahat (D x M), Bp (pair-major symmetrised
 // Bhat, M x M x D) and psi1 from the forward: dBhat (D x M x M), dsf2, dell, dZ, dmx, dvx.
 // dL/dpsi2 = sum_d dv_nd Bsym[d] is symmetric by construction (kernels.py:415-418).
-// _main produces dBhat, dmx and dvx (what the layer below needs) and leaves the point and
-// pair statistics in `work`; _tail turns them into dsf2, dell, dZ and may run on another
-// stream once _main has completed there.
-int gpx_fused_mm_bwd_main(int N, int M, int Q, int D, const double *mx, const double *vx, const double *z,
-                          const double *ls, const double *sf, const double *Ahat, const double *Bp,
-                          const double *psi1, const double *dm_all, const double *dv, int splits, int nsplit,
-                          double *dBhat, double *dmx, double *dvx, double *work, void *stream) {
+// The work splits into three ordered parts on one `work` buffer: _points (point and pair
+// features, the per-point pass and the row finish: dmx and dvx, what the layer below
+// needs), _pairs (the per-pair pass: dBhat and LW) and _tail (dsf2, dell, dZ).  _pairs
+// and _tail may run on another stream once _points has completed there; _main is _points
+// then _pairs.
+int gpx_fused_mm_bwd_points(int N, int M, int Q, int D, const double *mx, const double *vx, const double *z,
+                            const double *ls, const double *sf, const double *Ahat, const double *Bp,
+                            const double *psi1, const double *dm_all, const double *dv, int splits, int nsplit,
+                            double *dmx, double *dvx, double *work, void *stream) {
   if (!fused_bwd_args_ok(N, M, Q, D, splits, nsplit)) return GPX_EINVAL;
-  if (!mx || !vx || !z || !ls || !sf || !Ahat || !Bp || !psi1 || !dm_all || !dv || !dBhat || !dmx || !dvx ||
-      !work)
+  if (!mx || !vx || !z || !ls || !sf || !Ahat || !Bp || !psi1 || !dm_all || !dv || !dmx || !dvx || !work)
     return GPX_EINVAL;
   hipStream_t st = (hipStream_t)stream;
-  const int T = (M + 15) / 16, ntiles = T * (T + 1) / 2;
   FusedBwdWork W(work, N, M, Q, D, splits, nsplit);
   hipLaunchKernelGGL(fused_point_feat_kernel, dim3(gpx_cdiv(N, 256)), dim3(256), 0, st, N, Q, mx, vx, ls, W.pf);
   GPX_CHECK_LAUNCH();
   hipLaunchKernelGGL(pair_p_kernel, dim3(gpx_cdiv((long)M * M, 256)), dim3(256), 0, st, M, Q, z, ls, W.Pab);
   GPX_CHECK_LAUNCH();
-  // per-pair pass over point ranges
-  const int nchunk = ((N + nsplit - 1) / nsplit + 15) / 16 * 16;
-  dim3 gp(ntiles * 4 / PAIRS_NR, nsplit);
   const int ks = (2 * Q + 2 + 3) / 4;  // compile-time slice counts for Q = 4, 5 (3) and 7, 8 (5)
-#define GPX_PAIRS(DBV, DSV, KSV)                                                                               \
-  hipLaunchKernelGGL((fused_bwd_pairs_kernel<DBV, DSV, KSV, PAIRS_NR>), gp, dim3(256), 0, st, N, M, Q, D, z, ls, sf, \
-                     Bp, dv, \
-                     (const double *)W.pf, nsplit, nchunk, W.dBpart, W.LWpart)
-  // DS = ceil(D / 4) rounded up to a power of two: slices of the D contraction
-  if (D <= 4) {
-    if (ks == 3) GPX_PAIRS(1, 1, 3); else if (ks == 5) GPX_PAIRS(1, 1, 5); else GPX_PAIRS(1, 1, 0);
-  } else if (D <= 8) {
-    if (ks == 3) GPX_PAIRS(1, 2, 3); else if (ks == 5) GPX_PAIRS(1, 2, 5); else GPX_PAIRS(1, 2, 0);
-  } else if (D <= 16) {
-    if (ks == 3) GPX_PAIRS(1, 4, 3); else if (ks == 5) GPX_PAIRS(1, 4, 5); else GPX_PAIRS(1, 4, 0);
-  } else {
-    if (ks == 3) GPX_PAIRS(2, 8, 3); else if (ks == 5) GPX_PAIRS(2, 8, 5); else GPX_PAIRS(2, 8, 0);
-  }
-#undef GPX_PAIRS
-  GPX_CHECK_LAUNCH();
-  hipLaunchKernelGGL(fused_pairs_finish, dim3(ntiles, D + 1), dim3(256), 0, st, M, Q, D, nsplit,
-                     (const double *)W.dBpart, (const double *)W.LWpart, dBhat, W.LW);
-  GPX_CHECK_LAUNCH();
   // per-point pass over tile splits
   dim3 gq(gpx_cdiv(N, 16), splits);
 #define GPX_POINTS1(DSV, KSV, FULL)                                                                             \
@@ -939,6 +917,51 @@ int gpx_fused_mm_bwd_main(int N, int M, int Q, int D, const double *mx, const do
 #undef GPX_ROWS
   GPX_CHECK_LAUNCH();
   return GPX_OK;
+}
+
+int gpx_fused_mm_bwd_pairs(int N, int M, int Q, int D, const double *z, const double *ls, const double *sf,
+                           const double *Bp, const double *dv, int splits, int nsplit, double *dBhat, double *work,
+                           void *stream) {
+  if (!fused_bwd_args_ok(N, M, Q, D, splits, nsplit)) return GPX_EINVAL;
+  if (!z || !ls || !sf || !Bp || !dv || !dBhat || !work) return GPX_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  const int T = (M + 15) / 16, ntiles = T * (T + 1) / 2;
+  FusedBwdWork W(work, N, M, Q, D, splits, nsplit);
+  // per-pair pass over point ranges
+  const int nchunk = ((N + nsplit - 1) / nsplit + 15) / 16 * 16;
+  dim3 gp(ntiles * 4 / PAIRS_NR, nsplit);
+  const int ks = (2 * Q + 2 + 3) / 4;
+#define GPX_PAIRS(DBV, DSV, KSV)                                                                               \
+  hipLaunchKernelGGL((fused_bwd_pairs_kernel<DBV, DSV, KSV, PAIRS_NR>), gp, dim3(256), 0, st, N, M, Q, D, z, ls, sf, \
+                     Bp, dv, \
+                     (const double *)W.pf, nsplit, nchunk, W.dBpart, W.LWpart)
+  // DS = ceil(D / 4) rounded up to a power of two: slices of the D contraction
+  if (D <= 4) {
+    if (ks == 3) GPX_PAIRS(1, 1, 3); else if (ks == 5) GPX_PAIRS(1, 1, 5); else GPX_PAIRS(1, 1, 0);
+  } else if (D <= 8) {
+    if (ks == 3) GPX_PAIRS(1, 2, 3); else if (ks == 5) GPX_PAIRS(1, 2, 5); else GPX_PAIRS(1, 2, 0);
+  } else if (D <= 16) {
+    if (ks == 3) GPX_PAIRS(1, 4, 3); else if (ks == 5) GPX_PAIRS(1, 4, 5); else GPX_PAIRS(1, 4, 0);
+  } else {
+    if (ks == 3) GPX_PAIRS(2, 8, 3); else if (ks == 5) GPX_PAIRS(2, 8, 5); else GPX_PAIRS(2, 8, 0);
+  }
+#undef GPX_PAIRS
+  GPX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(fused_pairs_finish, dim3(ntiles, D + 1), dim3(256), 0, st, M, Q, D, nsplit,
+                     (const double *)W.dBpart, (const double *)W.LWpart, dBhat, W.LW);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+int gpx_fused_mm_bwd_main(int N, int M, int Q, int D, const double *mx, const double *vx, const double *z,
+                          const double *ls, const double *sf, const double *Ahat, const double *Bp,
+                          const double *psi1, const double *dm_all, const double *dv, int splits, int nsplit,
+                          double *dBhat, double *dmx, double *dvx, double *work, void *stream) {
+  if (!dBhat) return GPX_EINVAL;
+  const int rc = gpx_fused_mm_bwd_points(N, M, Q, D, mx, vx, z, ls, sf, Ahat, Bp, psi1, dm_all, dv, splits, nsplit,
+                                         dmx, dvx, work, stream);
+  if (rc) return rc;
+  return gpx_fused_mm_bwd_pairs(N, M, Q, D, z, ls, sf, Bp, dv, splits, nsplit, dBhat, work, stream);
 }
 
 int gpx_fused_mm_bwd_tail(int N, int M, int Q, int D, const double *z, const double *ls, const double *sf,

@@ -306,31 +306,36 @@ def fused_mm_bwd(mx, vx, z, ls, sf, Ahat, Bp, psi1, dm_all, dv, plan=None):
     return outs
 
 
-class FusedBwdTail(object):
-    """The second part of a split fused adjoint: call() on the stream it should run on
-    (after that stream has waited for the first part) to get (dsf2, dell, dZ).  `work`
-    must stay alive until then."""
+class FusedBwdRest(object):
+    """The rest of a split fused adjoint (the per-pair pass and the tail): call() on the
+    stream it should run on, after that stream has waited for the first part, to get
+    (dBhat, dsf2, dell, dZ).  The tensors in `keep` must stay alive until then."""
 
-    def __init__(self, N, M, Q, D, plan, z, ls, sf, work):
-        self.args = (N, M, Q, D, plan, z, ls, sf)
+    def __init__(self, N, M, Q, D, plan, z, ls, sf, Bp, dv, work):
+        self.args = (N, M, Q, D, plan, z, ls, sf, Bp, dv)
         self.work = work
+        self.keep = [z, ls, sf, Bp, dv, work]
 
     def __call__(self):
-        N, M, Q, D, plan, z, ls, sf = self.args
+        N, M, Q, D, plan, z, ls, sf, Bp, dv = self.args
+        dBhat = empty(D, M, M, like=z)
+        call('gpx_fused_mm_bwd_pairs', N, M, Q, D, P(z), P(ls), P(sf), P(Bp), P(dv), plan['splits_b'],
+             plan['nsplit'], P(dBhat), P(self.work), S())
         outs = [empty(1, like=z), empty(Q, like=z), empty(M, Q, like=z)]
         call('gpx_fused_mm_bwd_tail', N, M, Q, D, P(z), P(ls), P(sf), plan['splits_b'], plan['nsplit'],
              *[P(o) for o in outs], P(self.work), S())
-        return outs
+        return [dBhat] + outs
 
 
 def fused_mm_bwd_split(mx, vx, z, ls, sf, Ahat, Bp, psi1, dm_all, dv, plan=None):
-    """fused_mm_bwd's first part now: (dBhat, dmx, dvx, tail) with tail a FusedBwdTail."""
+    """fused_mm_bwd's input gradients now: (dmx, dvx, rest), rest a FusedBwdRest giving
+    (dBhat, dsf2, dell, dZ) on the stream it is called on."""
     N, Q = mx.shape
     M = z.shape[0]
     D = Ahat.shape[0]
     plan = plan or fused_plan(N, M, Q, D)
-    dBhat, dmx, dvx = empty(D, M, M, like=mx), empty(N, Q, like=mx), empty(N, Q, like=mx)
+    dmx, dvx = empty(N, Q, like=mx), empty(N, Q, like=mx)
     work = empty(query('gpx_fused_bwd_workspace', N, M, Q, D, plan['splits_b'], plan['nsplit']), like=mx)
-    call('gpx_fused_mm_bwd_main', N, M, Q, D, P(mx), P(vx), P(z), P(ls), P(sf), P(Ahat), P(Bp), P(psi1),
-         P(dm_all), P(dv), plan['splits_b'], plan['nsplit'], P(dBhat), P(dmx), P(dvx), P(work), S())
-    return dBhat, dmx, dvx, FusedBwdTail(N, M, Q, D, plan, z, ls, sf, work)
+    call('gpx_fused_mm_bwd_points', N, M, Q, D, P(mx), P(vx), P(z), P(ls), P(sf), P(Ahat), P(Bp), P(psi1),
+         P(dm_all), P(dv), plan['splits_b'], plan['nsplit'], P(dmx), P(dvx), P(work), S())
+    return dmx, dvx, FusedBwdRest(N, M, Q, D, plan, z, ls, sf, Bp, dv, work)

@@ -443,10 +443,11 @@ class SGP_Layer(object):
         if dm_all is None:
             dm_all = dops.dmall(dm, dv, m)
         dAhat = mm(dm_all, psi1, tA=True)
-        tail = None
+        rest = None
         if psi2 is None and side is not None:
-            # the psi adjoint's hyper-parameter tail joins the forked chain below
-            dBhat, dmx, dvx, tail = dops.fused_mm_bwd_split(
+            # the psi adjoint's per-pair pass and hyper-parameter tail join the forked chain
+            # below: the layer below needs only dmx, dvx
+            dmx, dvx, rest = dops.fused_mm_bwd_split(
                 mx, vx, self.zu, self.ls, self.sf, self.Ahat, self._Bp, psi1, dm_all, dv)
         elif psi2 is None:
             dBhat, dsf2, dell, dzu, dmx, dvx = dops.fused_mm_bwd(
@@ -463,13 +464,13 @@ class SGP_Layer(object):
             grad = self._lvm_hyper_chain(dv, dAhat, dBhat, dsf2, dell, dzu, B, alpha)
         else:
             side.wait_stream(torch.cuda.current_stream())
-            if tail is None:
+            if rest is None:
                 keep.extend([dv, dAhat, dBhat, dsf2, dell, dzu])
             else:
-                keep.extend([dv, dAhat, dBhat, tail.work])
+                keep.extend([dv, dAhat] + rest.keep)
             with torch.cuda.stream(side):
-                if tail is not None:
-                    dsf2, dell, dzu = tail()
+                if rest is not None:
+                    dBhat, dsf2, dell, dzu = rest()
                 grad = self._lvm_hyper_chain(dv, dAhat, dBhat, dsf2, dell, dzu, B, alpha)
         return grad, {'mx': dmx, 'vx': dvx}
 

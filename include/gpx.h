@@ -236,11 +236,19 @@ int gpx_fused_mm_bwd(int N, int M, int Q, int D, const double *mx, const double 
                      const double *psi1, const double *dm_all, const double *dv, int splits, int nsplit,
                      double *dBhat, double *dsf2, double *dell, double *dZ, double *dmx, double *dvx,
                      double *work, void *stream);
-/* gpx_fused_mm_bwd in two parts over the same workspace: _main gives dBhat, dmx, dvx (the
- * input gradients the layer below needs first); _tail gives dsf2, dell, dZ and may be
- * queued on another stream that waits for _main, so a deep GP's backward can continue
- * down the layers while the hyper-parameter adjoint finishes.  Same reference code as
+/* gpx_fused_mm_bwd in parts over the same workspace, run in this order: _points gives dmx,
+ * dvx (the input gradients the layer below needs first); _pairs gives dBhat; _tail gives
+ * dsf2, dell, dZ.  _pairs and _tail may be queued on another stream that waits for
+ * _points, so a deep GP's backward can continue down the layers while the
+ * hyper-parameter adjoint finishes.  _main is _points then _pairs.  Same reference code as
  * gpx_fused_mm_bwd. */
+int gpx_fused_mm_bwd_points(int N, int M, int Q, int D, const double *mx, const double *vx, const double *z,
+                            const double *ls, const double *sf, const double *Ahat, const double *Bp,
+                            const double *psi1, const double *dm_all, const double *dv, int splits, int nsplit,
+                            double *dmx, double *dvx, double *work, void *stream);
+int gpx_fused_mm_bwd_pairs(int N, int M, int Q, int D, const double *z, const double *ls, const double *sf,
+                           const double *Bp, const double *dv, int splits, int nsplit, double *dBhat, double *work,
+                           void *stream);
 int gpx_fused_mm_bwd_main(int N, int M, int Q, int D, const double *mx, const double *vx, const double *z,
                           const double *ls, const double *sf, const double *Ahat, const double *Bp,
                           const double *psi1, const double *dm_all, const double *dv, int splits, int nsplit,

@@ -72,14 +72,28 @@ def test_fused_layer_matches_plain(N, M, Q, D):
     plain = [dBhat] + list(dops.psi_bwd(mxd, vxd, L.zu, L.ls, L.sf, dops.mm(dm_all, L.Ahat), p10, dpsi2, p20))
     for a, b, name in zip(fused, plain, ('dBhat', 'dsf2', 'dell', 'dZ', 'dmx', 'dvx')):
         assert nrel(a.cpu().numpy(), b.cpu().numpy()) < 1e-12, name
-    # the split entry (main part here, tail on another stream) gives the same bits
+    # the split entry (per-point part here, per-pair part and tail on another stream)
+    # gives the same bits
     side = torch.cuda.Stream()
-    dB2, dmx2, dvx2, tail = dops.fused_mm_bwd_split(mxd, vxd, L.zu, L.ls, L.sf, L.Ahat, L._Bp, p11, dm_all, dv)
+    dmx2, dvx2, rest = dops.fused_mm_bwd_split(mxd, vxd, L.zu, L.ls, L.sf, L.Ahat, L._Bp, p11, dm_all, dv)
     side.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(side):
-        t3 = tail()
+        t4 = rest()
     torch.cuda.current_stream().wait_stream(side)
-    for a, b, name in zip([dB2] + list(t3) + [dmx2, dvx2], fused, ('dBhat', 'dsf2', 'dell', 'dZ', 'dmx', 'dvx')):
+    for a, b, name in zip(list(t4) + [dmx2, dvx2], fused, ('dBhat', 'dsf2', 'dell', 'dZ', 'dmx', 'dvx')):
+        assert torch.equal(a, b), name
+    # so does _main (per-point then per-pair part) followed by _tail, through the C-ABI
+    from geepee_amd import _lib
+    plan = dops.fused_plan(N, M, Q, D)
+    sb, ns = plan['splits_b'], plan['nsplit']
+    work = torch.empty(_lib.load().gpx_fused_bwd_workspace(N, M, Q, D, sb, ns), dtype=torch.float64, device='cuda')
+    outs = [torch.empty_like(t) for t in fused]
+    P = _lib.P
+    _lib.call('gpx_fused_mm_bwd_main', N, M, Q, D, P(mxd), P(vxd), P(L.zu), P(L.ls), P(L.sf), P(L.Ahat), P(L._Bp),
+              P(p11), P(dm_all), P(dv), sb, ns, P(outs[0]), P(outs[4]), P(outs[5]), P(work), _lib.S())
+    _lib.call('gpx_fused_mm_bwd_tail', N, M, Q, D, P(L.zu), P(L.ls), P(L.sf), sb, ns, P(outs[1]), P(outs[2]),
+              P(outs[3]), P(work), _lib.S())
+    for a, b, name in zip(outs, fused, ('dBhat', 'dsf2', 'dell', 'dZ', 'dmx', 'dvx')):
         assert torch.equal(a, b), name
     # layer level: the rest of the chain is shared code through Kuu^-1; the problems are
     # chosen so that a 1e-15 relative change of psi2 moves no layer gradient by more than
