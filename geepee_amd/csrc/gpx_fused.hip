@@ -73,21 +73,6 @@ __device__ __forceinline__ double pair_feature(int k, int Q, const double *za, c
   return 0.0;
 }
 
-// pair_feature with the k = 2Q entry P_ab read from Pab (pair_p_kernel) instead of
-// summed over q: the hot loops of the forward and per-point passes evaluate features
-// per row block, and that sum was most of their feature work.
-__device__ __forceinline__ double pair_feature_p(int k, int Q, const double *za, const double *zb,
-                                                 const double *pab) {
-  if (k == 2 * Q) return *pab;
-  if (k < Q) return 0.5 * (za[k] + zb[k]);
-  if (k < 2 * Q) {
-    const double h = 0.5 * (za[k - Q] + zb[k - Q]);
-    return -h * h;
-  }
-  if (k == 2 * Q + 1) return 1.0;
-  return 0.0;
-}
-
 // Branch-free pair feature for the hot loops.  Feature k = (lane>>4) + 4s is fixed per lane
 // and slice, so its kind is set up once (PairSel); each evaluation reads both z rows at a
 // clamped index and selects, so no lane's load sits behind a branch (a load under a
@@ -109,7 +94,10 @@ __device__ __forceinline__ PairSel pair_sel(int k, int Q) {
   return p;
 }
 
-// Same value as pair_feature_p for the same (k, za, zb, P_ab).
+// Same value as pair_feature for the same (k, za, zb), with the k = 2Q entry P_ab read
+// from Pab (pair_p_kernel) instead of summed over q: the hot loops of the forward and
+// per-point passes evaluate features per row block, and that sum was most of their
+// feature work.
 __device__ __forceinline__ double pair_feature_sel(const PairSel &p, const double *za, const double *zb,
                                                    double pab) {
   const double h = 0.5 * (za[p.kq] + zb[p.kq]);
@@ -169,16 +157,12 @@ __global__ __launch_bounds__(256) void fused_fwd_kernel(int N, int M, int Q, int
                                                         const double *__restrict__ Pab, int S,
                                                         double *__restrict__ part) {
   __shared__ double zs[FM_MAX][FQ_MAX + 1];
-  __shared__ double lam[FQ_MAX], il4[FQ_MAX];
+  __shared__ double lam[FQ_MAX];
   __shared__ double red[4][DB * 16][17];
   const int n0 = blockIdx.x * 16 * NSUB, split = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int e = tid; e < M * Q; e += 256) zs[e / Q][e % Q] = z[e];
-  if (tid < Q) {
-    const double l = exp(2.0 * ls[tid]);
-    lam[tid] = l;
-    il4[tid] = 0.25 / l;
-  }
+  if (tid < Q) lam[tid] = exp(2.0 * ls[tid]);
   __syncthreads();
   constexpr int KSN = KSC ? KSC : FKS_MAX;
   const int KS = KSC ? KSC : (2 * Q + 2 + 3) / 4;
