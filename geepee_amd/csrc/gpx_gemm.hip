@@ -216,11 +216,21 @@ __global__ __launch_bounds__(256) void gemm_fullk_kernel(
   __syncthreads();
   const int lane = tid & 63, wave = tid >> 6, wm = wave >> 1, wn = wave & 1;
   gpx_d4 acc = gpx_d4{0.0, 0.0, 0.0, 0.0};
-  for (int kq = 0; kq < kp; kq += 4) {
-    const double a = As[kq + (lane >> 4)][wm * 16 + (lane & 15)];
-    const double b = Bs[kq + (lane >> 4)][wn * 16 + (lane & 15)];
-    acc = gpx_mfma16(a, b, acc);
+  // eight 4-wide slices at a time: their LDS reads issue ahead of the MFMAs, which then
+  // run back to back on one accumulator (same order as one slice at a time)
+  const int ar = wm * 16 + (lane & 15), bc = wn * 16 + (lane & 15), g = lane >> 4;
+  int kq = 0;
+  for (; kq + 32 <= kp; kq += 32) {
+    double a[8], b[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      a[u] = As[kq + 4 * u + g][ar];
+      b[u] = Bs[kq + 4 * u + g][bc];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc = gpx_mfma16(a[u], b[u], acc);
   }
+  for (; kq < kp; kq += 4) acc = gpx_mfma16(As[kq + g][ar], Bs[kq + g][bc], acc);
   C += bidx * sC;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
