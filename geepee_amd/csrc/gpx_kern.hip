@@ -173,13 +173,18 @@ __global__ __launch_bounds__(256) void dtrace_kernel(int M, int Q, const double 
     }
 }
 
-__global__ void dtrace_final(int M, int Q, const double *__restrict__ part, double *__restrict__ gsf,
-                             double *__restrict__ gls) {
-  const int j = threadIdx.x;
-  if (j > Q) return;
+// One wave per output column j <= Q: lanes stride over the M rows, then a fixed-order
+// wave reduction (a single thread per column summing M dependent loads was 16 us).
+__global__ __launch_bounds__(64) void dtrace_final(int M, int Q, const double *__restrict__ part,
+                                                   double *__restrict__ gsf, double *__restrict__ gls) {
+  const int j = blockIdx.x, lane = threadIdx.x;
   double s = 0.0;
-  for (int a = 0; a < M; ++a) s += part[(long)a * (Q + 1) + j];
-  if (j == 0) gsf[0] = s; else gls[j - 1] = s;
+  for (int a = lane; a < M; a += 64) s += part[(long)a * (Q + 1) + j];
+  s = gpx_wave_sum(s);
+  if (lane == 0) {
+    if (j == 0) gsf[0] = s;
+    else gls[j - 1] = s;
+  }
 }
 
 // One block per column, threads stride over the rows (fixed-order block reduction).
@@ -188,6 +193,7 @@ __global__ __launch_bounds__(256) void colsum_tall_kernel(int rows, int cols, co
   __shared__ double red[16];
   const int j = blockIdx.x;
   double s = 0.0;
+#pragma unroll 8
   for (long i = threadIdx.x; i < rows; i += 256) s += P[i * cols + j];
   s = gpx_block_sum(s, red);
   if (threadIdx.x == 0) out[j] = (beta == 0.0) ? alpha * s : alpha * s + beta * out[j];
@@ -217,6 +223,8 @@ __global__ void colsum_kernel(int rows, int cols, const double *__restrict__ P, 
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= cols) return;
   double s = 0.0;
+  // unrolled: eight loads in flight ahead of the (same-order) adds
+#pragma unroll 8
   for (int i = 0; i < rows; ++i) s += P[(long)i * cols + j];
   out[j] = (beta == 0.0) ? alpha * s : alpha * s + beta * out[j];
 }
@@ -266,9 +274,14 @@ __global__ __launch_bounds__(256) void pack_dR_kernel(int D, int M, const double
     double av[4], bv[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
+      // loads at clamped indices, unconditional (so they issue together), then zeroed
+      // where the padding needs it by a 0/1 weight (a select lets the compiler sink the
+      // load back under a branch); the clamped values are finite
       const int k = K * 16 + 4 * s + lr;
-      av[s] = (i < M && k < M) ? Rd[(long)i * M + k] : 0.0;
-      bv[s] = (j < M && k < M) ? Gd[(long)k * M + j] + Gd[(long)j * M + k] : 0.0;
+      const int ic = min(i, M - 1), jc = min(j, M - 1), kc = min(k, M - 1);
+      const double a = Rd[(long)ic * M + kc], b = Gd[(long)kc * M + jc] + Gd[(long)jc * M + kc];
+      av[s] = a * ((i < M && k < M) ? 1.0 : 0.0);
+      bv[s] = b * ((j < M && k < M) ? 1.0 : 0.0);
     }
 #pragma unroll
     for (int s = 0; s < 4; ++s) acc = gpx_mfma16(av[s], bv[s], acc);
@@ -333,6 +346,7 @@ __global__ void batched_dot_kernel(int batch, int n, const double *__restrict__ 
   __shared__ double red[16];
   const int b = blockIdx.x;
   double s = 0.0;
+#pragma unroll 4
   for (int i = threadIdx.x; i < n; i += blockDim.x) s += x[(long)b * n + i] * y[(long)b * n + i];
   s = gpx_block_sum(s, red);
   if (threadIdx.x == 0) out[b] = s;
@@ -606,6 +620,7 @@ __global__ void wsum_kernel(int N, int D, int M, double alpha, const double *__r
   if (e >= NM) return;
   const int n = (int)(e / M);
   double s = 0.0;
+#pragma unroll 8
   for (int d = 0; d < D; ++d) s += w[(long)n * D + d] * T[(long)d * NM + e];
   out[e] = (beta == 0.0) ? alpha * s : alpha * s + beta * out[e];
 }
@@ -742,7 +757,7 @@ int gpx_dtrace_kzz(int M, int Q, const double *Mm, const double *z, const double
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(dtrace_kernel, dim3(gpx_cdiv(M, 4)), dim3(256), 0, st, M, Q, Mm, z, ls, sf, gz, work);
   GPX_CHECK_LAUNCH();
-  hipLaunchKernelGGL(dtrace_final, dim3(1), dim3(64), 0, st, M, Q, (const double *)work, gsf, gls);
+  hipLaunchKernelGGL(dtrace_final, dim3(Q + 1), dim3(64), 0, st, M, Q, (const double *)work, gsf, gls);
   GPX_CHECK_LAUNCH();
   return GPX_OK;
 }

@@ -307,6 +307,7 @@ __global__ __launch_bounds__(256) void colsum_kernel(int rows, int cols, const d
   __shared__ double red[16];
   const int j = blockIdx.x;
   double s = 0.0;
+#pragma unroll 8
   for (long i = threadIdx.x; i < rows; i += 256) s += P[i * cols + j];
   s = gpx_block_sum(s, red);
   if (threadIdx.x == 0) out[j] = s;
