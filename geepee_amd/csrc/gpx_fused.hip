@@ -770,6 +770,7 @@ __global__ __launch_bounds__(256) void fused_fwd_finish(int N, int M, int Q, int
     const int nn = e / D, d = e % D, n = n0 + nn;
     if (n >= N) continue;
     double m = 0.0;
+#pragma unroll 8
     for (int k = 0; k < M; ++k) m += p1[nn][k] * Ahat[(long)d * M + k];
     double s = 0.0;
     for (int sp = 0; sp < S; ++sp) s += part[((long)sp * N + n) * D + d];
