@@ -22,15 +22,6 @@
 #define FQ_MAX 16
 #define FM_MAX 256
 #define FKS_MAX 9  // ceil((2*FQ_MAX+2)/4)
-#ifndef PAIRS_NR
-#define PAIRS_NR 1
-#endif
-#ifndef POINTS_PREFETCH
-#define POINTS_PREFETCH 1
-#endif
-#ifndef POINTS_DFULL
-#define POINTS_DFULL 1
-#endif
 
 namespace {
 
@@ -346,16 +337,10 @@ __global__ __launch_bounds__(256) void fused_bwd_points_kernel(
       const int A = it >> 4, a = it;
       const bool off = (A != B);
       const double w = off ? 2.0 : 1.0;
-#if POINTS_PREFETCH
       const int itn = min(it + 4, nit - 1);
 #pragma unroll
       for (int s = 0; s < DS; ++s) bnxt[s] = Bp[((long)itn * M + jl) * D + dcl[s]];
       pnxt = Pab[(long)itn * M + jl];
-#else
-#pragma unroll
-      for (int s = 0; s < DS; ++s) bcur[s] = Bp[((long)it * M + jl) * D + dcl[s]];
-      pcur = Pab[(long)it * M + jl];
-#endif
       const double *za = zs + a * ZS;
       gpx_d4 e = gpx_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -384,11 +369,9 @@ __global__ __launch_bounds__(256) void fused_bwd_points_kernel(
         const double zz = w * za[qcl] * zs[b * ZS + qcl];
         vacc = gpx_mfma16(zz, L[r], vacc);
       }
-#if POINTS_PREFETCH
 #pragma unroll
       for (int s = 0; s < DS; ++s) bcur[s] = bnxt[s];
       pcur = pnxt;
-#endif
     }
     if (B > 0) {
       // merge the column sums of tile column B: stage per wave, then one thread per
@@ -876,7 +859,7 @@ int gpx_fused_mm_bwd_points(int N, int M, int Q, int D, const double *mx, const 
   hipLaunchKernelGGL((fused_bwd_points_kernel<DSV, KSV, FULL>), gq, dim3(256), points_lds_bytes(M, Q), st, N, M, Q, \
                      D, mx, vx, z, ls, sf, Bp, dv, (const double *)W.Pab, splits, W.Tpart, W.Vpart)
 #define GPX_POINTS(DSV, KSV)                                      \
-  if (POINTS_DFULL && D == 4 * DSV) GPX_POINTS1(DSV, KSV, true); \
+  if (D == 4 * DSV) GPX_POINTS1(DSV, KSV, true);                 \
   else GPX_POINTS1(DSV, KSV, false);
   if (D <= 4) {
     if (ks == 3) { GPX_POINTS(1, 3) } else if (ks == 5) { GPX_POINTS(1, 5) } else { GPX_POINTS(1, 0) }
@@ -914,10 +897,11 @@ int gpx_fused_mm_bwd_pairs(int N, int M, int Q, int D, const double *z, const do
   FusedBwdWork W(work, N, M, Q, D, splits, nsplit);
   // per-pair pass over point ranges
   const int nchunk = ((N + nsplit - 1) / nsplit + 15) / 16 * 16;
-  dim3 gp(ntiles * 4 / PAIRS_NR, nsplit);
+  constexpr int NR = 1;  // rows per wave (one: three waves per SIMD at D = 32; two: one)
+  dim3 gp(ntiles * 4 / NR, nsplit);
   const int ks = (2 * Q + 2 + 3) / 4;
 #define GPX_PAIRS(DBV, DSV, KSV)                                                                               \
-  hipLaunchKernelGGL((fused_bwd_pairs_kernel<DBV, DSV, KSV, PAIRS_NR>), gp, dim3(256), 0, st, N, M, Q, D, z, ls, sf, \
+  hipLaunchKernelGGL((fused_bwd_pairs_kernel<DBV, DSV, KSV, NR>), gp, dim3(256), 0, st, N, M, Q, D, z, ls, sf, \
                      Bp, dv, \
                      (const double *)W.pf, nsplit, nchunk, W.dBpart, W.LWpart)
   // DS = ceil(D / 4) rounded up to a power of two: slices of the D contraction
