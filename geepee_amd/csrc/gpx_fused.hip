@@ -114,6 +114,22 @@ __global__ __launch_bounds__(256) void pair_p_kernel(int M, int Q, const double 
   P[e] = p;
 }
 
+// v summed over the four 16-lane rows of the wave (lanes l, l^16, l^32, l^48), with the
+// gfx950 cross-row swaps on the vector ALU instead of LDS permutes.  permlane16_swap(x, x)
+// gives x[l & ~16] and x[l | 16], permlane32_swap(x, x) gives x[l % 32] and
+// x[l % 32 + 32]: the same additions as two xor-shuffle steps, so the same bits.
+__device__ __forceinline__ double sum_rows4(double v) {
+  unsigned lo = __double2loint(v), hi = __double2hiint(v);
+  const auto a = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+  const auto b = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+  v = __hiloint2double(b[0], a[0]) + __hiloint2double(b[1], a[1]);
+  lo = __double2loint(v);
+  hi = __double2hiint(v);
+  const auto c = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+  const auto d = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+  return __hiloint2double(d[0], c[0]) + __hiloint2double(d[1], c[1]);
+}
+
 // Per-point constants: mu, d = 1/(2v+l), c = sum [log(l/(l+2v))/2 - d mu^2].
 __device__ __forceinline__ double point_consts(int n, int N, int Q, const double *mx, const double *vx,
                                                const double *lam, double *mu, double *d) {
@@ -359,8 +375,7 @@ __global__ __launch_bounds__(256) void fused_bwd_points_kernel(
         rs += L[r];
         if (off) colp[r] += L[r];
       }
-      rs += __shfl_xor(rs, 16, 64);
-      rs += __shfl_xor(rs, 32, 64);
+      rs = sum_rows4(rs);
       if ((lane >> 4) == 0) Tacc[(lane & 15) * TS + a] += rs;
       // V[q x n] += w z_aq z_bq L[pair x n]  (padding pairs have L = 0)
 #pragma unroll
