@@ -15,10 +15,32 @@
 
 static inline unsigned gpx_cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
-// Wave64 sum reduction with cross-lane shuffles.
+// v + v[lane ^ O] for O = 32 or 16, with gfx950's cross-row lane swaps on the vector ALU:
+// permlane32_swap(x, x) gives x[l % 32] and x[l % 32 + 32], permlane16_swap(x, x) gives
+// x[l & ~16] and x[l | 16], so their sum is the xor-shuffle step's sum (the two operands
+// in either order), bit for bit.
+template <int O>
+__device__ __forceinline__ double gpx_swap_add(double v) {
+  static_assert(O == 16 || O == 32, "cross-row swaps pair lanes 16 or 32 apart");
+  const unsigned lo = __double2loint(v), hi = __double2hiint(v);
+  if constexpr (O == 32) {
+    const auto a = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto b = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    return __hiloint2double(b[0], a[0]) + __hiloint2double(b[1], a[1]);
+  } else {
+    const auto a = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto b = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    return __hiloint2double(b[0], a[0]) + __hiloint2double(b[1], a[1]);
+  }
+}
+
+// Wave64 sum reduction (every lane gets the total): the xor butterfly 32, 16, 8, 4, 2, 1,
+// its first two steps as lane swaps.
 __device__ __forceinline__ double gpx_wave_sum(double v) {
+  v = gpx_swap_add<32>(v);
+  v = gpx_swap_add<16>(v);
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
 

@@ -114,21 +114,9 @@ __global__ __launch_bounds__(256) void pair_p_kernel(int M, int Q, const double 
   P[e] = p;
 }
 
-// v summed over the four 16-lane rows of the wave (lanes l, l^16, l^32, l^48), with the
-// gfx950 cross-row swaps on the vector ALU instead of LDS permutes.  permlane16_swap(x, x)
-// gives x[l & ~16] and x[l | 16], permlane32_swap(x, x) gives x[l % 32] and
-// x[l % 32 + 32]: the same additions as two xor-shuffle steps, so the same bits.
-__device__ __forceinline__ double sum_rows4(double v) {
-  unsigned lo = __double2loint(v), hi = __double2hiint(v);
-  const auto a = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
-  const auto b = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
-  v = __hiloint2double(b[0], a[0]) + __hiloint2double(b[1], a[1]);
-  lo = __double2loint(v);
-  hi = __double2hiint(v);
-  const auto c = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
-  const auto d = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
-  return __hiloint2double(d[0], c[0]) + __hiloint2double(d[1], c[1]);
-}
+// v summed over the four 16-lane rows of the wave (lanes l, l^16, l^32, l^48): the
+// xor-shuffle steps 16 then 32 as cross-row lane swaps (same bits, no LDS).
+__device__ __forceinline__ double sum_rows4(double v) { return gpx_swap_add<32>(gpx_swap_add<16>(v)); }
 
 // Per-point constants: mu, d = 1/(2v+l), c = sum [log(l/(l+2v))/2 - d mu^2].
 __device__ __forceinline__ double point_consts(int n, int N, int Q, const double *mx, const double *vx,
