@@ -706,6 +706,10 @@ __global__ __launch_bounds__(256) void fused_rows_finish(
 }
 
 // psi1, m = psi1 Ahat^T and v = sf2 + sum_s part - m^2 for 16 points per block.
+// Points per block of the forward finish (8: twice the blocks of 16, so the latency-bound
+// psi1 and mean loops have two waves per SIMD at N = 4096).
+constexpr int FIN_PTS = 8;
+
 __global__ __launch_bounds__(256) void fused_fwd_finish(int N, int M, int Q, int D, const double *__restrict__ mx,
                                                         const double *__restrict__ vx,
                                                         const double *__restrict__ z,
@@ -715,19 +719,19 @@ __global__ __launch_bounds__(256) void fused_fwd_finish(int N, int M, int Q, int
                                                         const double *__restrict__ part, int S,
                                                         double *__restrict__ psi1, double *__restrict__ mout,
                                                         double *__restrict__ vout) {
-  __shared__ double p1[16][FM_MAX + 1];
-  __shared__ double den[16][FQ_MAX], mus[16][FQ_MAX], lcs[16];
-  const int n0 = blockIdx.x * 16, tid = threadIdx.x;
+  __shared__ double p1[FIN_PTS][FM_MAX + 1];
+  __shared__ double den[FIN_PTS][FQ_MAX], mus[FIN_PTS][FQ_MAX], lcs[FIN_PTS];
+  const int n0 = blockIdx.x * FIN_PTS, tid = threadIdx.x;
   const double sf2 = exp(2.0 * sf[0]);
   // per point: 2 (v + l) and mx per dimension, and sum_q log(l / (l + v)) / 2 (the
   // logs do not depend on the inducing point)
-  if (tid < 16 * Q) {
+  if (tid < FIN_PTS * Q) {
     const int nn = tid / Q, q = tid % Q, n = n0 + nn;
     const double l = exp(2.0 * ls[q]), v = n < N ? vx[(long)n * Q + q] : 0.0;
     den[nn][q] = 2.0 * (v + l);
     mus[nn][q] = n < N ? mx[(long)n * Q + q] : 0.0;
   }
-  if (tid < 16) {
+  if (tid < FIN_PTS) {
     const int n = n0 + tid;
     double lc = 0.0;
     for (int q = 0; q < Q && n < N; ++q) {
@@ -737,7 +741,7 @@ __global__ __launch_bounds__(256) void fused_fwd_finish(int N, int M, int Q, int
     lcs[tid] = lc;
   }
   __syncthreads();
-  for (int e = tid; e < 16 * M; e += 256) {
+  for (int e = tid; e < FIN_PTS * M; e += 256) {
     const int nn = e / M, m = e % M, n = n0 + nn;
     double val = 0.0;
     if (n < N) {
@@ -752,7 +756,7 @@ __global__ __launch_bounds__(256) void fused_fwd_finish(int N, int M, int Q, int
     p1[nn][m] = val;
   }
   __syncthreads();
-  for (int e = tid; e < 16 * D; e += 256) {
+  for (int e = tid; e < FIN_PTS * D; e += 256) {
     const int nn = e / D, d = e % D, n = n0 + nn;
     if (n >= N) continue;
     double m = 0.0;
@@ -1015,7 +1019,7 @@ int gpx_fused_mm_fwd(int N, int M, int Q, int D, const double *mx, const double 
 #undef GPX_FWD
 #undef GPX_FWD1
   GPX_CHECK_LAUNCH();
-  hipLaunchKernelGGL(fused_fwd_finish, dim3(gpx_cdiv(N, 16)), dim3(256), 0, st, N, M, Q, D, mx, vx, z, ls, sf,
+  hipLaunchKernelGGL(fused_fwd_finish, dim3(gpx_cdiv(N, FIN_PTS)), dim3(256), 0, st, N, M, Q, D, mx, vx, z, ls, sf,
                      Ahat, (const double *)work, splits, psi1, mout, vout);
   GPX_CHECK_LAUNCH();
   return GPX_OK;
