@@ -24,6 +24,7 @@ SIGNATURES = {
                                  _d, _p, _i, _l, _i, _i, _p, _i, _p]),
     'gpx_spd_max_m': (_i, []),
     'gpx_spd_inv_batched': (_i, [_i, _i, _p, _i, _l, _p, _i, _l, _p, _p, _p]),
+    'gpx_status_fold': (_i, [_i, _p, _p, _p, _p]),
     'gpx_se_kernel': (_i, [_i, _i, _i, _p, _i, _p, _p, _p, _d, _p, _p]),
     'gpx_kfu_bwd_workspace': (_l, [_i, _i, _i]),
     'gpx_kfu_bwd': (_i, [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),

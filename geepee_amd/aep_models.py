@@ -88,6 +88,9 @@ class Base_Model(object):
         self._dp = None
         _lib.require_device()
         self.y_dev = torch.as_tensor(self.y_train, device='cuda')
+        # SPD status of the last evaluation ([0]) and a flag that stays set until
+        # optimise_device clears it ([1]); written on the device by gpx_status_fold
+        self._status = torch.zeros(2, dtype=torch.int32, device='cuda')
 
     # parameter transfer -------------------------------------------------
     def _device_params(self, params):
@@ -105,7 +108,6 @@ class Base_Model(object):
         895, 1024)."""
         _check_mode(prop_mode)
         energy, grads, div = self._objective(params, mb_size, alpha)
-        self._check_info(self._layers())
         return self._finish(energy, grads, params, div)
 
     def objective_function_device(self, params, mb_size, alpha=1.0, prop_mode=PROP_MM):
@@ -122,13 +124,33 @@ class Base_Model(object):
         flat = torch.cat([energy_dev.reshape(-1)] + [grads_dev[k].reshape(-1) for k in keys])
         if divide_by is not None:
             lincomb(flat, 1.0 / divide_by, flat)
+        dops.status_fold(self._infos(), self._status)
         return flat
+
+    def _infos(self):
+        out = []
+        for L in self._layers():
+            for name in ('info_Kuu', 'info_Su', 'info_Suhat'):
+                out.append(getattr(L, name, None))
+        return out
+
+    def check_status(self, sticky=False):
+        """Raise numpy.linalg.LinAlgError (what np.linalg.inv raises at
+        base_models.py:464, 471 and aep_models.py:525) if an SPD inverse of the last
+        evaluation -- or, with sticky=True, of any evaluation since the flag was
+        cleared -- met a non-positive pivot.  One device-to-host copy."""
+        st = self._status.cpu().numpy()
+        if st[1 if sticky else 0] > 0:
+            raise np.linalg.LinAlgError(
+                'matrix not positive definite (pivot %d of an SPD inverse)%s'
+                % (int(st[1 if sticky else 0]) - 1, ' during optimisation' if sticky else ''))
 
     def _finish(self, energy_dev, grads_dev, params, divide_by):
         """Zero fixed params, divide by N, return (float, numpy dict) with one D2H copy."""
         keys = sorted(grads_dev)
         flat = self._flat_device(energy_dev, grads_dev, divide_by)
         out = flat.cpu().numpy()
+        self.check_status()
         energy = float(out[0])
         grads = {}
         o = 1
@@ -151,16 +173,6 @@ class Base_Model(object):
             if isinstance(getattr(self, name, None), SGP_Layer):
                 out.append(getattr(self, name))
         return out + list(getattr(self, 'sgp_layers', []))
-
-    def _check_info(self, layers):
-        bad = []
-        for L in layers:
-            for name in ('info_Kuu', 'info_Su', 'info_Suhat'):
-                t = getattr(L, name, None)
-                if t is not None and int(t.max().item()) > 0:
-                    bad.append(name)
-        if bad:
-            raise np.linalg.LinAlgError('matrix not positive definite: %s' % ', '.join(bad))
 
     # optimisation (base_models.py:59-134) --------------------------------
     def optimise(self, method='L-BFGS-B', tol=None, reinit_hypers=True, callback=None,
@@ -223,10 +235,14 @@ class Base_Model(object):
         n = dp.total
         mom = torch.zeros(2, n, dtype=F64, device=dp.buf.device)
         costs = torch.empty(max(maxiter, 1), dtype=F64, device=dp.buf.device)
+        self._status[1:].zero_()
         for i in range(maxiter):
             out = step()
             call('gpx_adam_step', n, P(dp.buf), P(out[1:]), P(mom[0]), P(mom[1]), float(adam_lr),
                  float(b1), float(b2), float(eps), i + 1, P(out), P(costs[i:]), S())
+        # the reference's np.linalg.inv raises inside the loop; here the device flag is read
+        # once, after the last step
+        self.check_status(sticky=True)
         vec = dp.buf.cpu().numpy()
         final, o = {}, 0
         for k, sz in zip(dp.keys, dp.sizes):
@@ -312,8 +328,6 @@ class SGPR(Base_Model):
         return energy, grads, N
 
     def predict_f(self, inputs):
-        if not self.updated:
-            self.updated = True
         x = torch.as_tensor(np.asarray(inputs, dtype=np.float64), device='cuda')
         mf, vf = self.sgp_layer.forward_prop_thru_post(x)
         return mf.cpu().numpy(), vf.cpu().numpy()
@@ -448,7 +462,10 @@ class SDGPR(Base_Model):
             cur.wait_event(ev)
         return phis
 
-    def predict_f(self, inputs):
+    def predict_f(self, inputs, prop_mode=PROP_MM, no_samples=200):
+        """Base_SDGPR.predict_f (base_models.py:1132-1158): moment-matched propagation
+        through the layers' posteriors."""
+        _check_mode(prop_mode)
         x = torch.as_tensor(np.asarray(inputs, dtype=np.float64), device='cuda')
         mf, vf = self._predict_dev(x)
         return mf.cpu().numpy(), vf.cpu().numpy()

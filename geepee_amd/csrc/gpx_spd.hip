@@ -241,9 +241,49 @@ const gj_fn kGJ[16] = {launch_gj<1>,  launch_gj<2>,  launch_gj<3>,  launch_gj<4>
                              launch_gj<9>,  launch_gj<10>, launch_gj<11>, launch_gj<12>,
                              launch_gj<13>, launch_gj<14>, launch_gj<15>, launch_gj<16>};
 
+// Pivot-failure flags of several SPD launches folded into one status word pair:
+// status[0] = the largest info entry (0 = every matrix was positive definite),
+// status[1] = max(status[1], status[0]) (sticky across evaluations until the host
+// clears it).  One wave; the list descriptors travel by value in the kernel arguments,
+// so a captured launch needs no host memory.
+constexpr int kStatusLists = 16;
+struct StatusLists {
+  const int *ptr[kStatusLists];
+  int count[kStatusLists];
+};
+
+__global__ void __launch_bounds__(64) status_fold_kernel(int nlists, StatusLists L, int *status) {
+  int m = 0;
+  for (int j = 0; j < nlists; ++j) {
+    const int *p = L.ptr[j];
+    for (int i = threadIdx.x; i < L.count[j]; i += 64) m = max(m, p[i]);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+  if (threadIdx.x == 0) {
+    status[0] = m;
+    status[1] = max(status[1], m);
+  }
+}
+
 }  // namespace
 
 extern "C" {
+
+int gpx_status_fold(int nlists, const int *const *lists, const int *counts, int *status,
+                    void *stream) {
+  if (nlists < 0 || nlists > kStatusLists || !status || (nlists > 0 && (!lists || !counts)))
+    return GPX_EINVAL;
+  StatusLists L = {};
+  for (int j = 0; j < nlists; ++j) {
+    if (counts[j] < 0 || (counts[j] > 0 && !lists[j])) return GPX_EINVAL;
+    L.ptr[j] = lists[j];
+    L.count[j] = counts[j];
+  }
+  hipLaunchKernelGGL(status_fold_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, nlists, L, status);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
 
 // Largest M the SPD kit accepts.
 int gpx_spd_max_m(void) { return 256; }

@@ -54,6 +54,20 @@ def spd_inv(A, out=None):
     return out, ld, info
 
 
+def status_fold(infos, status):
+    """status[0] = max of the int32 info tensors (0: every SPD launch succeeded),
+    status[1] = max(status[1], status[0]); one launch, no host synchronisation."""
+    import ctypes
+    infos = [t for t in infos if t is not None and t.numel() > 0]
+    if len(infos) > 16:
+        raise ValueError('at most 16 info arrays per fold')
+    ptrs = (ctypes.c_void_p * max(1, len(infos)))(*[t.data_ptr() for t in infos])
+    counts = (ctypes.c_int * max(1, len(infos)))(*[t.numel() for t in infos])
+    call('gpx_status_fold', len(infos), ctypes.cast(ptrs, ctypes.c_void_p),
+         ctypes.cast(counts, ctypes.c_void_p), P(status), S())
+    return status
+
+
 def lincomb(out, a, x, b=0.0, y=None, c=0.0, w=None):
     call('gpx_lincomb', out.numel(), float(a), P(x), x.numel(), float(b), P(y),
          y.numel() if y is not None else 1, float(c), P(w), w.numel() if w is not None else 1,

@@ -73,6 +73,7 @@ class GraphedObjective(object):
         if alpha is not None and alpha != self.alpha:
             raise ValueError('graph captured for alpha=%g' % self.alpha)
         out = self(params).cpu().numpy()
+        self.model.check_status()
         energy = float(out[0])
         grads = {}
         o = 1

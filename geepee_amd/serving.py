@@ -215,6 +215,25 @@ def make_requests(n_models, n_req, rate, dins, seed=0, max_batch=4096):
     return reqs
 
 
+def admit(sizes, used, budget, max_points, idle):
+    """How many queued requests (in arrival order) one launch takes.  sizes: an iterable
+    of (bytes, points) over a non-empty queue, read only as far as the launch reaches;
+    used: request I/O bytes of the launches in flight.  Requests
+    are taken while the in-flight bytes stay within `budget` and the points within
+    `max_points`.  A request larger than the whole budget runs alone, and only when
+    nothing else is in flight (`idle`); otherwise the queue head waits for memory."""
+    n, need, pts = 0, 0, 0
+    for b, p in sizes:
+        if used + need + b > budget or pts + p > max_points:
+            break
+        n += 1
+        need += b
+        pts += p
+    if n == 0 and idle and used == 0:
+        return 1
+    return n
+
+
 class PredictServer(object):
     """Continuous batching over two launch slots with an HBM budget for request I/O."""
 
@@ -252,17 +271,16 @@ class PredictServer(object):
                     done_count += len(h['ids'])
                     inflight[s] = None
             free = [s for s, h in enumerate(inflight) if h is None]
+            ntake = 0
             if queue and free:
-                take, pts, need = [], 0, 0
-                for j in queue:
-                    _, mid, X = requests[j]
-                    b = 8 * X.shape[0] * (self.store.Din[mid] + 2 * self.store.Dout[mid])
-                    if take and (used + need + b > self.budget or pts + X.shape[0] > self.max_points):
-                        break
-                    take.append(j)
-                    pts += X.shape[0]
-                    need += b
-                queue = queue[len(take):]
+                sizes = ((8 * requests[j][2].shape[0] * (self.store.Din[requests[j][1]]
+                                                         + 2 * self.store.Dout[requests[j][1]]),
+                          requests[j][2].shape[0]) for j in queue)
+                ntake = admit(sizes, used, self.budget, self.max_points,
+                              idle=all(h is None for h in inflight))
+            if ntake:
+                take = queue[:ntake]
+                queue = queue[ntake:]
                 s = free[0]
                 h = self.store.launch([(requests[j][1], requests[j][2]) for j in take], noisy,
                                       stream=self.streams[s])

@@ -61,6 +61,16 @@ int gpx_spd_max_m(void);
 int gpx_spd_inv_batched(int M, int batch, const double *A, int lda, long strideA, double *out,
                         int ldo, long strideOut, double *logdet, int *info, void *stream);
 
+/* Folds the `info` arrays of up to 16 gpx_spd_inv_batched calls (host arrays of nlists
+ * device pointers and their lengths) into a device status pair: status[0] = the largest
+ * entry (0 when every matrix was positive definite), status[1] = max(status[1],
+ * status[0]), a flag that stays set across evaluations until the caller clears it.
+ * The host raises numpy.linalg.LinAlgError from it, as np.linalg.inv does at
+ * geepee/base_models.py:464, 471 and geepee/aep_models.py:525, without synchronising
+ * inside a captured evaluation. */
+int gpx_status_fold(int nlists, const int *const *lists, const int *counts, int *status,
+                    void *stream);
+
 /* ---- SE kernel and psi statistics ------------------------------------------ */
 
 /* K[n,m] = exp(2 sf) exp(-sum_q (x_nq - z_mq)^2 / (2 exp(2 ls_q))) (+ jitter if n == m).

@@ -882,6 +882,33 @@ def sdgpr_predict(params, Ms, N, xs, noisy=False):
     return mf, vf
 
 
+def sgplvm_predict(params, M, N, xs, noisy=False):
+    """Base_SGPLVM.predict_f / predict_y (base_models.py:734-761): the layer's posterior
+    at deterministic inputs, plus exp(2 sn) for predict_y (lik_layers.py:238-249)."""
+    L = layer_update(params, '', M, N, True)
+    mf, vf = fwd_det_post(L, xs)
+    if noisy:
+        vf = vf + np.exp(2.0 * params['sn'])
+    return mf, vf
+
+
+def sgpssm_predict(params, M, T, xs, noisy=False, gp_emi=False):
+    """Base_SGPSSM.predict_f (base_models.py:1441-1451): the dynamics layer's posterior
+    at deterministic inputs.  predict_y (:1544-1560): through the GP emission layer's
+    moment-matched posterior plus exp(2 sn_emission), or through the linear-Gaussian
+    emission, keeping the diagonal of C diag(vf) C^T + diag(R) (lik_layers.py:558-571)."""
+    dyn = layer_update(params, '_dynamic', M, T - 1, True)
+    mf, vf = fwd_det_post(dyn, xs)
+    if not noisy:
+        return mf, vf
+    if gp_emi:
+        emi = layer_update(params, '_emission', M, T, True)
+        mg, vg = fwd_mm_post(emi, mf, vf)
+        return mg, vg + np.exp(2.0 * params['sn_emission'])
+    my, _, vy = emission_predict(params['C_emission'], params['R_emission'], mf, vf)
+    return my, np.diagonal(vy, axis1=1, axis2=2).copy()
+
+
 # ----------------------------------------------------------------------------
 # utils.py
 # ----------------------------------------------------------------------------

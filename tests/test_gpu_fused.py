@@ -35,7 +35,10 @@ def _spread(M, Q):
 # not; M and N not multiples of 16 exercise the padded columns and points
 @pytest.mark.parametrize('N,M,Q,D', [(37, 20, 3, 2), (130, 40, 2, 5), (300, 100, 5, 5),
                                      (256, 128, 8, 32), (64, 33, 1, 17), (100, 50, 4, 4),
-                                     (70, 24, 2, 8), (50, 16, 2, 16)])
+                                     (70, 24, 2, 8), (50, 16, 2, 16),
+                                     # the corners fused_fits admits: M = 256 with Q = 15 and
+                                     # D = 32 (the largest LDS request), M in (200, 256]
+                                     (70, 256, 15, 32), (45, 210, 9, 3)])
 def test_fused_layer_matches_plain(N, M, Q, D):
     from geepee_amd import dops
     from geepee_amd.aep_models import DeviceParams, _shapes

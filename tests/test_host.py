@@ -143,3 +143,46 @@ def test_model_init_keys_and_shapes():
     P = init.pca_reduce(y, 2)
     w, V = np.linalg.eigh(np.cov(y.T))
     assert np.allclose(np.abs(P), np.abs((y - y.mean(0)).dot(V[:, ::-1][:, :2])))
+
+
+def test_admission_respects_the_budget():
+    from geepee_amd.serving import admit
+    # fits: three requests of 100 bytes under a 350-byte budget
+    assert admit([(100, 1), (100, 1), (100, 1), (100, 1)], 0, 350, 1 << 16, idle=True) == 3
+    # the point cap stops a launch as the byte budget does
+    assert admit([(10, 40), (10, 40)], 0, 1000, 64, idle=True) == 1
+    # in-flight bytes count: the head waits while another launch holds the memory
+    assert admit([(100, 1)], 300, 350, 1 << 16, idle=False) == 0
+    # a request larger than the whole budget waits while anything is in flight ...
+    assert admit([(500, 1), (10, 1)], 100, 350, 1 << 16, idle=False) == 0
+    assert admit([(500, 1)], 0, 350, 1 << 16, idle=False) == 0
+    # ... and then runs alone
+    assert admit([(500, 1), (10, 1)], 0, 350, 1 << 16, idle=True) == 1
+    # the queue is read only as far as the launch reaches
+    seen = []
+
+    def sizes():
+        for i in range(10):
+            seen.append(i)
+            yield (100, 1)
+    assert admit(sizes(), 0, 250, 1 << 16, idle=True) == 2
+    assert seen == [0, 1, 2]
+
+
+def test_spd_status_raises_linalg_error():
+    """check_status reads the device status pair written by gpx_status_fold: entry 0 is
+    the last evaluation's largest info (k+1 for a failed pivot k), entry 1 the sticky
+    flag of the optimiser loop."""
+    import types
+    import torch
+    from geepee_amd.aep_models import Base_Model
+    m = types.SimpleNamespace(_status=torch.tensor([0, 0], dtype=torch.int32))
+    Base_Model.check_status(m)
+    Base_Model.check_status(m, sticky=True)
+    m._status = torch.tensor([4, 4], dtype=torch.int32)
+    with pytest.raises(np.linalg.LinAlgError, match='pivot 3'):
+        Base_Model.check_status(m)
+    m._status = torch.tensor([0, 2], dtype=torch.int32)
+    Base_Model.check_status(m)
+    with pytest.raises(np.linalg.LinAlgError, match='during optimisation'):
+        Base_Model.check_status(m, sticky=True)
