@@ -10,8 +10,12 @@ moment-matched kernels (psi2 recomputed, never stored); --path plain materialise
 sgpssm, sgpr, serving, copy (counter calibration).
 
 Prints one JSON line.  ``roofline`` reports the dominant kernel timed with HIP events
-on the launch stream over the timed region; ``cpu_baseline`` times the CPU oracle in
-its faithful-cost mode on a bounded sample (rank 0, N=1 only).
+on the launch stream over the timed region.  The default line also carries
+``c3_fused_vs_plain`` (the deep-GP evaluation on the plain path against the fused path
+replayed as a hipGraph) and ``serving`` (served predictions at three Poisson arrival
+rates, continuous batching against one request at a time), and ``cpu_baseline``: the CPU
+oracle on the same SGPLVM evaluation, faithful cost on one thread, with the all-thread and
+optimised-contraction variants beside it (rank 0, N=1 only; about a minute of host time).
 """
 import argparse
 import json
@@ -82,31 +86,66 @@ def _cpu_info():
     return model
 
 
-def cpu_baseline_sgplvm(w, budget_s=40.0):
-    """Faithful-cost oracle (naive einsums, scalar C psi loop, LU inverses) on the same
-    synthetic SGPLVM problem at N=512 and N=1024 rows, extrapolated linearly in N to
-    the full N (the per-point work is linear in N; the D*M^4 einsums are not)."""
+def _blas_info():
+    from threadpoolctl import threadpool_info
+    for p in threadpool_info():
+        if p.get('user_api') == 'blas':
+            return '%s %s (%s, %s)' % (p.get('internal_api'), p.get('version'), p.get('threading_layer'),
+                                       p.get('architecture'))
+    return 'unknown'
+
+
+def _host_threads():
+    """Threads this process may use on the host: the CPU affinity set, capped by
+    OMP_NUM_THREADS when the environment sets it (the GPU box gives each GPU 16)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get('OMP_NUM_THREADS')
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n
+
+
+def cpu_baseline_sgplvm(w, evals=3, variant_evals=2):
+    """The oracle on the benchmarked SGPLVM evaluation itself (N=4096, Q=8, D=32, M=128)
+    in four settings: faithful cost (the reference's naive einsums, scalar C psi loop, LU
+    inverses) and optimised contraction order (einsum optimize=True, vectorised psi),
+    each at one BLAS thread and at every host thread the process may use.  The faithful
+    single-thread median over `evals` evaluations is the reported value; the other three
+    settings take the median of `variant_evals`."""
     from threadpoolctl import threadpool_limits
     from oracle import aep_oracle as O
     from geepee_amd import workloads as W
-    O.FAITHFUL = True
-    try:
-        times = {}
-        with threadpool_limits(limits=1):
-            for n in (512, 1024):
-                y, p = W.sgplvm_problem(N=n, Q=w['Q'], D=w['D'], M=w['M'], seed=1)
-                t0 = time.perf_counter()
-                O.sgplvm_objective(y, p, w['M'], n, alpha=0.5)
-                times[n] = time.perf_counter() - t0
-    finally:
-        O.FAITHFUL = False
-    slope = (times[1024] - times[512]) / 512.0
-    t_full = times[1024] + slope * (w['N'] - 1024)
-    return {'value': 1.0 / t_full, 'unit': 'evals/s', 'cores': 1, 'kind': 'port',
-            'sample': ('oracle faithful-cost SGPLVM eval (numpy einsums without path optimisation, '
-                       'LU inverses, the scalar C psi loop), one thread, timed at N=512 (%.2fs) and '
-                       'N=1024 (%.2fs) and extrapolated linearly in N to N=%d (%.1fs/eval); %s; numpy %s'
-                       % (times[512], times[1024], w['N'], t_full, _cpu_info(), np.__version__))}
+    y, p = W.sgplvm_problem(N=w['N'], Q=w['Q'], D=w['D'], M=w['M'], seed=1)
+    nthreads = _host_threads()
+    runs = {}
+    for mode in ('faithful', 'optimised'):
+        for threads in (1, nthreads):
+            key = '%s_%dt' % (mode, threads)
+            if key in runs:
+                continue
+            O.FAITHFUL = mode == 'faithful'
+            ts = []
+            n_ev = evals if key == 'faithful_1t' else variant_evals
+            try:
+                with threadpool_limits(limits=threads):
+                    for _ in range(n_ev):
+                        t0 = time.perf_counter()
+                        O.sgplvm_objective(y, p, w['M'], w['N'], alpha=0.5)
+                        ts.append(time.perf_counter() - t0)
+            finally:
+                O.FAITHFUL = False
+            runs[key] = {'median_s': float(np.median(ts)), 'evals': n_ev, 'threads': threads,
+                         'evals_per_s': 1.0 / float(np.median(ts))}
+            print('cpu baseline %s: %s' % (key, json.dumps(runs[key])), file=sys.stderr, flush=True)
+    main = runs['faithful_1t']
+    return {'value': main['evals_per_s'], 'unit': 'evals/s', 'cores': 1, 'kind': 'port',
+            'sample': ('the oracle on the full SGPLVM evaluation (N=%d, Q=%d, D=%d, M=%d, alpha=0.5), '
+                       'median of %d evaluations, faithful cost (numpy einsums without path '
+                       'optimisation, LU inverses, the scalar C psi loop) on one thread'
+                       % (w['N'], w['Q'], w['D'], w['M'], evals)),
+            'host': {'cpu': _cpu_info(), 'nproc': os.cpu_count(), 'threads_available': nthreads,
+                     'blas': _blas_info(), 'numpy': np.__version__},
+            'variants': runs}
 
 
 def mm_layers(name, w):
@@ -224,6 +263,26 @@ def run_serving(args):
                         % args.requests}
 
 
+def run_c3_ratio(steps=20, warmup=3):
+    """The deep-GP evaluation (SDGPR 10-5-5-1, N=20000, M=100, minibatch 1000) on the plain
+    path (one kernel per reference operation, psi2 materialised, eager) and on the fused
+    path replayed as a hipGraph, same parameters and minibatch draws."""
+    import argparse as _ap
+    w = WORKLOADS['sdgpr']
+    out = {'workload': w['desc']}
+    for path, n in (('plain', max(5, steps // 2)), ('graph', steps)):
+        a = _ap.Namespace(path=path, steps=n, warmup=warmup, probe=None)
+        r = run_evals(a, 'sdgpr', w)
+        out[path + '_ms_per_eval'] = r['ms_per_step']
+        out[path + '_steps'] = n
+        print('c3 %s: %.3f ms/eval' % (path, r['ms_per_step']), file=sys.stderr, flush=True)
+    out['fused_graph_evals_per_s'] = 1e3 / out['graph_ms_per_eval']
+    out['plain_evals_per_s'] = 1e3 / out['plain_ms_per_eval']
+    out['ratio'] = out['plain_ms_per_eval'] / out['graph_ms_per_eval']
+    out['target_ratio'] = 5.0
+    return out
+
+
 def run_copy(args):
     """Calibration for HBM counters: gpx_lincomb as a plain copy (out = 1 * x) of
     64 Mi doubles, i.e. 512 MiB read and 512 MiB written per launch."""
@@ -254,8 +313,10 @@ def main():
     ap.add_argument('--probe', default=None,
                     help='C-ABI entry point timed for the roofline (default: the path\'s dominant one)')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-extras', action='store_true',
+                    help='skip the deep-GP fused/plain ratio and the serving run of the default line')
     ap.add_argument('--rates', type=float, nargs='+', default=[1e2, 1e3, 1e4])
-    ap.add_argument('--requests', type=int, default=1000)
+    ap.add_argument('--requests', type=int, default=600)
     ap.add_argument('--plain-requests', type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -295,6 +356,13 @@ def main():
     }
     if 'roofline' in res:
         line['roofline'] = res['roofline']
+    if not args.no_extras and args.workload == 'sgplvm':
+        # the other two halves of the headline metric, measured in the same run: the
+        # deep-GP fused+graph against plain ratio, and served predictions against one
+        # request at a time
+        line['c3_fused_vs_plain'] = run_c3_ratio()
+        r = run_serving(args)
+        line['serving'] = {'metric': r['metric'], 'workload': r['workload'], 'rates': r['rates']}
     if not args.no_cpu and args.workload == 'sgplvm':
         line['cpu_baseline'] = cpu_baseline_sgplvm(w)
     print(json.dumps(line), flush=True)

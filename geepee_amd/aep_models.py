@@ -9,12 +9,10 @@ or device tensors (``objective_function_device`` keeps everything resident).
 Minibatches are drawn on the host with numpy's legacy global RNG exactly as the
 reference does, so the same seed selects the same points.
 """
-import pickle
-
 import numpy as np
 import torch
 
-from . import _lib, config, dops
+from . import _lib, checkpoint, config, dops
 from .config import PROP_MM, PROP_MC
 from .dops import mm, lincomb, colsum
 from . import init
@@ -260,13 +258,13 @@ class Base_Model(object):
             self.fixed_params.append(params)
 
     def save_model(self, fname='/tmp/model.pickle'):
-        with open(fname, 'wb') as f:
-            pickle.dump({k: np.asarray(v) for k, v in self.get_hypers().items()}, f)
+        """Pickle get_hypers() as base_models.py:149-156 does."""
+        checkpoint.save_params(self.get_hypers(), fname)
 
     def load_model(self, fname='/tmp/model.pickle'):
-        with open(fname, 'rb') as f:
-            params = pickle.load(f)
-        self.update_hypers(params)
+        """base_models.py:158-165, reading the file as arrays only (no code from the
+        file runs; see checkpoint.load_params)."""
+        self.update_hypers(checkpoint.load_params(fname))
 
     def get_hypers(self):
         return {k: np.array(v, copy=True) for k, v in self._params_host.items()}

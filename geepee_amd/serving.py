@@ -19,7 +19,7 @@ import time
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, checkpoint
 from .aep_models import DeviceParams, _shapes
 from .layers import SGP_Layer
 
@@ -52,10 +52,14 @@ class ModelStore(object):
     """Resident SGPR / two-layer SDGPR models in posterior (predictive) form."""
 
     def __init__(self, specs):
-        """specs: list of dicts with kind ('sgpr' | 'sdgpr'), Din, Ms, N, hidden, params."""
+        """specs: list of dicts with kind ('sgpr' | 'sdgpr'), Din, Ms, N, hidden and either
+        params (a params dict) or checkpoint (the path of a file save_model wrote, read as
+        arrays only)."""
         _lib.require_device()
         _check_layout()
         self.tile = _lib.query('gpx_serve_tile')
+        specs = [sp if 'params' in sp else dict(sp, params=checkpoint.load_params(sp['checkpoint']))
+                 for sp in specs]
         self.specs = specs
         self.layers = []
         self._keep = []
