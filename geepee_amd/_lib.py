@@ -65,6 +65,8 @@ SIGNATURES = {
     'gpx_adam_step': (_i, [_l, _p, _p, _p, _p, _d, _d, _d, _d, _i, _p, _p, _p]),
     'gpx_fused_fwd_workspace': (_l, [_i, _i, _i, _i]),
     'gpx_fused_mm_fwd': (_i, [_i, _i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p, _p, _p]),
+    'gpx_fused_fwd_noalloc_max': (_l, []),
+    'gpx_fused_mm_fwd_noalloc': (_i, [_i, _i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p, _p]),
     'gpx_fused_bwd_workspace': (_l, [_i, _i, _i, _i, _i, _i]),
     'gpx_fused_mm_bwd': (_i, [_i, _i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p, _p, _p,
                               _p, _p, _p, _p, _p]),
@@ -82,6 +84,9 @@ SIGNATURES = {
     'gpx_psi_bwd': (_i, [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p,
                          _p, _p]),
 }
+
+
+GPX_EINVAL, GPX_EHIP, GPX_ETOOBIG = -1, -2, -3
 
 
 class GpxError(RuntimeError):
@@ -146,6 +151,9 @@ def call(name, *args):
     else:
         rc = getattr(load(), name)(*args)
     if rc != 0:
+        if rc == GPX_ETOOBIG:
+            raise GpxError('%s: the shape needs more workspace than the allocation-free '
+                           'entry point\'s fixed scratch holds (status %d)' % (name, rc))
         raise GpxError('%s failed with status %d' % (name, rc))
     return rc
 

@@ -6,6 +6,7 @@
 #define GPX_OK 0
 #define GPX_EINVAL (-1)
 #define GPX_EHIP (-2)
+#define GPX_ETOOBIG (-3)
 
 #define GPX_CHECK_LAUNCH()                                   \
   do {                                                       \

@@ -19,6 +19,8 @@
 // dpsi2_n = sum_d dv_nd Bsym[d].
 #include "gpx_common.h"
 
+#include <mutex>
+
 #define FQ_MAX 16
 #define FM_MAX 256
 #define FKS_MAX 9  // ceil((2*FQ_MAX+2)/4)
@@ -982,18 +984,23 @@ long gpx_fused_fwd_workspace(int N, int M, int D, int splits) { return (long)spl
 // psi1 (N x M), mout = psi1 Ahat^T, vout = sf2 + <Bsym[d], psi2_n> - mout^2 (N x D).
 // Bp is the symmetrised Bhat in pair-major layout (gpx_sym_pairmajor); `splits`
 // spreads the pair tiles over grid.y and `nsub` (1, 2 or 4) sets 16*nsub points per block.
-int gpx_fused_mm_fwd(int N, int M, int Q, int D, const double *mx, const double *vx, const double *z,
-                     const double *ls, const double *sf, const double *Ahat, const double *Bp,
-                     int splits, int nsub, double *psi1, double *mout, double *vout, double *work,
-                     void *stream) {
+static int fused_fwd_check(int N, int M, int Q, int D, const double *mx, const double *vx, const double *z,
+                           const double *ls, const double *sf, const double *Ahat, const double *Bp, int splits,
+                           int nsub, const double *psi1, const double *mout, const double *vout) {
   if (N < 1 || M < 1 || M > FM_MAX || Q < 1 || Q > FQ_MAX || D < 1 || D > 32 || splits < 1)
     return GPX_EINVAL;
   if (nsub != 1 && nsub != 2 && nsub != 4) return GPX_EINVAL;
-  if (!mx || !vx || !z || !ls || !sf || !Ahat || !Bp || !psi1 || !mout || !vout || !work)
-    return GPX_EINVAL;
-  hipStream_t st = (hipStream_t)stream;
+  if (!mx || !vx || !z || !ls || !sf || !Ahat || !Bp || !psi1 || !mout || !vout) return GPX_EINVAL;
   const int T = (M + 15) / 16, ntiles = T * (T + 1) / 2;
   if (splits > ntiles) return GPX_EINVAL;
+  return GPX_OK;
+}
+
+// The launches of gpx_fused_mm_fwd over a checked argument set and a workspace of
+// gpx_fused_fwd_workspace(N, M, D, splits) doubles.
+static int fused_fwd_launch(int N, int M, int Q, int D, const double *mx, const double *vx, const double *z,
+                            const double *ls, const double *sf, const double *Ahat, const double *Bp, int splits,
+                            int nsub, double *psi1, double *mout, double *vout, double *work, hipStream_t st) {
   double *Pab = work + (long)splits * N * D;
   hipLaunchKernelGGL(pair_p_kernel, dim3(gpx_cdiv((long)M * M, 256)), dim3(256), 0, st, M, Q, z, ls, Pab);
   GPX_CHECK_LAUNCH();
@@ -1023,6 +1030,67 @@ int gpx_fused_mm_fwd(int N, int M, int Q, int D, const double *mx, const double 
                      Ahat, (const double *)work, splits, psi1, mout, vout);
   GPX_CHECK_LAUNCH();
   return GPX_OK;
+}
+
+int gpx_fused_mm_fwd(int N, int M, int Q, int D, const double *mx, const double *vx, const double *z,
+                     const double *ls, const double *sf, const double *Ahat, const double *Bp,
+                     int splits, int nsub, double *psi1, double *mout, double *vout, double *work,
+                     void *stream) {
+  const int rc = fused_fwd_check(N, M, Q, D, mx, vx, z, ls, sf, Ahat, Bp, splits, nsub, psi1, mout, vout);
+  if (rc != GPX_OK) return rc;
+  if (!work) return GPX_EINVAL;
+  return fused_fwd_launch(N, M, Q, D, mx, vx, z, ls, sf, Ahat, Bp, splits, nsub, psi1, mout, vout, work,
+                          (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free forward.  The workspace of gpx_fused_mm_fwd (per-split partial sums
+// and the pair constants P_ab) lives in a file-scope device array instead of caller
+// memory.  Each GPU holds its own copy of the array from module load to process exit;
+// host code never passes the array's name to a kernel (in host code that name is a
+// host-side stand-in), only the device address hipGetSymbolAddress returns for the
+// current device, looked up once per device and kept.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr long kNoAllocScratch = 4L << 20;  // doubles (32 MiB)
+__device__ __align__(256) double scratch[kNoAllocScratch];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (NULL on failure).
+double *scratch_on_current_device() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
+extern "C" {
+
+long gpx_fused_fwd_noalloc_max(void) { return kNoAllocScratch; }
+
+int gpx_fused_mm_fwd_noalloc(int N, int M, int Q, int D, const double *mx, const double *vx, const double *z,
+                             const double *ls, const double *sf, const double *Ahat, const double *Bp, int splits,
+                             int nsub, double *psi1, double *mout, double *vout, void *stream) {
+  const int rc = fused_fwd_check(N, M, Q, D, mx, vx, z, ls, sf, Ahat, Bp, splits, nsub, psi1, mout, vout);
+  if (rc != GPX_OK) return rc;
+  if (gpx_fused_fwd_workspace(N, M, D, splits) > kNoAllocScratch) return GPX_ETOOBIG;
+  double *work = scratch_on_current_device();
+  if (!work) return GPX_EHIP;
+  return fused_fwd_launch(N, M, Q, D, mx, vx, z, ls, sf, Ahat, Bp, splits, nsub, psi1, mout, vout, work,
+                          (hipStream_t)stream);
 }
 
 }  // extern "C"

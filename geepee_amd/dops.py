@@ -307,6 +307,19 @@ def fused_mm_fwd(mx, vx, z, ls, sf, Ahat, Bp, plan=None):
     return psi1, mout, vout
 
 
+def fused_mm_fwd_noalloc(mx, vx, z, ls, sf, Ahat, Bp, psi1, mout, vout, plan=None):
+    """fused_mm_fwd into caller-provided outputs through the allocation-free entry point
+    (its workspace is the library's fixed per-GPU scratch; overlapping calls on one GPU
+    race)."""
+    N, Q = mx.shape
+    M = z.shape[0]
+    D = Ahat.shape[0]
+    plan = plan or fused_plan(N, M, Q, D)
+    call('gpx_fused_mm_fwd_noalloc', N, M, Q, D, P(mx), P(vx), P(z), P(ls), P(sf), P(Ahat), P(Bp),
+         plan['splits_f'], plan['nsub'], P(psi1), P(mout), P(vout), S())
+    return psi1, mout, vout
+
+
 def fused_mm_bwd(mx, vx, z, ls, sf, Ahat, Bp, psi1, dm_all, dv, plan=None):
     N, Q = mx.shape
     M = z.shape[0]

@@ -26,6 +26,7 @@ extern "C" {
 #define GPX_OK 0
 #define GPX_EINVAL (-1)
 #define GPX_EHIP (-2)
+#define GPX_ETOOBIG (-3) /* the allocation-free entry point's scratch is too small */
 
 /* ---- dense algebra -------------------------------------------------------- */
 
@@ -235,6 +236,20 @@ int gpx_fused_mm_fwd(int N, int M, int Q, int D, const double *mx, const double 
                      const double *ls, const double *sf, const double *Ahat, const double *Bp,
                      int splits, int nsub, double *psi1, double *mout, double *vout, double *work,
                      void *stream);
+
+/* gpx_fused_mm_fwd without the `work` argument and without any allocation: the workspace
+ * is a file-scope device array of gpx_fused_fwd_noalloc_max() doubles (32 MiB), one copy
+ * per GPU, held until the process ends.  No counterpart in the reference: this entry is for
+ * callers that cannot allocate device memory (code under hipGraph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Returns GPX_ETOOBIG, before any launch, when
+ * gpx_fused_fwd_workspace(N, M, D, splits) exceeds the array.  Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (concurrent streams, host threads) race;
+ * the kernels write every workspace entry they read, so no clearing is needed. */
+long gpx_fused_fwd_noalloc_max(void);
+int gpx_fused_mm_fwd_noalloc(int N, int M, int Q, int D, const double *mx, const double *vx,
+                             const double *z, const double *ls, const double *sf, const double *Ahat,
+                             const double *Bp, int splits, int nsub, double *psi1, double *mout,
+                             double *vout, void *stream);
 
 /* Adjoint of the same layer without psi2 or dL/dpsi2 in HBM: from dm_all = dm - 2 dv m and dv
  * (N x D) produces dBhat = sum_n dv_nd psi2_n (D x M x M), and the psi adjoints dsf2, dell, dZ,

@@ -61,3 +61,16 @@ def test_library_has_no_undefined_gpx_symbols():
     out = subprocess.run(['nm', '-D', '--undefined-only', path], capture_output=True, text=True, check=True)
     undefined = [l.split()[-1] for l in out.stdout.splitlines() if 'gpx_' in l]
     assert not undefined, undefined
+
+
+def test_noalloc_rejects_oversize_before_any_launch():
+    """The allocation-free fused forward checks its workspace against the fixed scratch on
+    the host: an oversized shape returns GPX_ETOOBIG (-3) without a device call."""
+    L = _lib.load()
+    cap = L.gpx_fused_fwd_noalloc_max()
+    assert cap == 4 << 20
+    N, M, Q, D, splits = 40000, 128, 8, 32, 16
+    assert L.gpx_fused_fwd_workspace(N, M, D, splits) > cap
+    assert L.gpx_fused_mm_fwd_noalloc(N, M, Q, D, 1, 1, 1, 1, 1, 1, 1, splits, 1, 1, 1, 1, None) == -3
+    # invalid arguments are still GPX_EINVAL
+    assert L.gpx_fused_mm_fwd_noalloc(N, M, 99, D, 1, 1, 1, 1, 1, 1, 1, splits, 1, 1, 1, 1, None) == -1
