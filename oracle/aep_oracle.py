@@ -34,7 +34,10 @@ import numpy as np
 from scipy.spatial.distance import cdist
 
 JITTER = 1e-5          # geepee/config.py:11
+GH_DEGREE = 10         # geepee/config.py:12
 PROP_MM = 'MM'         # geepee/config.py:13
+PROP_MC = 'MC'         # geepee/config.py:15
+MC_NO_SAMPLES = 5      # geepee/config.py:16
 
 FAITHFUL = False
 # 'lu': np.linalg.inv as the reference calls it.  'chol': a Cholesky-based inverse,
@@ -492,6 +495,74 @@ def bwd_mm(L, m, dm, dv, psi1, psi2, mx, vx, alpha):
     return grad, {'mx': dmx, 'vx': dvx}
 
 
+def fwd_mc_cav(L, mx, vx, eps):
+    """_forward_prop_random_thru_cav_mc (aep_models.py:160-180): the deterministic
+    cavity propagation at x = eps sqrt(vx) + mx for standard-normal draws eps (S, B, Din),
+    which the reference takes from np.random.randn(MC_NO_SAMPLES, B, Din).  Returns the
+    (S, B, Dout) means and variances and the stacked kfu (S*B, M) and x (S*B, Din)."""
+    S, B, Din = eps.shape
+    x = eps * np.sqrt(vx) + mx
+    x_stk = x.reshape(S * B, Din)
+    m_stk, v_stk, kfu_stk = fwd_det_cav(L, x_stk)
+    return m_stk.reshape(S, B, -1), v_stk.reshape(S, B, -1), kfu_stk, x_stk
+
+
+def bwd_mc(L, dm, dv, kfu, x, alpha):
+    """backprop_grads_lvm_mc (aep_models.py:307-410): hyper grads of a layer evaluated at
+    the stacked Monte Carlo inputs x (S*B rows; dm, dv already scaled, any shape with
+    S*B*Dout entries), and the grads w.r.t. those inputs."""
+    N, M = L.N, L.M
+    ell = np.exp(L.ls)
+    sf2 = np.exp(2 * L.sf)
+    dm = dm.reshape(kfu.shape[0], L.Dout)
+    dv = dv.reshape(kfu.shape[0], L.Dout)
+    beta = (N - alpha) * 1.0 / N
+    scale_post = N * 1.0 / alpha - 1.0
+    scale_cav = -N * 1.0 / alpha
+    kfuKuuinv = np.dot(kfu, L.Kuuinv)
+    dkfu = ein('nd,dm->nm', dm, L.Ahat) + 2 * ein('nd,dab,na->nb', dv, L.Bhat_det, kfu)
+    dsf2, dls, dzu, dx = kfu_grads(dkfu, kfu, sf2, ell, L.zu, x, grad_x=True)
+    dls = dls * ell
+    dsf2 = dsf2 + np.sum(dv)
+    dsf = 2 * sf2 * dsf2
+    SKf = ein('dab,nb->nda', L.Suhat, kfuKuuinv)
+    dSinv = -ein('nda,nd,ndb->dab', SKf, dv, SKf) - ein('nda,nd,db->dab', SKf, dm, L.muhat)
+    dSinvM = ein('nda,nd->da', SKf, dm)
+    dtheta1 = -0.5 * scale_post * L.Splusmm - 0.5 * scale_cav * beta * L.Splusmmhat + beta * dSinv
+    dtheta2 = scale_post * L.mu + scale_cav * beta * L.muhat + beta * dSinvM
+    dAhat = ein('nd,nm->dm', dm, kfu)
+    dKi_m = ein('da,db->ab', dAhat, L.muhat)
+    KuuinvSmmd = ein('ab,dbc->dac', L.Kuuinv, L.Suhat)
+    dBhat = ein('nd,na,nb->dab', dv, kfu, kfu)
+    dKi_v1 = 2 * ein('dab,dac->bc', KuuinvSmmd, dBhat) - np.sum(dBhat, axis=0)
+    dKuuinv = dKi_m + dKi_v1 + np.sum(dSinv, axis=0)
+    Minner = scale_post * np.sum(L.Splusmm, axis=0) + scale_cav * np.sum(L.Splusmmhat, axis=0) \
+        - 2.0 * dKuuinv
+    M_all = 0.5 * (L.Dout * L.Kuuinv + np.dot(L.Kuuinv, np.dot(Minner, L.Kuuinv)))
+    g_sf, g_ls, g_z = dtrace_MKzz(L.ls, L.sf, L.zu, M_all, L.Kuu - np.diag(JITTER * np.ones(M)))
+    grad = {'sf': dsf + 2 * g_sf, 'ls': dls + 2 * g_ls, 'zu': dzu + g_z,
+            'eta1_R': pack_dR(L.R, dtheta1), 'eta2': dtheta2}
+    return grad, dx
+
+
+def reparam_bwd(dx, vx, eps):
+    """backprop_grads_reparam (base_models.py:373-388): grads w.r.t. the input means and
+    variances of x = eps sqrt(vx) + mx."""
+    dx = dx.reshape(eps.shape)
+    # zero-variance columns (control inputs) divide by zero here as in the reference; their
+    # gradients are discarded by the callers
+    with np.errstate(divide='ignore', invalid='ignore'):
+        return np.sum(dx, axis=0), np.sum(dx * eps, axis=0) / (2 * np.sqrt(vx))
+
+
+def fwd_mc_post(L, mx, vx, eps):
+    """_forward_prop_random_thru_post_mc (base_models.py:309-332) for draws eps."""
+    S, B, Din = eps.shape
+    x = (eps * np.sqrt(vx) + mx).reshape(S * B, Din)
+    m, v = fwd_det_post(L, x)
+    return m.reshape(S, B, -1), v.reshape(S, B, -1)
+
+
 # ----------------------------------------------------------------------------
 # lik_layers.py
 # ----------------------------------------------------------------------------
@@ -509,6 +580,63 @@ def gauss_logZ(sn, mout, vout, y, alpha):
     dm = (y - mout) / vt
     dv = -0.5 / vt + 0.5 * (y - mout) ** 2 / vt ** 2
     return float(np.ravel(logZ)[0]), dm, dv
+
+
+def _lse_weights(logZ):
+    """Per-sample log-sum-exp over axis 0 minus log S, and the softmax weights (the
+    3-D branches of lik_layers.py:139-152 and aep_models.py:1356-1367)."""
+    mx = np.max(logZ, axis=0)
+    e = np.exp(logZ - mx)
+    se = np.sum(e, axis=0)
+    return mx + np.log(se) - np.log(logZ.shape[0]), e / se
+
+
+def gauss_logZ_mc(sn, mout, vout, y, alpha):
+    """Gauss_Layer.compute_log_Z, 3-D branch (lik_layers.py:139-152): a log-mean-exp over
+    the S Monte Carlo samples of the per-sample tilted normaliser."""
+    sn2 = np.exp(2.0 * sn)
+    vt = vout + sn2 / alpha
+    lz = -0.5 * (np.log(2 * np.pi * vt) + (y - mout) ** 2 / vt)
+    lz = lz + (0.5 * np.log(2 * np.pi * sn2 / alpha) - 0.5 * alpha * np.log(2 * np.pi * sn2))
+    lse, w = _lse_weights(lz)
+    dm = w * (y - mout) / vt
+    dv = w * (-0.5 / vt + 0.5 * (y - mout) ** 2 / vt ** 2)
+    return float(np.sum(lse)), dm, dv
+
+
+def probit_logZ(mout, vout, y, alpha):
+    """Probit_Layer.compute_log_Z (lik_layers.py:303-411), 2-D (B, D) or 3-D (S, B, D)
+    inputs: the closed form at alpha = 1, Gauss-Hermite quadrature of degree GH_DEGREE
+    otherwise, and the small constants the reference adds (1e-16, or 1e-8 in the 2-D
+    quadrature branch, to Z and to the gradients)."""
+    from scipy import special
+    three = mout.ndim == 3
+    if alpha == 1.0:
+        t = y * mout / np.sqrt(1 + vout)
+        Z = 0.5 * (1 + special.erf(t / np.sqrt(2)))
+        eps = 1e-16
+        lz = np.log(Z + eps)
+        g = 1 / (Z + eps) / np.sqrt(2 * np.pi) * np.exp(-t ** 2.0 / 2)
+        dm = g * (y / np.sqrt(1 + vout))
+        dv = g * (-0.5 * y * mout / (1 + vout) ** 1.5)
+        if not three:
+            return float(np.sum(lz)), dm, dv
+        lse, w = _lse_weights(lz)
+        return float(np.sum(lse)), w * dm, w * dv
+    gx, gw = np.polynomial.hermite.hermgauss(GH_DEGREE)
+    shape = (-1,) + (1,) * mout.ndim
+    gx, gw = gx.reshape(shape), gw.reshape(shape)
+    eps = 1e-16 if three else 1e-8
+    ts = gx * np.sqrt(2 * vout)[None] + mout[None]
+    pdfs = 0.5 * (1 + special.erf(y * ts / np.sqrt(2))) + eps
+    Zt = np.sum(pdfs ** alpha * gw, axis=0) / np.sqrt(np.pi)
+    a = pdfs ** (alpha - 1.0) * np.exp(-ts ** 2 / 2)
+    dZdm = np.sum(gw * a, axis=0) * y * alpha / np.pi / np.sqrt(2)
+    dZdv = np.sum(gw * (a * gx), axis=0) * y * alpha / np.pi / np.sqrt(2) / np.sqrt(2 * vout)
+    if not three:
+        return float(np.sum(np.log(Zt))), dZdm / Zt + eps, dZdv / Zt + eps
+    lse, w = _lse_weights(np.log(Zt))
+    return float(np.sum(lse)), w * dZdm / Zt + eps, w * dZdv / Zt + eps
 
 
 def gauss_dsn(sn, dv, B, D, alpha, scale):
@@ -566,7 +694,18 @@ def _finish(energy, grads, fixed, divide_by):
     return float(np.ravel(energy)[0]), grads
 
 
-def sgpr_objective(x, y, params, M, mb_size, alpha=1.0, nat_param=True, fixed=(), idxs=None):
+def _lik(name, params, mout, vout, y, alpha, B, scale):
+    """(logZ, dm, dv, lik grads) of the likelihood layer: Gauss_Layer (with its sn
+    gradient) or Probit_Layer (no hyper-parameters; lik_layers.py:285-471)."""
+    if name.lower() == 'probit':
+        logZ, dm, dv = probit_logZ(mout, vout, y, alpha)
+        return logZ, dm, dv, {}
+    logZ, dm, dv = gauss_logZ(params['sn'], mout, vout, y, alpha)
+    return logZ, dm, dv, {'sn': gauss_dsn(params['sn'], dv, B, y.shape[1], alpha, scale)}
+
+
+def sgpr_objective(x, y, params, M, mb_size, alpha=1.0, nat_param=True, fixed=(), idxs=None,
+                   lik='Gaussian'):
     """SGPR.objective_function (aep_models.py:610-667).  Draws the minibatch with
     np.random.choice(N, mb_size, replace=False) like :628 unless ``idxs`` is given."""
     N = y.shape[0]
@@ -578,14 +717,14 @@ def sgpr_objective(x, y, params, M, mb_size, alpha=1.0, nat_param=True, fixed=()
     L = layer_update(params, '', M, N, nat_param)
     layer_cavity(L, alpha)
     mout, vout, kfu = fwd_det_cav(L, xb)
-    logZ, dm, dv = gauss_logZ(params['sn'], mout, vout, yb, alpha)
+    logZ, dm, dv, glik = _lik(lik, params, mout, vout, yb, alpha, B, scale_logZ)
     grads = bwd_det(L, scale_logZ * dm, scale_logZ * dv, kfu, xb, alpha)
-    grads['sn'] = gauss_dsn(params['sn'], dv, B, y.shape[1], alpha, scale_logZ)
+    grads.update(glik)
     energy = scale_logZ * logZ + layer_phi(L, alpha)
     return _finish(energy, grads, fixed, N)
 
 
-def sdgpr_objective(x, y, params, Ms, mb_size, alpha=1.0, fixed=(), idxs=None):
+def sdgpr_objective(x, y, params, Ms, mb_size, alpha=1.0, fixed=(), idxs=None, lik='Gaussian'):
     """SDGPR.objective_function (aep_models.py:895-988); layer i uses suffix '_i'."""
     N = y.shape[0]
     Lnum = len(Ms)
@@ -605,7 +744,7 @@ def sdgpr_objective(x, y, params, Ms, mb_size, alpha=1.0, fixed=(), idxs=None):
         else:
             m, v, a, b = fwd_mm_cav(Li, ms[i - 1], vs[i - 1])
             ms.append(m); vs.append(v); p1.append(a); p2.append(b)
-    logZ, dm, dv = gauss_logZ(params['sn'], ms[-1], vs[-1], yb, alpha)
+    logZ, dm, dv, glik = _lik(lik, params, ms[-1], vs[-1], yb, alpha, B, scale_logZ)
     dmi, dvi = scale_logZ * dm, scale_logZ * dv
     grads = {}
     for i in range(Lnum - 1, -1, -1):
@@ -617,13 +756,13 @@ def sdgpr_objective(x, y, params, Ms, mb_size, alpha=1.0, fixed=(), idxs=None):
             dmi, dvi = gin['mx'], gin['vx']
         for k, val in g.items():
             grads[k + '_%d' % i] = val
-    grads['sn'] = gauss_dsn(params['sn'], dv, B, y.shape[1], alpha, scale_logZ)
+    grads.update(glik)
     energy = scale_logZ * logZ + sum(layer_phi(Li, alpha) for Li in layers)
     return _finish(energy, grads, fixed, N)
 
 
 def sgplvm_objective(y, params, M, mb_size, alpha=1.0, prior_mean=0.0, prior_var=1.0,
-                     nat_param=True, fixed=(), idxs=None):
+                     nat_param=True, fixed=(), idxs=None, prop_mode=PROP_MM, eps=None):
     """SGPLVM.objective_function (aep_models.py:694-815) with x-factor helpers
     (:817-867) and Base_SGPLVM.update_hypers / compute_posterior_grad_x
     (base_models.py:895-929).  The energy is NOT divided by N (:808-815)."""
@@ -658,9 +797,19 @@ def sgplvm_objective(y, params, M, mb_size, alpha=1.0, prior_mean=0.0, prior_var
     mcav = cav_t1 / cav_t2
     vpost = 1.0 / x_post_2[idxs]
     mpost = x_post_1[idxs] / x_post_2[idxs]
-    mout, vout, psi1, psi2 = fwd_mm_cav(L, mcav, vcav)
-    logZ, dm, dv = gauss_logZ(params['sn'], mout, vout, yb, alpha)
-    grads, gin = bwd_mm(L, mout, scale_logZ * dm, scale_logZ * dv, psi1, psi2, mcav, vcav, alpha)
+    if prop_mode == PROP_MC:
+        # :745-760; the draws follow the minibatch draw on the legacy global RNG
+        if eps is None:
+            eps = np.random.randn(MC_NO_SAMPLES, B, Din)
+        m3, v3, kfu_s, x_s = fwd_mc_cav(L, mcav, vcav, eps)
+        logZ, dm, dv = gauss_logZ_mc(params['sn'], m3, v3, yb, alpha)
+        grads, dx = bwd_mc(L, scale_logZ * dm, scale_logZ * dv, kfu_s, x_s, alpha)
+        gmx, gvx = reparam_bwd(dx, vcav, eps)
+        gin = {'mx': gmx, 'vx': gvx}
+    else:
+        mout, vout, psi1, psi2 = fwd_mm_cav(L, mcav, vcav)
+        logZ, dm, dv = gauss_logZ(params['sn'], mout, vout, yb, alpha)
+        grads, gin = bwd_mm(L, mout, scale_logZ * dm, scale_logZ * dv, psi1, psi2, mcav, vcav, alpha)
     grads['sn'] = gauss_dsn(params['sn'], dv, B, y.shape[1], alpha, scale_logZ)
 
     def phi_x(m, v):   # compute_phi_x (:863-867)
@@ -722,9 +871,26 @@ def transition_tilted(sn, m_prop, v_prop, m_t, v_t, alpha, scale, Din):
     return logZ, -dmt, dvt, dmt, dvt, dsn
 
 
+def transition_tilted_mc(sn, m_prop, v_prop, m_t, v_t, alpha, scale, Din):
+    """SGPSSM.compute_transition_tilted, 3-D branch (aep_models.py:1344-1372): m_prop,
+    v_prop are (S, B, D) Monte Carlo propagations, m_t, v_t the (B, D) cavities.
+    Returns (logZ*scale, dmprop, dvprop, dmt, dvt, dsn), all scaled."""
+    sn2 = np.exp(2 * sn)
+    v_sum = v_t + v_prop + sn2 / alpha
+    m_diff = m_t - m_prop
+    lz = -0.5 * m_diff ** 2 / v_sum - 0.5 * np.log(1 + alpha * (v_t + v_prop) / sn2) \
+        - 0.5 * alpha * np.log(2 * np.pi * sn2)
+    lse, w = _lse_weights(lz)
+    dl = scale * w
+    dl_dm = dl * m_diff / v_sum
+    dl_dv = dl * (-0.5 / v_sum + 0.5 * m_diff ** 2 / v_sum ** 2)
+    dsn = np.sum(dl_dv) * 2 * sn2 / alpha + scale * m_prop.shape[1] * Din * (1 - alpha)
+    return scale * np.sum(lse), dl_dm, dl_dv, -np.sum(dl_dm, axis=0), np.sum(dl_dv, axis=0), dsn
+
+
 def sgpssm_objective(y, params, M, Din, mb_size, alpha=1.0, prior_mean=0.0, prior_var=1.0,
                      x_control=None, gp_emi=False, control_to_emi=True, fixed=(),
-                     start_idx=None):
+                     start_idx=None, prop_mode=PROP_MM, eps=None):
     """SGPSSM.objective_function (aep_models.py:1024-1206) with Base_SGPSSM.update_hypers
     (base_models.py:1705-1728) and the x-factor helpers (aep_models.py:1208-1437).
     The dynamics layer is built with N-1 training points (:1018)."""
@@ -779,19 +945,40 @@ def sgpssm_objective(y, params, M, Din, mb_size, alpha=1.0, prior_mean=0.0, prio
         cav_up_vc = np.hstack((cav_up_v, np.zeros((Be, Dcon_emi))))
     else:
         cav_up_mc, cav_up_vc = cav_up_m, cav_up_v
-    mprop, vprop, psi1, psi2 = fwd_mm_cav(dyn, cav_tm1_mc, cav_tm1_vc)
-    logZ_dyn, dmprop, dvprop, dmt, dvt, dsn = transition_tilted(
-        sn, mprop, vprop, cav_t_m, cav_t_v, alpha, scale_dyn, Din)
-    g_dyn, gin_dyn = bwd_mm(dyn, mprop, dmprop, dvprop, psi1, psi2, cav_tm1_mc, cav_tm1_vc, alpha)
+    mc = prop_mode == PROP_MC
+    if mc:
+        # :1114-1143; dynamics draws first, then the GP emission's
+        eps_dyn = np.random.randn(MC_NO_SAMPLES, Bd, Din + Dcon) if eps is None else eps[0]
+        m3, v3, kfu_s, x_s = fwd_mc_cav(dyn, cav_tm1_mc, cav_tm1_vc, eps_dyn)
+        logZ_dyn, dmprop, dvprop, dmt, dvt, dsn = transition_tilted_mc(
+            sn, m3, v3, cav_t_m, cav_t_v, alpha, scale_dyn, Din)
+        g_dyn, dx = bwd_mc(dyn, dmprop, dvprop, kfu_s, x_s, alpha)
+        gmx, gvx = reparam_bwd(dx, cav_tm1_vc, eps_dyn)
+        gin_dyn = {'mx': gmx, 'vx': gvx}
+    else:
+        mprop, vprop, psi1, psi2 = fwd_mm_cav(dyn, cav_tm1_mc, cav_tm1_vc)
+        logZ_dyn, dmprop, dvprop, dmt, dvt, dsn = transition_tilted(
+            sn, mprop, vprop, cav_t_m, cav_t_v, alpha, scale_dyn, Din)
+        g_dyn, gin_dyn = bwd_mm(dyn, mprop, dmprop, dvprop, psi1, psi2, cav_tm1_mc, cav_tm1_vc,
+                                alpha)
     grads = {'sn': dsn}
     for k, v in g_dyn.items():
         grads[k + '_dynamic'] = v
-    if gp_emi:
+    if gp_emi and mc:
+        eps_emi = np.random.randn(MC_NO_SAMPLES, Be, Din + Dcon_emi) if eps is None else eps[1]
+        m3, v3, kfu_s, x_s = fwd_mc_cav(emi, cav_up_mc, cav_up_vc, eps_emi)
+        logZ_e, dm, dv = gauss_logZ_mc(params['sn_emission'], m3, v3, yb, alpha)
+        logZ_emi = scale_emi * logZ_e
+        g_emi, dx = bwd_mc(emi, scale_emi * dm, scale_emi * dv, kfu_s, x_s, alpha)
+        gmx, gvx = reparam_bwd(dx, cav_up_vc, eps_emi)
+        gin_emi = {'mx': gmx, 'vx': gvx}
+    elif gp_emi:
         mout, vout, e1, e2 = fwd_mm_cav(emi, cav_up_mc, cav_up_vc)
         logZ_e, dm, dv = gauss_logZ(params['sn_emission'], mout, vout, yb, alpha)
         logZ_emi = scale_emi * logZ_e
         g_emi, gin_emi = bwd_mm(emi, mout, scale_emi * dm, scale_emi * dv, e1, e2,
                                 cav_up_mc, cav_up_vc, alpha)
+    if gp_emi:
         for k, v in g_emi.items():
             grads[k + '_emission'] = v
         grads['sn_emission'] = gauss_dsn(params['sn_emission'], dv, Be, Dout, alpha, scale_emi)
