@@ -41,6 +41,11 @@ SIGNATURES = {
     'gpx_mm_var': (_i, [_l, _p, _p, _p, _p, _p]),
     'gpx_batched_dot': (_i, [_i, _i, _p, _p, _p, _p]),
     'gpx_gauss_tilted': (_i, [_i, _i, _p, _p, _p, _p, _d, _d, _p, _p, _p, _p, _p]),
+    'gpx_gauss_tilted_mc': (_i, [_i, _i, _i, _p, _p, _p, _p, _d, _d, _p, _p, _p, _p]),
+    'gpx_probit_tilted': (_i, [_i, _i, _i, _i, _p, _p, _p, _d, _d, _p, _p, _p, _p, _p]),
+    'gpx_mc_inputs': (_i, [_i, _i, _i, _p, _p, _p, _p, _p]),
+    'gpx_mc_reparam_bwd': (_i, [_i, _i, _i, _p, _p, _p, _p, _p, _p]),
+    'gpx_transition_tilted_mc': (_i, [_i, _i, _i, _p, _p, _p, _p, _p, _d, _d, _p, _p, _p, _p, _p, _p]),
     'gpx_transition_tilted': (_i, [_i, _i, _p, _p, _p, _p, _p, _d, _d, _p, _p, _p, _p, _p, _p]),
     'gpx_emission_workspace': (_l, [_i, _i, _i]),
     'gpx_emission_tilted': (_i, [_i, _i, _i, _p, _p, _p, _i, _p, _i, _p, _i, _d, _d, _p, _p, _p,

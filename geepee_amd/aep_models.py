@@ -71,8 +71,50 @@ class Gauss_Layer(object):
         ones = torch.ones(1, dtype=F64, device=vf.device)
         return mf, lincomb(out, 1.0, vf, sn2 / alpha, ones)
 
+    def compute_log_Z_mc(self, m3, v3, y, alpha=1.0, scale=1.0):
+        """3-D branch (lik_layers.py:139-152): (S, B, D) Monte Carlo propagations;
+        returns (dm, dv, scal) scaled like compute_log_Z."""
+        return dops.gauss_tilted_mc(m3, v3, y, self.sn, alpha, scale)
+
+    def hyper_grads(self, scal, key_suffix=''):
+        return {'sn' + key_suffix: scal[1:2]}
+
     def update_hypers(self, params, key_suffix=''):
         self.sn = params['sn' + key_suffix]
+
+
+class Probit_Layer(object):
+    """Probit likelihood for y in {-1, +1} (geepee/lik_layers.py:285-471) on the GPU:
+    closed form at alpha = 1, 10-point Gauss-Hermite quadrature otherwise.  It has no
+    hyper-parameters."""
+
+    def __init__(self, N, D):
+        self.N = N
+        self.D = D
+
+    def compute_log_Z(self, mout, vout, y, alpha=1.0, scale=1.0, want_dmall=False):
+        return dops.probit_tilted(mout, vout, y, alpha, scale, want_dmall)
+
+    def compute_log_Z_mc(self, m3, v3, y, alpha=1.0, scale=1.0):
+        dm, dv, _, scal = dops.probit_tilted(m3, v3, y, alpha, scale)
+        return dm, dv, scal
+
+    def hyper_grads(self, scal, key_suffix=''):
+        return {}
+
+    def output_probabilistic(self, mf, vf, alpha=1.0):
+        raise NotImplementedError('TODO: return probablity of y=1')   # lik_layers.py:457-471
+
+    def update_hypers(self, params, key_suffix=''):
+        pass
+
+
+def _make_lik(lik, N, D):
+    if lik.lower() == 'gaussian':
+        return Gauss_Layer(N, D)
+    if lik.lower() == 'probit':
+        return Probit_Layer(N, D)
+    raise NotImplementedError('likelihood not implemented')
 
 
 class Base_Model(object):
@@ -104,16 +146,25 @@ class Base_Model(object):
     def objective_function(self, params, mb_size, alpha=1.0, prop_mode=PROP_MM):
         """(energy, grad dict) as numpy, the reference contract (aep_models.py:610, 694,
         895, 1024)."""
-        _check_mode(prop_mode)
-        energy, grads, div = self._objective(params, mb_size, alpha)
+        energy, grads, div = self._objective(params, mb_size, alpha, prop_mode=self._mode(prop_mode))
         return self._finish(energy, grads, params, div)
 
     def objective_function_device(self, params, mb_size, alpha=1.0, prop_mode=PROP_MM):
         """Device-resident variant: params is a dict of device tensors; returns one flat
         device vector [energy, grads in sorted-key order] without any host copy."""
-        _check_mode(prop_mode)
-        energy, grads, div = self._objective(params, mb_size, alpha)
+        energy, grads, div = self._objective(params, mb_size, alpha, prop_mode=self._mode(prop_mode))
         return self._flat_device(energy, grads, div)
+
+    # the models whose objective branches on prop_mode (SGPLVM, SGPSSM) accept MM and MC;
+    # SGPR and SDGPR ignore it, as the reference's do (aep_models.py:611, 895)
+    _prop_modes = None
+
+    def _mode(self, prop_mode):
+        if self._prop_modes is None:
+            return PROP_MM
+        if prop_mode not in self._prop_modes:
+            raise NotImplementedError('propagation mode %s not implemented' % prop_mode)
+        return prop_mode
 
     def _flat_device(self, energy_dev, grads_dev, divide_by):
         keys = sorted(grads_dev)
@@ -271,15 +322,14 @@ class Base_Model(object):
 
 
 def _check_mode(prop_mode):
-    if prop_mode == PROP_MC:
-        raise NotImplementedError('Monte Carlo propagation is not on the GPU path')
-    if prop_mode != PROP_MM:
-        raise NotImplementedError('propagation mode not implemented')
+    if prop_mode not in (PROP_MM, PROP_MC):
+        raise NotImplementedError('prop_mode %s unknown' % prop_mode)
 
 
-def _check_lik(lik):
-    if lik.lower() != 'gaussian':
-        raise NotImplementedError('likelihood %s is not on the GPU path' % lik)
+def _mc_draws(*shape):
+    """Standard-normal draws in the reference's call (np.random.randn on the legacy global
+    RNG, aep_models.py:171), uploaded in one copy."""
+    return torch.as_tensor(np.random.randn(*shape), device='cuda')
 
 
 class SGPR(Base_Model):
@@ -287,14 +337,13 @@ class SGPR(Base_Model):
 
     def __init__(self, x_train, y_train, no_pseudo, lik='Gaussian', nat_param=True):
         super(SGPR, self).__init__(y_train)
-        _check_lik(lik)
         self.x_train = np.asarray(x_train, dtype=np.float64)
         self.Din = self.x_train.shape[1]
         self.Dout = self.y_train.shape[1]
         self.M = no_pseudo
         self.nat_param = nat_param
         self.x_dev = torch.as_tensor(self.x_train, device='cuda')
-        self.lik_layer = Gauss_Layer(self.N, self.Dout)
+        self.lik_layer = _make_lik(lik, self.N, self.Dout)
         self.sgp_layer = SGP_Layer(self.N, self.Din, self.Dout, self.M, nat_param)
 
     def update_hypers(self, params):
@@ -303,7 +352,7 @@ class SGPR(Base_Model):
         self.lik_layer.update_hypers(dp)
         return dp
 
-    def _objective(self, params, mb_size, alpha, idx=None):
+    def _objective(self, params, mb_size, alpha, idx=None, prop_mode=PROP_MM):
         N = self.N
         if idx is None and mb_size < N:
             idx = torch.as_tensor(np.random.choice(N, mb_size, replace=False), device='cuda')
@@ -320,7 +369,7 @@ class SGPR(Base_Model):
         mout, vout, kfu = layer.forward_prop_thru_cav(xb)
         dm, dv, _, scal = self.lik_layer.compute_log_Z(mout, vout, yb, alpha, scale_logZ)
         grads = layer.backprop_grads_reg(mout, vout, dm, dv, kfu, xb, alpha)
-        grads['sn'] = scal[1:2]
+        grads.update(self.lik_layer.hyper_grads(scal))
         phi = layer.compute_phi(alpha)
         energy = lincomb(torch.empty(1, dtype=F64, device='cuda'), scale_logZ, scal[0:1], 1.0, phi)
         return energy, grads, N
@@ -348,7 +397,6 @@ class SDGPR(Base_Model):
 
     def __init__(self, x_train, y_train, no_pseudos, hidden_sizes, lik='Gaussian'):
         super(SDGPR, self).__init__(y_train)
-        _check_lik(lik)
         self.x_train = np.asarray(x_train, dtype=np.float64)
         self.Din = self.x_train.shape[1]
         self.Dout = self.y_train.shape[1]
@@ -356,7 +404,7 @@ class SDGPR(Base_Model):
         self.L = len(self.size) - 1
         self.Ms = list(no_pseudos) if isinstance(no_pseudos, (list, tuple)) else [no_pseudos] * self.L
         self.x_dev = torch.as_tensor(self.x_train, device='cuda')
-        self.lik_layer = Gauss_Layer(self.N, self.Dout)
+        self.lik_layer = _make_lik(lik, self.N, self.Dout)
         self.sgp_layers = [SGP_Layer(self.N, self.size[i], self.size[i + 1], self.Ms[i])
                            for i in range(self.L)]
 
@@ -377,7 +425,7 @@ class SDGPR(Base_Model):
         self.lik_layer.update_hypers(dp)
         return dp
 
-    def _objective(self, params, mb_size, alpha, idx=None):
+    def _objective(self, params, mb_size, alpha, idx=None, prop_mode=PROP_MM):
         N = self.N
         if idx is None and mb_size < N:
             idx = torch.as_tensor(np.random.choice(N, mb_size, replace=False), device='cuda')
@@ -423,7 +471,7 @@ class SDGPR(Base_Model):
             for st in streams:
                 cur.wait_stream(st)
         del keep
-        grads['sn'] = scal[1:2]
+        grads.update(self.lik_layer.hyper_grads(scal))
         energy = lincomb(torch.empty(1, dtype=F64, device='cuda'), scale_logZ, scal[0:1])
         for phi in phis:
             lincomb(energy, 1.0, energy, 1.0, phi)
@@ -461,12 +509,33 @@ class SDGPR(Base_Model):
         return phis
 
     def predict_f(self, inputs, prop_mode=PROP_MM, no_samples=200):
-        """Base_SDGPR.predict_f (base_models.py:1132-1158): moment-matched propagation
-        through the layers' posteriors."""
+        """Base_SDGPR.predict_f (base_models.py:1132-1138): moment-matched propagation
+        through the layers' posteriors (predict_f_mm, :1140-1158), or predict_f_mc."""
         _check_mode(prop_mode)
+        if prop_mode == PROP_MC:
+            return self.predict_f_mc(inputs, no_samples)
         x = torch.as_tensor(np.asarray(inputs, dtype=np.float64), device='cuda')
         mf, vf = self._predict_dev(x)
         return mf.cpu().numpy(), vf.cpu().numpy()
+
+    def predict_f_mc(self, inputs, no_samples):
+        """base_models.py:1159-1180: samples through the layers; the first layer draws
+        (no_samples, N, D_1) normals, each later layer one normal per sample and output,
+        on the legacy global RNG.  Returns (samples (no_samples, N, Dout), mf, vf of the
+        last layer at the propagated samples)."""
+        x = torch.as_tensor(np.asarray(inputs, dtype=np.float64), device='cuda')
+        N = x.shape[0]
+        samples = x
+        mf = vf = None
+        for i, layer in enumerate(self.sgp_layers):
+            mf, vf = layer.forward_prop_thru_post(samples)
+            if i == 0:
+                eps = _mc_draws(no_samples, mf.shape[0], mf.shape[1])
+            else:
+                eps = _mc_draws(1, mf.shape[0], mf.shape[1])
+            samples = dops.mc_inputs(eps, mf, vf).reshape(-1, mf.shape[1])
+        out = samples.reshape(no_samples, N, self.sgp_layers[-1].Dout)
+        return out.cpu().numpy(), mf.cpu().numpy(), vf.cpu().numpy()
 
     def _predict_dev(self, x):
         mf = vf = None
@@ -491,14 +560,13 @@ class SGPLVM(Base_Model):
     def __init__(self, y_train, hidden_size, no_pseudo, lik='Gaussian', prior_mean=0, prior_var=1,
                  nat_param=True):
         super(SGPLVM, self).__init__(y_train)
-        _check_lik(lik)
         self.Dout = self.y_train.shape[1]
         self.Din = hidden_size
         self.M = no_pseudo
         self.nat_param = nat_param
         self.prior_mean = prior_mean
         self.prior_var = prior_var
-        self.lik_layer = Gauss_Layer(self.N, self.Dout)
+        self.lik_layer = _make_lik(lik, self.N, self.Dout)
         self.sgp_layer = SGP_Layer(self.N, self.Din, self.Dout, self.M, nat_param)
         from . import xfactors
         self._xf = xfactors
@@ -514,7 +582,9 @@ class SGPLVM(Base_Model):
         self.lik_layer.update_hypers(dp)
         return dp
 
-    def _objective(self, params, mb_size, alpha, idx=None):
+    _prop_modes = (PROP_MM, PROP_MC)
+
+    def _objective(self, params, mb_size, alpha, idx=None, prop_mode=PROP_MM):
         N = self.N
         if idx is None and mb_size != N:
             idx = torch.as_tensor(np.random.choice(N, mb_size, replace=False), dtype=torch.int64,
@@ -531,13 +601,21 @@ class SGPLVM(Base_Model):
         self.lik_layer.update_hypers(dp)
         xs = self._xf.lvm_x_state(dp['x1'], dp['x2'], idx, alpha, self.prior_mean,
                                   self.prior_var, self.nat_param)
-        mout, vout, psi1, psi2 = layer.forward_prop_thru_cav(xs['mcav'], xs['vcav'])
-        dm, dv, dma, scal = self.lik_layer.compute_log_Z(mout, vout, yb, alpha, scale_logZ,
-                                                          want_dmall=True)
-        grads, gin = layer.backprop_grads_lvm_mm(mout, vout, dm, dv, psi1, psi2, xs['mcav'],
-                                                 xs['vcav'], alpha, dm_all=dma)
-        del psi2
-        grads['sn'] = scal[1:2]
+        if prop_mode == PROP_MC:
+            # aep_models.py:745-760: the draws follow the minibatch draw
+            eps = _mc_draws(config.MC_NO_SAMPLES, B, self.Din)
+            m3, v3, kfu, x = layer.forward_prop_thru_cav(xs['mcav'], xs['vcav'], PROP_MC, eps=eps)
+            dm, dv, scal = self.lik_layer.compute_log_Z_mc(m3, v3, yb, alpha, scale_logZ)
+            grads, dx = layer.backprop_grads_lvm_mc(m3, v3, dm, dv, kfu, x, alpha)
+            gin = layer.backprop_grads_reparam(dx, xs['mcav'], xs['vcav'], eps)
+        else:
+            mout, vout, psi1, psi2 = layer.forward_prop_thru_cav(xs['mcav'], xs['vcav'])
+            dm, dv, dma, scal = self.lik_layer.compute_log_Z(mout, vout, yb, alpha, scale_logZ,
+                                                              want_dmall=True)
+            grads, gin = layer.backprop_grads_lvm_mm(mout, vout, dm, dv, psi1, psi2, xs['mcav'],
+                                                     xs['vcav'], alpha, dm_all=dma)
+            del psi2
+        grads.update(self.lik_layer.hyper_grads(scal))
         gx1, gx2, xcontrib = self._xf.lvm_x_grads(xs, gin['mx'], gin['vx'], N, B, alpha,
                                                  self.prior_mean, self.prior_var, self.Din,
                                                  self.nat_param)
@@ -589,7 +667,10 @@ class SGPSSM(Base_Model):
     def __init__(self, y_train, hidden_size, no_pseudo, lik='Gaussian', prior_mean=0, prior_var=1,
                  x_control=None, gp_emi=False, control_to_emi=True):
         super(SGPSSM, self).__init__(y_train)
-        _check_lik(lik)
+        if lik.lower() not in ('gaussian', 'probit') or (lik.lower() == 'probit' and not gp_emi):
+            # base_models.py:1404-1414: a probit emission needs the GP emission layer (the
+            # reference's linear probit emission class does not exist)
+            raise NotImplementedError('likelihood %s not implemented' % lik)
         self.Dout = self.y_train.shape[1]
         self.Din = hidden_size
         self.M = no_pseudo
@@ -607,7 +688,7 @@ class SGPSSM(Base_Model):
         self.dyn_layer = SGP_Layer(self.N - 1, self.Din + self.Dcon_dyn, self.Din, self.M)
         if gp_emi:
             self.emi_layer = SGP_Layer(self.N, self.Din + self.Dcon_emi, self.Dout, self.M)
-            self.lik_layer = Gauss_Layer(self.N, self.Dout)
+            self.lik_layer = _make_lik(lik, self.N, self.Dout)
         else:
             self.emi_layer = Gauss_Emis(self.y_dev, self.Dout, self.Din + self.Dcon_emi)
 
@@ -632,7 +713,9 @@ class SGPSSM(Base_Model):
         u = self.u_dev[s:e]
         return (torch.cat([m, u], dim=1), torch.cat([v, torch.zeros_like(u)], dim=1))
 
-    def _objective(self, params, mb_size, alpha, idx=None):
+    _prop_modes = (PROP_MM, PROP_MC)
+
+    def _objective(self, params, mb_size, alpha, idx=None, prop_mode=PROP_MM):
         if idx is not None:
             raise NotImplementedError('SGPSSM draws contiguous windows, not index sets')
         N, Din = self.N, self.Din
@@ -661,13 +744,24 @@ class SGPSSM(Base_Model):
             tm1_m, tm1_v = tm1_m.contiguous(), tm1_v.contiguous()
         if self.Dcon_emi == 0:
             up_m, up_v = up_m.contiguous(), up_v.contiguous()
-        mprop, vprop, psi1, psi2 = dyn.forward_prop_thru_cav(tm1_m, tm1_v)
-        dmall, dvprop, dmt, dvt, scal_t = dops.transition(mprop, vprop, cm[s + 1:e].contiguous(),
-                                                          cv[s + 1:e].contiguous(), self.sn, alpha,
-                                                          scale_dyn)
-        g_dyn, gin_dyn = dyn.backprop_grads_lvm_mm(mprop, vprop, None, dvprop, psi1, psi2, tm1_m, tm1_v,
-                                                   alpha, dm_all=dmall)
-        del psi2
+        mc = prop_mode == PROP_MC
+        if mc:
+            # aep_models.py:1114-1126: the dynamics draws come first
+            eps = _mc_draws(config.MC_NO_SAMPLES, Bd, Din + self.Dcon_dyn)
+            m3, v3, kfu, x = dyn.forward_prop_thru_cav(tm1_m, tm1_v, PROP_MC, eps=eps)
+            dmp, dvp, dmt, dvt, scal_t = dops.transition_mc(m3, v3, cm[s + 1:e].contiguous(),
+                                                            cv[s + 1:e].contiguous(), self.sn, alpha,
+                                                            scale_dyn)
+            g_dyn, dx = dyn.backprop_grads_lvm_mc(m3, v3, dmp, dvp, kfu, x, alpha)
+            gin_dyn = dyn.backprop_grads_reparam(dx, tm1_m, tm1_v, eps)
+        else:
+            mprop, vprop, psi1, psi2 = dyn.forward_prop_thru_cav(tm1_m, tm1_v)
+            dmall, dvprop, dmt, dvt, scal_t = dops.transition(mprop, vprop, cm[s + 1:e].contiguous(),
+                                                              cv[s + 1:e].contiguous(), self.sn, alpha,
+                                                              scale_dyn)
+            g_dyn, gin_dyn = dyn.backprop_grads_lvm_mm(mprop, vprop, None, dvprop, psi1, psi2, tm1_m,
+                                                       tm1_v, alpha, dm_all=dmall)
+            del psi2
         grads = {'sn': scal_t[1:2]}
         for k, v in g_dyn.items():
             grads[k + '_dynamic'] = v
@@ -675,15 +769,23 @@ class SGPSSM(Base_Model):
                          dyn.compute_phi(alpha))
         if self.gp_emi:
             emi = self.emi_layer
-            mout, vout, e1, e2 = emi.forward_prop_thru_cav(up_m, up_v)
-            dm, dv, dma, scal_e = self.lik_layer.compute_log_Z(mout, vout, self.y_dev[s:e].contiguous(),
-                                                               alpha, scale_emi, want_dmall=True)
-            g_emi, gin_emi = emi.backprop_grads_lvm_mm(mout, vout, dm, dv, e1, e2, up_m, up_v, alpha,
-                                                       dm_all=dma)
-            del e2
+            ye = self.y_dev[s:e].contiguous()
+            if mc:
+                eps = _mc_draws(config.MC_NO_SAMPLES, Be, Din + self.Dcon_emi)
+                m3, v3, kfu, x = emi.forward_prop_thru_cav(up_m, up_v, PROP_MC, eps=eps)
+                dm, dv, scal_e = self.lik_layer.compute_log_Z_mc(m3, v3, ye, alpha, scale_emi)
+                g_emi, dx = emi.backprop_grads_lvm_mc(m3, v3, dm, dv, kfu, x, alpha)
+                gin_emi = emi.backprop_grads_reparam(dx, up_m, up_v, eps)
+            else:
+                mout, vout, e1, e2 = emi.forward_prop_thru_cav(up_m, up_v)
+                dm, dv, dma, scal_e = self.lik_layer.compute_log_Z(mout, vout, ye, alpha, scale_emi,
+                                                                   want_dmall=True)
+                g_emi, gin_emi = emi.backprop_grads_lvm_mm(mout, vout, dm, dv, e1, e2, up_m, up_v, alpha,
+                                                           dm_all=dma)
+                del e2
             for k, v in g_emi.items():
                 grads[k + '_emission'] = v
-            grads['sn_emission'] = scal_e[1:2]
+            grads.update(self.lik_layer.hyper_grads(scal_e, '_emission'))
             lincomb(energy, 1.0, energy, scale_emi, scal_e[0:1], 1.0, emi.compute_phi(alpha))
             dup, vup = gin_emi['mx'], gin_emi['vx']
         else:

@@ -6,6 +6,8 @@ import torch
 from . import _lib
 from ._lib import P, S, call, query, Probe, set_probe  # noqa: F401
 
+S_ = S
+
 F64 = torch.float64
 
 
@@ -238,6 +240,54 @@ def transition(mp, vp, mt, vt, sn, alpha, scale):
     call('gpx_transition_tilted', B, D, P(mp), P(vp), P(mt), P(vt), P(sn), float(alpha),
          float(scale), *[P(o) for o in outs], P(scal), S())
     return outs + [scal]
+
+
+def gauss_tilted_mc(m3, v3, y, sn, alpha, scale):
+    """Gaussian tilted normaliser over the samples of (S, B, D) propagations."""
+    S, B, D = m3.shape
+    dm, dv = torch.empty_like(m3), torch.empty_like(m3)
+    scal = empty(2 + 2 * 64, like=m3)
+    call('gpx_gauss_tilted_mc', S, B, D, P(m3), P(v3), P(y), P(sn), float(alpha), float(scale),
+         P(dm), P(dv), P(scal), S_())
+    return dm, dv, scal
+
+
+def probit_tilted(m, v, y, alpha, scale, want_dmall=False):
+    """Probit tilted normaliser of (B, D) or (S, B, D) propagations; scal[0] = logZ."""
+    three = m.dim() == 3
+    Sn = m.shape[0] if three else 1
+    B, D = m.shape[-2:]
+    dm, dv = torch.empty_like(m), torch.empty_like(m)
+    dma = torch.empty_like(m) if want_dmall else None
+    scal = empty(2 + 2 * 64, like=m)
+    call('gpx_probit_tilted', Sn, int(three), B, D, P(m), P(v), P(y), float(alpha), float(scale),
+         P(dm), P(dv), P(dma), P(scal), S_())
+    return dm, dv, dma, scal
+
+
+def transition_mc(mp3, vp3, mt, vt, sn, alpha, scale):
+    Sn, B, D = mp3.shape
+    dmp, dvp = torch.empty_like(mp3), torch.empty_like(mp3)
+    dmt, dvt = torch.empty_like(mt), torch.empty_like(mt)
+    scal = empty(2 + 2 * 64, like=mp3)
+    call('gpx_transition_tilted_mc', Sn, B, D, P(mp3), P(vp3), P(mt), P(vt), P(sn), float(alpha),
+         float(scale), P(dmp), P(dvp), P(dmt), P(dvt), P(scal), S_())
+    return dmp, dvp, dmt, dvt, scal
+
+
+def mc_inputs(eps, mx, vx):
+    """x[s] = eps[s] sqrt(vx) + mx for eps (S, N, Q)."""
+    Sn, N, Q = eps.shape
+    x = torch.empty_like(eps)
+    call('gpx_mc_inputs', Sn, N, Q, P(eps), P(mx), P(vx), P(x), S_())
+    return x
+
+
+def mc_reparam_bwd(dx, eps, vx):
+    Sn, N, Q = eps.shape
+    dmx, dvx = torch.empty_like(vx), torch.empty_like(vx)
+    call('gpx_mc_reparam_bwd', Sn, N, Q, P(dx), P(eps), P(vx), P(dmx), P(dvx), S_())
+    return dmx, dvx
 
 
 def emission(C, Rp, y, mx, vx, alpha, scale):

@@ -240,14 +240,27 @@ class SGP_Layer(object):
     # ------------------------------------------------------------------
     # forward propagation (aep_models.py:116-199; base_models.py:235-307)
     # ------------------------------------------------------------------
-    def forward_prop_thru_cav(self, mx, vx=None, mode=PROP_MM):
+    def forward_prop_thru_cav(self, mx, vx=None, mode=PROP_MM, eps=None):
+        """aep_models.py:116-140.  mode=PROP_MC takes the standard-normal draws eps
+        (S, B, Din) as a device tensor (the reference draws them inside, with
+        np.random.randn(MC_NO_SAMPLES, B, Din)) and returns ((S, B, Dout) means and
+        variances, the stacked kfu (S*B, M) and inputs (S*B, Din))."""
         if vx is None:
             return self._forward_prop_deterministic_thru_cav(mx)
         if mode == PROP_MM:
             return self._forward_prop_random_thru_cav_mm(mx, vx)
-        if mode in (PROP_MC, PROP_LIN):
+        if mode == PROP_MC:
+            return self._forward_prop_random_thru_cav_mc(mx, vx, eps)
+        if mode == PROP_LIN:
             raise NotImplementedError('propagation mode %s is not on the GPU path' % mode)
         raise NotImplementedError('unknown propagation mode')
+
+    def _forward_prop_random_thru_cav_mc(self, mx, vx, eps):
+        """_forward_prop_random_thru_cav_mc (aep_models.py:160-180)."""
+        S, B, Q = eps.shape
+        x = dops.mc_inputs(eps, mx, vx).reshape(S * B, Q)
+        m, v, kfu = self._forward_prop_deterministic_thru_cav(x)
+        return m.reshape(S, B, self.Dout), v.reshape(S, B, self.Dout), kfu, x
 
     def _forward_prop_deterministic_thru_cav(self, x):
         kfu = dops.se(x, self.zu, self.ls, self.sf)
@@ -269,6 +282,14 @@ class SGP_Layer(object):
         Bp = mm(psi2.reshape(N, M * M), self.Bhat_sto.reshape(D, M * M), tB=True)
         vout = dops.mm_var(mout, Bp, self.sf)
         return mout, vout, psi1, psi2
+
+    def forward_prop_thru_post_mc(self, mx, vx, eps):
+        """_forward_prop_random_thru_post_mc (base_models.py:309-332) for draws eps
+        (S, B, Din): (S, B, Dout) means and variances."""
+        S, B, Q = eps.shape
+        x = dops.mc_inputs(eps, mx, vx).reshape(S * B, Q)
+        m, v = self.forward_prop_thru_post(x)
+        return m.reshape(S, B, self.Dout), v.reshape(S, B, self.Dout)
 
     def forward_prop_thru_post(self, mx, vx=None, mode=PROP_MM, return_info=False):
         self._posterior_predictive()
@@ -422,6 +443,52 @@ class SGP_Layer(object):
         dls, dsf = dops.hyper(self.ls, self.sf, dell, gls, dsf2, sumdv, gsf)
         lincomb(dzu, 1.0, dzu, 1.0, gz)
         return {'sf': dsf, 'ls': dls, 'zu': dzu, 'eta1_R': deta1, 'eta2': deta2}
+
+    def backprop_grads_lvm_mc(self, m, v, dm, dv, kfu, x, alpha=1.0):
+        """backprop_grads_lvm_mc (aep_models.py:307-410): hyper grads of the layer at the
+        stacked Monte Carlo inputs x (S*B rows) and the grads w.r.t. x; dm, dv already
+        scaled, S*B*Dout entries in sample-major order.  The cavity terms are formed as
+        -Suhat (sum_n dv kfuKi kfuKi^T) Suhat - (Suhat sum_n dm kfuKi) muhat^T, the
+        regrouping of the reference's per-point products Suhat Kuu^-1 kfu_n."""
+        N = self.N
+        D, M = self.Dout, self.M
+        B = kfu.shape[0]
+        Ki = self.Kuuinv
+        beta = (N - alpha) * 1.0 / N
+        dm = dm.reshape(B, D)
+        dv = dv.reshape(B, D)
+        kfuKi = mm(kfu, Ki)
+        dkfu = mm(dm, self.Ahat)
+        dops.wsum(dv, self._T_det, dkfu, alpha=2.0, beta=1.0)
+        dsf2, dell, dzu, dx = dops.kfu_bwd(dkfu, kfu, x, self.zu, self.ls, self.sf, want_dx=True)
+        sumdv = colsum(dv, B * D, 1)
+        dmucav = mm(dm, kfuKi, tA=True)
+        dSucav = mm(dops.rowscale(dv, kfuKi), kfuKi, tA=True)
+        dSinv = _sandwich(self.Suhat, dSucav, self.Suhat, alpha=-1.0)
+        dSinvM = mm(self.Suhat, _col(dmucav)).reshape(D, M)
+        _add_outer(dSinv, dSinvM, self.muhat, -1.0)
+        c = self._adjoint('mm', alpha)
+        dtheta1 = lincomb(torch.empty_like(dSinv), beta, dSinv, 1.0, c['T1'])
+        dtheta2 = lincomb(torch.empty_like(dSinvM), beta, dSinvM, 1.0, c['c2'])
+        deta1 = dops.pack_dR(self.theta_1_R, dtheta1)
+        dAhat = mm(dm, kfu, tA=True)
+        dKi = colsum(dSinv, D, M * M).reshape(M, M)
+        mm(dAhat, self.muhat, dKi, tA=True, beta=1.0)
+        dBhat = mm(dops.rowscale(dv, kfu), kfu, tA=True)
+        KiS2 = mm(Ki, self.Suhat, alpha=2.0, E=_eye(M, Ki), gamma=-1.0)
+        mm(KiS2.reshape(D * M, M), dBhat.reshape(D * M, M), dKi, tA=True, beta=1.0)
+        Minner = lincomb(torch.empty_like(dKi), 1.0, c['Mc'], -2.0, dKi)
+        M_all = _sandwich(Ki, Minner, Ki, alpha=0.5, E=Ki, gamma=0.5 * D)
+        gsf, gls, gz = dops.dtrace(M_all, self.zu, self.ls, self.sf)
+        dls, dsf = dops.hyper(self.ls, self.sf, dell, gls, dsf2, sumdv, gsf)
+        lincomb(dzu, 1.0, dzu, 1.0, gz)
+        return {'sf': dsf, 'ls': dls, 'zu': dzu, 'eta1_R': deta1, 'eta2': dtheta2}, dx
+
+    def backprop_grads_reparam(self, dx, mx, vx, eps):
+        """backprop_grads_reparam (base_models.py:373-388): {'mx', 'vx'} of
+        x = eps sqrt(vx) + mx."""
+        dmx, dvx = dops.mc_reparam_bwd(dx, eps, vx)
+        return {'mx': dmx, 'vx': dvx}
 
     def backprop_grads_lvm_mm(self, m, v, dm, dv, psi1, psi2, mx, vx, alpha=1.0, dm_all=None,
                               side=None, keep=None):

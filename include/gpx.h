@@ -196,6 +196,42 @@ int gpx_gauss_tilted(int B, int D, const double *m, const double *v, const doubl
                      const double *sn, double alpha, double scale, double *dm, double *dv,
                      double *dmall, double *scal, void *stream);
 
+/* The Gaussian tilted log-normaliser over S Monte Carlo samples: m, v, dm, dv are
+ * (S, B, D), y is (B, D); per (n, d) the log-mean-exp over the samples, dm, dv the
+ * softmax-weighted per-sample gradients times scale; scal as gpx_gauss_tilted (130 doubles).
+ * Replaces the 3-D branch of Gauss_Layer.compute_log_Z + backprop_grads
+ * (geepee/lik_layers.py:139-152, 154-181). */
+int gpx_gauss_tilted_mc(int S, int B, int D, const double *m, const double *v, const double *y,
+                        const double *sn, double alpha, double scale, double *dm, double *dv,
+                        double *scal, void *stream);
+
+/* Probit tilted log-normaliser for y in {-1, +1}: closed form at alpha = 1, 10-point
+ * Gauss-Hermite quadrature otherwise, with the reference's small additive constants.
+ * three = 0: (B, D) arrays (S must be 1); three = 1: (S, B, D) Monte Carlo samples, averaged
+ * by log-mean-exp.  scal[0] = logZ (scal holds 130 doubles); dm, dv scaled; dmall optional.
+ * Replaces Probit_Layer.compute_log_Z (geepee/lik_layers.py:285-411). */
+int gpx_probit_tilted(int S, int three, int B, int D, const double *m, const double *v,
+                      const double *y, double alpha, double scale, double *dm, double *dv,
+                      double *dmall, double *scal, void *stream);
+
+/* Monte Carlo inputs x[s] = eps[s] * sqrt(vx) + mx for draws eps (S, N, Q), and the adjoint
+ * dmx = sum_s dx[s], dvx = sum_s dx[s] eps[s] / (2 sqrt(vx)).
+ * Replace geepee/aep_models.py:170-173 (base_models.py:320-322) and backprop_grads_reparam
+ * (geepee/base_models.py:373-388). */
+int gpx_mc_inputs(int S, int N, int Q, const double *eps, const double *mx, const double *vx,
+                  double *x, void *stream);
+int gpx_mc_reparam_bwd(int S, int N, int Q, const double *dx, const double *eps, const double *vx,
+                       double *dmx, double *dvx, void *stream);
+
+/* GPSSM transition normaliser over S Monte Carlo propagations mp, vp (S, B, D) against the
+ * next-state cavities mt, vt (B, D): scal[0] = scale * logZ, scal[1] = dsn (130 doubles);
+ * dmp, dvp (S, B, D) and dmt, dvt (B, D) scaled.  Replaces the 3-D branch of
+ * SGPSSM.compute_transition_tilted (geepee/aep_models.py:1344-1372). */
+int gpx_transition_tilted_mc(int S, int B, int D, const double *mp, const double *vp,
+                             const double *mt, const double *vt, const double *sn, double alpha,
+                             double scale, double *dmp, double *dvp, double *dmt, double *dvt,
+                             double *scal, void *stream);
+
 /* GPSSM transition tilted normaliser (scaled): scal[0] = scale * logZ, scal[1] = dsn (scal
  * holds 130 doubles: the rest are block partials).
  * Replaces SGPSSM.compute_transition_tilted (geepee/aep_models.py:1317-1343). */

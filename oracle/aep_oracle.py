@@ -762,7 +762,8 @@ def sdgpr_objective(x, y, params, Ms, mb_size, alpha=1.0, fixed=(), idxs=None, l
 
 
 def sgplvm_objective(y, params, M, mb_size, alpha=1.0, prior_mean=0.0, prior_var=1.0,
-                     nat_param=True, fixed=(), idxs=None, prop_mode=PROP_MM, eps=None):
+                     nat_param=True, fixed=(), idxs=None, prop_mode=PROP_MM, eps=None,
+                     lik='Gaussian'):
     """SGPLVM.objective_function (aep_models.py:694-815) with x-factor helpers
     (:817-867) and Base_SGPLVM.update_hypers / compute_posterior_grad_x
     (base_models.py:895-929).  The energy is NOT divided by N (:808-815)."""
@@ -802,15 +803,22 @@ def sgplvm_objective(y, params, M, mb_size, alpha=1.0, prior_mean=0.0, prior_var
         if eps is None:
             eps = np.random.randn(MC_NO_SAMPLES, B, Din)
         m3, v3, kfu_s, x_s = fwd_mc_cav(L, mcav, vcav, eps)
-        logZ, dm, dv = gauss_logZ_mc(params['sn'], m3, v3, yb, alpha)
+        if lik.lower() == 'probit':
+            logZ, dm, dv = probit_logZ(m3, v3, yb, alpha)
+        else:
+            logZ, dm, dv = gauss_logZ_mc(params['sn'], m3, v3, yb, alpha)
         grads, dx = bwd_mc(L, scale_logZ * dm, scale_logZ * dv, kfu_s, x_s, alpha)
         gmx, gvx = reparam_bwd(dx, vcav, eps)
         gin = {'mx': gmx, 'vx': gvx}
     else:
         mout, vout, psi1, psi2 = fwd_mm_cav(L, mcav, vcav)
-        logZ, dm, dv = gauss_logZ(params['sn'], mout, vout, yb, alpha)
+        if lik.lower() == 'probit':
+            logZ, dm, dv = probit_logZ(mout, vout, yb, alpha)
+        else:
+            logZ, dm, dv = gauss_logZ(params['sn'], mout, vout, yb, alpha)
         grads, gin = bwd_mm(L, mout, scale_logZ * dm, scale_logZ * dv, psi1, psi2, mcav, vcav, alpha)
-    grads['sn'] = gauss_dsn(params['sn'], dv, B, y.shape[1], alpha, scale_logZ)
+    if lik.lower() != 'probit':
+        grads['sn'] = gauss_dsn(params['sn'], dv, B, y.shape[1], alpha, scale_logZ)
 
     def phi_x(m, v):   # compute_phi_x (:863-867)
         return np.sum(0.5 * (m ** 2 / v + np.log(v))), m / v, 0.5 * (-m ** 2 / v ** 2 + 1 / v)
@@ -1067,6 +1075,25 @@ def sdgpr_predict(params, Ms, N, xs, noisy=False):
     if noisy:
         vf = vf + np.exp(2.0 * params['sn'])
     return mf, vf
+
+
+def sdgpr_predict_mc(params, Ms, N, xs, no_samples):
+    """Base_SDGPR.predict_f_mc (base_models.py:1159-1180): samples pushed through the
+    layers' posteriors with draws from the legacy global RNG -- (no_samples, N, D_1) for
+    the first layer, one per sample and output for each later one.  Returns (samples,
+    mf, vf) of the last layer."""
+    samples = xs
+    mf = vf = None
+    for i, Mi in enumerate(Ms):
+        L = layer_update(params, '_%d' % i, Mi, N, True)
+        mf, vf = fwd_det_post(L, samples)
+        if i == 0:
+            eps = np.random.randn(no_samples, mf.shape[0], mf.shape[1])
+            samples = (np.sqrt(vf) * eps + mf).reshape(no_samples * mf.shape[0], mf.shape[1])
+        else:
+            eps = np.random.randn(mf.shape[0], mf.shape[1])
+            samples = np.sqrt(vf) * eps + mf
+    return samples.reshape(no_samples, xs.shape[0], -1), mf, vf
 
 
 def sgplvm_predict(params, M, N, xs, noisy=False):

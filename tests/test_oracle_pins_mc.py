@@ -168,3 +168,13 @@ def test_model_gradients_fd_probit():
     pd.pop('sn')
     yd = np.sign(rs.randn(15, 1))
     check_grads(lambda q: O.sdgpr_objective(xd, yd, q, Ms, 15, alpha=0.5, lik='probit'), pd, rtol=2e-5)
+
+
+def test_model_gradients_fd_sgplvm_probit():
+    y, p = W.sgplvm_problem(N=10, Q=2, D=3, M=5, seed=2)
+    yb = np.sign(y)
+    p.pop('sn')
+    check_grads(lambda q: O.sgplvm_objective(yb, q, 5, 10, alpha=0.5, lik='probit'), p, rtol=1e-5)
+    eps = _fixed_eps((O.MC_NO_SAMPLES, 10, 2), 11)
+    check_grads(lambda q: O.sgplvm_objective(yb, q, 5, 10, alpha=0.5, lik='probit', prop_mode='MC',
+                                             eps=eps), p, rtol=1e-5)
