@@ -59,6 +59,9 @@ SIGNATURES = {
     'gpx_lvm_x_cavity': (_i, [_i, _i, _p, _p, _p, _d, _d, _d, _i, _p, _p, _p]),
     'gpx_lvm_x_grads': (_i, [_i, _i, _i, _p, _p, _p, _p, _p, _d, _d, _d, _i, _p, _p, _p, _p]),
     'gpx_ssm_x_cavity': (_i, [_i, _i, _p, _p, _d, _d, _d, _p, _p, _p]),
+    'gpx_ssm_x_posterior': (_i, [_i, _i, _p, _p, _d, _d, _p, _p, _p]),
+    'gpx_hidden_cavity': (_i, [_i, _i, _p, _p, _d, _p, _p, _p]),
+    'gpx_hidden_grads': (_i, [_i, _i, _p, _p, _d, _p, _p, _p, _p, _p, _p]),
     'gpx_ssm_x_grads': (_i, [_i, _i, _i, _i, _p, _p, _d, _d, _d, _p, _p, _i, _p, _p, _p, _p, _i, _p,
                              _p, _p, _p]),
     'gpx_emission_predict': (_i, [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p]),

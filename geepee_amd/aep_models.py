@@ -807,6 +807,130 @@ class SGPSSM(Base_Model):
         from . import xfactors
         return xfactors
 
+    # posterior latent path and rollouts (base_models.py:1453-1542, 1562-1600) -------
+    def _posterior_x_dev(self):
+        dp = self._dp.views if self._dp is not None else self._device_params(self.get_hypers())
+        return dops.ssm_posterior(dp['x_factor_1'], dp['x_factor_2'], self.prior_mean, self.prior_var)
+
+    def get_posterior_x(self, idxs=None):
+        """Base_SGPSSM.get_posterior_x (base_models.py:1562-1576)."""
+        pm, pv = self._posterior_x_dev()
+        pm, pv = pm.cpu().numpy(), pv.cpu().numpy()
+        if idxs is not None:
+            pm, pv = pm[idxs], pv[idxs]
+        return pm, pv
+
+    def get_posterior_y(self):
+        """Base_SGPSSM.get_posterior_y (base_models.py:1578-1595): (my, vf, vyn)."""
+        pm, pv = self._posterior_x_dev()
+        if self.Dcon_emi > 0:
+            pm = torch.cat([pm, self.u_dev], dim=1)
+            pv = torch.cat([pv, torch.zeros_like(self.u_dev)], dim=1)
+        if self.gp_emi:
+            mf, vf = self.emi_layer.forward_prop_thru_post(pm, pv)
+            my, vyn = self.lik_layer.output_probabilistic(mf, vf)
+        else:
+            my, vyn = self.emi_layer.output_probabilistic(pm, pv)
+            _, vf = dops.emission_predict(self.emi_layer.C, self._no_noise(), pm, pv)
+        return my.cpu().numpy(), vf.cpu().numpy(), vyn.cpu().numpy()
+
+    def _no_noise(self):
+        """Emission log-noise of -inf: gpx_emission_predict then returns the noiseless
+        variance diag(C diag(v) C^T) exactly."""
+        D = self.Dout
+        if getattr(self, '_ninf', None) is None or self._ninf.numel() != D:
+            self._ninf = torch.full((D,), float('-inf'), dtype=F64, device='cuda')
+        return self._ninf
+
+    def predict_forward(self, T, x_control=None, prop_mode=PROP_MM, no_samples=200):
+        """Base_SGPSSM.predict_forward (base_models.py:1453-1461)."""
+        if prop_mode == PROP_MM:
+            return self.predict_forward_mm(T, x_control)
+        if prop_mode == PROP_MC:
+            return self.predict_forward_mc(T, x_control, no_samples)
+        if prop_mode == 'LIN':
+            raise NotImplementedError('TODO')
+        raise NotImplementedError('unknown prop mode %s' % prop_mode)
+
+    def _emit(self, m, v, sn2):
+        """(my, vf, vyn) of latent inputs m, v (control columns already appended)."""
+        if self.gp_emi:
+            mf, vf = self.emi_layer.forward_prop_thru_post(m, v)
+            ones = torch.ones(1, dtype=F64, device='cuda')
+            return mf, vf, lincomb(torch.empty_like(vf), 1.0, vf, sn2, ones)
+        my, vyn = dops.emission_predict(self.emi_layer.C, self.emi_layer.Rp, m, v)
+        _, vf = dops.emission_predict(self.emi_layer.C, self._no_noise(), m, v)
+        return my, vf, vyn
+
+    def predict_forward_mm(self, T, x_control=None):
+        """base_models.py:1463-1502: T moment-matched steps of the dynamics posterior from
+        the last latent state, each emitted; every step stays on the device and the whole
+        path comes back in one copy.  Returns (mx, vx, my, vy_noiseless, vy)."""
+        pm, pv = self._posterior_x_dev()
+        u = None if x_control is None else torch.as_tensor(np.asarray(x_control, dtype=np.float64),
+                                                           device='cuda')
+        sn2 = float(np.exp(2.0 * np.asarray(self.get_hypers()['sn_emission']).ravel()[0])) \
+            if self.gp_emi else 0.0
+        mx = torch.empty(T, self.Din, dtype=F64, device='cuda')
+        vx = torch.empty_like(mx)
+        my = torch.empty(T, self.Dout, dtype=F64, device='cuda')
+        vf = torch.empty_like(my)
+        vy = torch.empty_like(my)
+        m, v = pm[-1:], pv[-1:]
+        for t in range(T):
+            if self.Dcon_dyn > 0:
+                m = torch.cat([m, u[t:t + 1]], dim=1)
+                v = torch.cat([v, torch.zeros_like(u[t:t + 1])], dim=1)
+            mt, vt = self.dyn_layer.forward_prop_thru_post(m, v)
+            if self.gp_emi:
+                if self.Dcon_emi > 0:
+                    mtc = torch.cat([mt, u[t:t + 1]], dim=1)
+                    vtc = torch.cat([vt, torch.zeros_like(u[t:t + 1])], dim=1)
+                else:
+                    mtc, vtc = mt, vt
+                a, b, c = self._emit(mtc, vtc, sn2)
+            else:
+                # the linear emission reads the latent state alone (base_models.py:1490)
+                a, b, c = self._emit(mt, vt, sn2)
+            mx[t:t + 1].copy_(mt)
+            vx[t:t + 1].copy_(vt)
+            my[t:t + 1].copy_(a)
+            vf[t:t + 1].copy_(b)
+            vy[t:t + 1].copy_(c)
+            m, v = mt, vt
+        return tuple(o.cpu().numpy() for o in (mx, vx, my, vf, vy))
+
+    def predict_forward_mc(self, T, x_control=None, no_samples=200):
+        """base_models.py:1504-1542: sampled latent paths; draws from numpy's legacy global
+        RNG as the reference makes them.  Returns (x, my, vy) with a leading time axis."""
+        if self.Dcon_emi > 0:
+            raise NotImplementedError('Monte Carlo rollout with control inputs to the emission')
+        pm, pv = self._posterior_x_dev()
+        u = None if x_control is None else torch.as_tensor(np.asarray(x_control, dtype=np.float64),
+                                                           device='cuda')
+        sn2 = float(np.exp(2.0 * np.asarray(self.get_hypers()['sn_emission']).ravel()[0])) \
+            if self.gp_emi else 0.0
+        S, Din = no_samples, self.Din
+        x = torch.empty(T, S, Din, dtype=F64, device='cuda')
+        my = torch.empty(T, S, self.Dout, dtype=F64, device='cuda')
+        vy = torch.empty_like(my)
+        xs = dops.mc_inputs(_mc_draws(S, 1, Din), pm[-1:], pv[-1:]).reshape(S, Din)
+        zeros = torch.zeros(S, Din, dtype=F64, device='cuda')
+        for t in range(T):
+            xc = torch.cat([xs, u[t:t + 1].expand(S, -1)], dim=1) if self.Dcon_dyn > 0 else xs
+            mt, vt = self.dyn_layer.forward_prop_thru_post(xc)
+            xs = dops.mc_inputs(_mc_draws(1, S, Din), mt, vt).reshape(S, Din)
+            if self.gp_emi:
+                mf, vf = self.emi_layer.forward_prop_thru_post(xs)
+                ones = torch.ones(1, dtype=F64, device='cuda')
+                a, c = mf, lincomb(torch.empty_like(vf), 1.0, vf, sn2, ones)
+            else:
+                a, c = dops.emission_predict(self.emi_layer.C, self.emi_layer.Rp, xs, zeros)
+            x[t].copy_(xs)
+            my[t].copy_(a)
+            vy[t].copy_(c)
+        return x.cpu().numpy(), my.cpu().numpy(), vy.cpu().numpy()
+
     def predict_f(self, inputs):
         x = torch.as_tensor(np.asarray(inputs, dtype=np.float64), device='cuda')
         mf, vf = self.dyn_layer.forward_prop_thru_post(x)
@@ -821,3 +945,85 @@ class SGPSSM(Base_Model):
         else:
             my, vy = self.emi_layer.output_probabilistic(mf, vf)
         return my.cpu().numpy(), vy.cpu().numpy()
+
+
+class SDGPR_H(SDGPR):
+    """Deep GP regression with Gaussian factors on the hidden layer values
+    (aep_models.py:1440-1863): each hidden value h_i[n] carries two tied factors with
+    natural parameters (h_factor_1_i, exp(2 h_factor_2_i)); the layers are linked by
+    Gaussian transitions of noise exp(2 sn_hidden[i]).  Layer i's output keys end in '_i'.
+    The objective is defined for the full batch (the reference's hidden-factor gradients
+    broadcast minibatch rows against all N rows, :1633-1652)."""
+
+    def __init__(self, x_train, y_train, no_pseudos, hidden_sizes, lik='Gaussian'):
+        super(SDGPR_H, self).__init__(x_train, y_train, no_pseudos, hidden_sizes, lik)
+
+    def init_hypers(self, y_train=None):
+        """aep_models.py:1817-1840: the layers as in SDGPR, sn_hidden = log 0.001, hidden
+        factors with h_factor_1 = 0 and h_factor_2 = log 0.01."""
+        p = super(SDGPR_H, self).init_hypers(y_train)
+        p['sn_hidden'] = np.log(0.001) * np.ones(self.L - 1)
+        for i in range(self.L - 1):
+            p['h_factor_1_%d' % i] = np.zeros((self.N, self.size[i + 1]))
+            p['h_factor_2_%d' % i] = np.log(0.01) * np.ones((self.N, self.size[i + 1]))
+        return p
+
+    def _objective(self, params, mb_size, alpha, idx=None, prop_mode=PROP_MM):
+        N, L = self.N, self.L
+        if idx is not None or mb_size < N:
+            raise NotImplementedError('SDGPR_H objective is defined for the full batch only')
+        if L < 2:
+            raise ValueError('SDGPR_H needs at least one hidden layer')
+        scale = -1.0 / alpha
+        dp = self._device_params(params)
+        self.lik_layer.update_hypers(dp)
+        energy = torch.zeros(1, dtype=F64, device='cuda')
+        for i, layer in enumerate(self.sgp_layers):
+            layer.prepare(dp, alpha, key_suffix='_%d' % i)
+            lincomb(energy, 1.0, energy, 1.0, layer.compute_phi(alpha))
+        h = [(dp['h_factor_1_%d' % i], dp['h_factor_2_%d' % i]) for i in range(L - 1)]
+        cav = [dops.hidden_cavity(h1, h2, alpha) for h1, h2 in h]
+        sn = dp['sn_hidden']
+        grads, dmc, dvc, dsn = {}, [None] * (L - 1), [None] * (L - 1), []
+        for i in range(L - 1):
+            layer = self.sgp_layers[i]
+            if i == 0:
+                mp, vp, kfu = layer.forward_prop_thru_cav(self.x_dev)
+            else:
+                mp, vp, p1, p2 = layer.forward_prop_thru_cav(cav[i - 1][0], cav[i - 1][1])
+            dmall, dvp, dmt, dvt, scal = dops.transition(mp, vp, cav[i][0], cav[i][1], sn[i:i + 1],
+                                                         alpha, scale)
+            if i == 0:
+                dmp = lincomb(torch.empty_like(dmt), -1.0, dmt)
+                g = layer.backprop_grads_reg(mp, vp, dmp, dvp, kfu, self.x_dev, alpha)
+            else:
+                g, gin = layer.backprop_grads_lvm_mm(mp, vp, None, dvp, p1, p2, cav[i - 1][0],
+                                                     cav[i - 1][1], alpha, dm_all=dmall)
+                del p2
+                lincomb(dmc[i - 1], 1.0, dmc[i - 1], 1.0, gin['mx'])
+                lincomb(dvc[i - 1], 1.0, dvc[i - 1], 1.0, gin['vx'])
+            dmc[i], dvc[i] = dmt, dvt
+            lincomb(energy, 1.0, energy, 1.0, scal[0:1])
+            dsn.append(scal[1:2])
+            for k, val in g.items():
+                grads[k + '_%d' % i] = val
+        i = L - 1
+        layer = self.sgp_layers[i]
+        mp, vp, p1, p2 = layer.forward_prop_thru_cav(cav[i - 1][0], cav[i - 1][1])
+        dm, dv, dma, scal = self.lik_layer.compute_log_Z(mp, vp, self.y_dev, alpha, scale, want_dmall=True)
+        g, gin = layer.backprop_grads_lvm_mm(mp, vp, dm, dv, p1, p2, cav[i - 1][0], cav[i - 1][1], alpha,
+                                             dm_all=dma)
+        del p2
+        lincomb(dmc[i - 1], 1.0, dmc[i - 1], 1.0, gin['mx'])
+        lincomb(dvc[i - 1], 1.0, dvc[i - 1], 1.0, gin['vx'])
+        lincomb(energy, 1.0, energy, scale, scal[0:1])
+        for k, val in g.items():
+            grads[k + '_%d' % i] = val
+        grads.update(self.lik_layer.hyper_grads(scal))
+        for j, (h1, h2) in enumerate(h):
+            g1, g2, hs = dops.hidden_grads(h1, h2, alpha, dmc[j], dvc[j])
+            grads['h_factor_1_%d' % j] = g1
+            grads['h_factor_2_%d' % j] = g2
+            lincomb(energy, 1.0, energy, 1.0, hs[0:1])
+        grads['sn_hidden'] = torch.cat(dsn)
+        return energy, grads, N

@@ -197,9 +197,108 @@ __global__ void ssm_grad_finish(int nblk, int Q, double px1, double px2, double 
   scal[0] = 0.5 * Q * (m0 * m0 / v0 + log(v0)) + pp + pc;
 }
 
+// SGPSSM posterior of every x_t (base_models.py:1562-1576 over the tied factors of
+// :1705-1728): three factors inside the chain, two at its ends, the prior at t = 0.
+__global__ void ssm_post_kernel(int N, int Q, const double *__restrict__ xf1, const double *__restrict__ x2,
+                                double px1, double px2, double *__restrict__ pm, double *__restrict__ pv) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)N * Q;
+       e += (long)gridDim.x * blockDim.x) {
+    const int t = (int)(e / Q);
+    const double w = (t == 0 || t == N - 1) ? 2.0 : 3.0;
+    const double p1 = w * xf1[e] + (t == 0 ? px1 : 0.0), p2 = w * exp(2.0 * x2[e]) + (t == 0 ? px2 : 0.0);
+    pm[e] = p1 / p2;
+    pv[e] = 1.0 / p2;
+  }
+}
+
+// SDGPR_H hidden-variable factors (aep_models.py:1620-1752): each hidden value has two
+// tied factors, so its posterior natural parameters are 2 (h1, exp(2 h2)) and its cavity
+// (2 - alpha) (h1, exp(2 h2)).  Cavity mean / variance of every row.
+__global__ void hidden_cav_kernel(long n, const double *__restrict__ h1, const double *__restrict__ h2,
+                                  double alpha, double *__restrict__ cm, double *__restrict__ cv) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const double c1 = h1[e] * (2.0 - alpha), c2 = exp(2.0 * h2[e]) * (2.0 - alpha);
+    cm[e] = c1 / c2;
+    cv[e] = 1.0 / c2;
+  }
+}
+
+// Gradients w.r.t. the hidden factors through the posterior, the cavity and the cavity
+// inputs (compute_grads_hidden, aep_models.py:1620-1660) and the posterior / cavity
+// log-partitions (compute_phi_posterior_h / cavity_h, :1662-1700), block partials in
+// scal[2 + b].
+__global__ __launch_bounds__(256) void hidden_grad_kernel(long n, const double *__restrict__ h1,
+                                                          const double *__restrict__ h2, double alpha,
+                                                          const double *__restrict__ dmc,
+                                                          const double *__restrict__ dvc, double *__restrict__ g1,
+                                                          double *__restrict__ g2, double *__restrict__ scal) {
+  __shared__ double red[16];
+  const double sp = -(1.0 - 1.0 / alpha), sc = -1.0 / alpha;
+  double phi = 0.0;
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
+    const double f2 = exp(2.0 * h2[e]);
+    const double p1 = 2.0 * h1[e], p2 = 2.0 * f2;
+    const double c1 = (2.0 - alpha) * h1[e], c2 = (2.0 - alpha) * f2;
+    const double d1p = 2.0 * (p1 / p2), d2p = -p1 * p1 / (p2 * p2) - 1.0 / p2;
+    const double d1c = (2.0 - alpha) * (c1 / c2), d2c = (2.0 - alpha) * (-0.5 * c1 * c1 / (c2 * c2) - 0.5 / c2);
+    const double d1z = (2.0 - alpha) * (dmc[e] / c2);
+    const double d2z = (2.0 - alpha) * (-dmc[e] * c1 / (c2 * c2) - dvc[e] / (c2 * c2));
+    g1[e] = d1z + sc * d1c + sp * d1p;
+    g2[e] = 2.0 * (d2z + sc * d2c + sp * d2p) * f2;
+    phi += sp * (0.5 * (p1 * p1 / p2 - log(p2))) + sc * (0.5 * (c1 * c1 / c2 - log(c2)));
+  }
+  phi = gpx_block_sum(phi, red);
+  if (threadIdx.x == 0) scal[2 + blockIdx.x] = phi;
+}
+
+__global__ void hidden_grad_finish(int nblk, double *__restrict__ scal) {
+  if (threadIdx.x != 0) return;
+  double t = 0.0;
+  for (int b = 0; b < nblk; ++b) t += scal[2 + b];
+  scal[0] = t;
+}
+
 }  // namespace
 
 extern "C" {
+
+int gpx_ssm_x_posterior(int N, int Q, const double *xf1, const double *x2, double prior_mean, double prior_var,
+                        double *pm, double *pv, void *stream) {
+  if (N < 2 || Q < 1 || !xf1 || !x2 || !pm || !pv || !(prior_var > 0)) return GPX_EINVAL;
+  unsigned g = gpx_cdiv((long)N * Q, 256);
+  if (g > 4096) g = 4096;
+  hipLaunchKernelGGL(ssm_post_kernel, dim3(g), dim3(256), 0, (hipStream_t)stream, N, Q, xf1, x2,
+                     prior_mean / prior_var, 1.0 / prior_var, pm, pv);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+int gpx_hidden_cavity(int N, int D, const double *h1, const double *h2, double alpha, double *cm, double *cv,
+                      void *stream) {
+  if (N < 1 || D < 1 || !h1 || !h2 || !cm || !cv || !(alpha > 0)) return GPX_EINVAL;
+  const long n = (long)N * D;
+  unsigned g = gpx_cdiv(n, 256);
+  if (g > 4096) g = 4096;
+  hipLaunchKernelGGL(hidden_cav_kernel, dim3(g), dim3(256), 0, (hipStream_t)stream, n, h1, h2, alpha, cm, cv);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+int gpx_hidden_grads(int N, int D, const double *h1, const double *h2, double alpha, const double *dmc,
+                     const double *dvc, double *g1, double *g2, double *scal, void *stream) {
+  if (N < 1 || D < 1 || !h1 || !h2 || !dmc || !dvc || !g1 || !g2 || !scal || !(alpha > 0)) return GPX_EINVAL;
+  const long n = (long)N * D;
+  unsigned nblk = gpx_cdiv(n, 1024);
+  if (nblk > GPX_XF_BLOCKS) nblk = GPX_XF_BLOCKS;
+  if (nblk < 1) nblk = 1;
+  hipLaunchKernelGGL(hidden_grad_kernel, dim3(nblk), dim3(256), 0, (hipStream_t)stream, n, h1, h2, alpha, dmc,
+                     dvc, g1, g2, scal);
+  GPX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(hidden_grad_finish, dim3(1), dim3(64), 0, (hipStream_t)stream, (int)nblk, scal);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
 
 int gpx_lvm_x_cavity(int B, int Q, const long *idx, const double *x1, const double *x2, double alpha,
                      double prior_mean, double prior_var, int nat_param, double *mcav, double *vcav,

@@ -290,6 +290,30 @@ def mc_reparam_bwd(dx, eps, vx):
     return dmx, dvx
 
 
+def ssm_posterior(xf1, x2, prior_mean, prior_var):
+    """Posterior means / variances of the GPSSM latent path from its tied factors."""
+    N, Q = xf1.shape
+    pm, pv = torch.empty_like(xf1), torch.empty_like(xf1)
+    call('gpx_ssm_x_posterior', N, Q, P(xf1), P(x2), float(prior_mean), float(prior_var), P(pm), P(pv), S_())
+    return pm, pv
+
+
+def hidden_cavity(h1, h2, alpha):
+    N, D = h1.shape
+    cm, cv = torch.empty_like(h1), torch.empty_like(h1)
+    call('gpx_hidden_cavity', N, D, P(h1), P(h2), float(alpha), P(cm), P(cv), S_())
+    return cm, cv
+
+
+def hidden_grads(h1, h2, alpha, dmc, dvc):
+    N, D = h1.shape
+    g1, g2 = torch.empty_like(h1), torch.empty_like(h1)
+    scal = empty(2 + 64, like=h1)
+    call('gpx_hidden_grads', N, D, P(h1), P(h2), float(alpha), P(dmc.contiguous()), P(dvc.contiguous()),
+         P(g1), P(g2), P(scal), S_())
+    return g1, g2, scal
+
+
 def emission(C, Rp, y, mx, vx, alpha, scale):
     Nb, Din = mx.shape
     Dout = C.shape[0]

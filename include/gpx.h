@@ -362,6 +362,23 @@ int gpx_lvm_x_grads(int N, int B, int Q, const long *idx, const double *x1, cons
 int gpx_ssm_x_cavity(int N, int Q, const double *xf1, const double *x2, double alpha, double prior_mean,
                      double prior_var, double *cm, double *cv, void *stream);
 
+/* Posterior means / variances of every latent state x_t from the tied factors (three inside
+ * the chain, two at its ends, the prior at t = 0).  Replaces Base_SGPSSM.get_posterior_x
+ * (geepee/base_models.py:1562-1576) over update_hypers (:1705-1728). */
+int gpx_ssm_x_posterior(int N, int Q, const double *xf1, const double *x2, double prior_mean,
+                        double prior_var, double *pm, double *pv, void *stream);
+
+/* SDGPR_H hidden-value factors (h1, h2 with precision factor exp(2 h2), two per value):
+ * cavity means / variances, and the gradients through posterior, cavity and cavity inputs
+ * (dmc, dvc) with scal[0] = the posterior and cavity log-partition terms (scal holds 66
+ * doubles).  Replace compute_cavity_h, compute_grads_hidden and compute_phi_posterior_h /
+ * compute_phi_cavity_h (geepee/aep_models.py:1620-1700, 1737-1752). */
+int gpx_hidden_cavity(int N, int D, const double *h1, const double *h2, double alpha, double *cm,
+                      double *cv, void *stream);
+int gpx_hidden_grads(int N, int D, const double *h1, const double *h2, double alpha,
+                     const double *dmc, const double *dvc, double *g1, double *g2, double *scal,
+                     void *stream);
+
 /* SGPSSM x-factor gradients and scal[0] = phi_prior_x + phi_post_x + phi_cav_x for the
  * window [s, e) (scal holds 130 doubles: the rest are block partials):
  * aep_models.py:1170-1196, 1208-1315, 1389-1437. */

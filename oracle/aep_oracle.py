@@ -1051,6 +1051,99 @@ def sgpssm_objective(y, params, M, Din, mb_size, alpha=1.0, prior_mean=0.0, prio
     return _finish(energy, grads, fixed, N)
 
 
+def transition_tilted_h(sn, m_prop, v_prop, m_t, v_t, alpha, scale, D):
+    """SDGPR_H.compute_transition_tilted (aep_models.py:1702-1735): the hidden-layer
+    transition normaliser, all outputs times scale."""
+    sn2 = np.exp(2 * sn)
+    v_sum = v_t + v_prop + sn2 / alpha
+    m_diff = m_t - m_prop
+    logZ = np.sum(-0.5 * m_diff ** 2 / v_sum - 0.5 * np.log(2 * np.pi * v_sum)
+                  + 0.5 * (1 - alpha) * np.log(2 * np.pi * sn2) - 0.5 * np.log(alpha))
+    dv = -0.5 / v_sum + 0.5 * m_diff ** 2 / v_sum ** 2
+    dsn = np.sum(dv) * 2 * sn2 / alpha + m_prop.shape[0] * D * (1 - alpha)
+    return scale * logZ, scale * m_diff / v_sum, scale * dv, -scale * m_diff / v_sum, scale * dv, scale * dsn
+
+
+def sdgpr_h_objective(x, y, params, Ms, mb_size, alpha=1.0, fixed=(), lik='Gaussian'):
+    """SDGPR_H.objective_function (aep_models.py:1506-1618): a deep GP whose hidden
+    layer values carry Gaussian factors (h_factor_1_i, h_factor_2_i), tied twice per value
+    (compute_cavity_h :1737-1752, compute_grads_hidden :1620-1660, log-partitions
+    :1662-1700).  Full batch only: the reference's hidden-factor gradients broadcast the
+    batch rows against all N rows (:1633-1652), which fails for a proper minibatch."""
+    N = y.shape[0]
+    if mb_size < N:
+        raise NotImplementedError('SDGPR_H objective is defined for the full batch only')
+    Lnum = len(Ms)
+    scale_logZ = -1.0 / alpha
+    layers = [layer_update(params, '_%d' % i, Ms[i], N, True) for i in range(Lnum)]
+    for Li in layers:
+        layer_cavity(Li, alpha)
+    h1 = [np.asarray(params['h_factor_1_%d' % i]) for i in range(Lnum - 1)]
+    h2 = [np.exp(2 * np.asarray(params['h_factor_2_%d' % i])) for i in range(Lnum - 1)]
+    cav_1 = [a * (2.0 - alpha) for a in h1]
+    cav_2 = [b * (2.0 - alpha) for b in h2]
+    cav_m = [a / b for a, b in zip(cav_1, cav_2)]
+    cav_v = [1.0 / b for b in cav_2]
+    sn = np.asarray(params['sn_hidden'], dtype=np.float64)
+    dsn = np.zeros_like(sn)
+    grads, dmc, dvc = {}, [], []
+    logZ = 0.0
+    for i in range(Lnum - 1):
+        Li = layers[i]
+        if i == 0:
+            mp, vp, kfu = fwd_det_cav(Li, x)
+        else:
+            mp, vp, psi1, psi2 = fwd_mm_cav(Li, cav_m[i - 1], cav_v[i - 1])
+        lz, dmp, dvp, dmi, dvi, dsni = transition_tilted_h(sn[i], mp, vp, cav_m[i], cav_v[i], alpha,
+                                                           scale_logZ, Li.Dout)
+        if i == 0:
+            g = bwd_det(Li, dmp, dvp, kfu, x, alpha)
+        else:
+            g, gin = bwd_mm(Li, mp, dmp, dvp, psi1, psi2, cav_m[i - 1], cav_v[i - 1], alpha)
+            dmc[i - 1] = dmc[i - 1] + gin['mx']
+            dvc[i - 1] = dvc[i - 1] + gin['vx']
+        logZ += lz
+        for k, val in g.items():
+            grads[k + '_%d' % i] = val
+        dmc.append(dmi)
+        dvc.append(dvi)
+        dsn[i] = dsni
+    i = Lnum - 1
+    Li = layers[-1]
+    mp, vp, psi1, psi2 = fwd_mm_cav(Li, cav_m[i - 1], cav_v[i - 1])
+    lz, dm, dv, glik = _lik(lik, params, mp, vp, y, alpha, N, scale_logZ)
+    g, gin = bwd_mm(Li, mp, scale_logZ * dm, scale_logZ * dv, psi1, psi2, cav_m[i - 1], cav_v[i - 1],
+                    alpha)
+    logZ += scale_logZ * lz
+    dmc[i - 1] = dmc[i - 1] + gin['mx']
+    dvc[i - 1] = dvc[i - 1] + gin['vx']
+    for k, val in g.items():
+        grads[k + '_%d' % i] = val
+    grads.update(glik)
+    sp, sc = -(1.0 - 1.0 / alpha), -1.0 / alpha
+    phi_h = 0.0
+    for j in range(Lnum - 1):
+        p1, p2 = 2 * h1[j], 2 * h2[j]
+        d1_post = 2 * (p1 / p2)
+        d2_post = -p1 ** 2 / p2 ** 2 - 1 / p2
+        d1_cav = (2 - alpha) * (cav_1[j] / cav_2[j])
+        d2_cav = (2 - alpha) * (-0.5 * cav_1[j] ** 2 / cav_2[j] ** 2 - 0.5 / cav_2[j])
+        d1_z = (2 - alpha) * (dmc[j] / cav_2[j])
+        d2_z = (2 - alpha) * (-dmc[j] * cav_1[j] / cav_2[j] ** 2 - dvc[j] / cav_2[j] ** 2)
+        grads['h_factor_1_%d' % j] = d1_z + sc * d1_cav + sp * d1_post
+        grads['h_factor_2_%d' % j] = 2 * (d2_z + sc * d2_cav + sp * d2_post) * h2[j]
+        phi_h += sp * np.sum(0.5 * (p1 ** 2 / p2 - np.log(p2))) \
+            + sc * np.sum(0.5 * (cav_1[j] ** 2 / cav_2[j] - np.log(cav_2[j])))
+    grads['sn_hidden'] = dsn
+    energy = logZ + sum(layer_phi(Lj, alpha) for Lj in layers) + phi_h
+    return _finish(energy, grads, fixed, N)
+
+
+def sdgpr_h_predict(params, Ms, N, xs, noisy=False):
+    """SDGPR_H.predict_f / predict_y (aep_models.py:1754-1815): as SDGPR."""
+    return sdgpr_predict(params, Ms, N, xs, noisy)
+
+
 # ----------------------------------------------------------------------------
 # Prediction (base_models.py:235-307, 734-761, 985-1034, 1140-1158, 1264-1275,
 # 1441-1560)
@@ -1121,6 +1214,104 @@ def sgpssm_predict(params, M, T, xs, noisy=False, gp_emi=False):
         return mg, vg + np.exp(2.0 * params['sn_emission'])
     my, _, vy = emission_predict(params['C_emission'], params['R_emission'], mf, vf)
     return my, np.diagonal(vy, axis1=1, axis2=2).copy()
+
+
+def sgpssm_posterior_x(params, prior_mean=0.0, prior_var=1.0):
+    """Base_SGPSSM.get_posterior_x (base_models.py:1562-1576) over the tied factors of
+    update_hypers (:1705-1728)."""
+    xf1 = np.asarray(params['x_factor_1'])
+    xf2 = np.exp(2 * np.asarray(params['x_factor_2']))
+    p1, p2 = 3 * xf1, 3 * xf2
+    p1[[0, -1], :] = 2 * xf1[[0, -1], :]
+    p2[[0, -1], :] = 2 * xf2[[0, -1], :]
+    p1[0, :] += prior_mean / prior_var
+    p2[0, :] += 1.0 / prior_var
+    return p1 / p2, 1.0 / p2
+
+
+def sgpssm_posterior_y(params, M, T_train, x_control=None, control_to_emi=True, gp_emi=False,
+                       prior_mean=0.0, prior_var=1.0):
+    """Base_SGPSSM.get_posterior_y (base_models.py:1578-1595): (my, vf, vyn)."""
+    mx, vx = sgpssm_posterior_x(params, prior_mean, prior_var)
+    if x_control is not None and control_to_emi:
+        mx = np.hstack((mx, x_control))
+        vx = np.hstack((vx, np.zeros_like(x_control)))
+    if gp_emi:
+        emi = layer_update(params, '_emission', M, T_train, True)
+        mf, vf = fwd_mm_post(emi, mx, vx)
+        return mf, vf, vf + np.exp(2.0 * params['sn_emission'])
+    my, vy, vyn = emission_predict(params['C_emission'], params['R_emission'], mx, vx)
+    return my, np.diagonal(vy, axis1=1, axis2=2).copy(), np.diagonal(vyn, axis1=1, axis2=2).copy()
+
+
+def sgpssm_predict_forward(params, M, T_train, Din, T, x_control=None, control_to_emi=True,
+                           gp_emi=False, prior_mean=0.0, prior_var=1.0):
+    """Base_SGPSSM.predict_forward_mm (base_models.py:1463-1502): T moment-matched steps of
+    the dynamics posterior from the last latent state, each emitted; returns (mx, vx, my,
+    vy_noiseless, vy)."""
+    dyn = layer_update(params, '_dynamic', M, T_train - 1, True)
+    emi = layer_update(params, '_emission', M, T_train, True) if gp_emi else None
+    Dcon = 0 if x_control is None else x_control.shape[1]
+    Dcon_emi = Dcon if control_to_emi else 0
+    Dout = params['C_emission'].shape[0] if not gp_emi else params['eta2_emission'].shape[0]
+    out = [np.zeros((T, Din)), np.zeros((T, Din)), np.zeros((T, Dout)), np.zeros((T, Dout)),
+           np.zeros((T, Dout))]
+    pm, pv = sgpssm_posterior_x(params, prior_mean, prior_var)
+    m, v = pm[[-1], :], pv[[-1], :]
+    for t in range(T):
+        if Dcon > 0:
+            m = np.hstack((m, x_control[[t], :]))
+            v = np.hstack((v, np.zeros((1, Dcon))))
+        mt, vt = fwd_mm_post(dyn, m, v)
+        if Dcon_emi > 0:
+            mtc, vtc = np.hstack((mt, x_control[[t], :])), np.hstack((vt, np.zeros((1, Dcon_emi))))
+        else:
+            mtc, vtc = mt, vt
+        if gp_emi:
+            mf, vf = fwd_mm_post(emi, mtc, vtc)
+            my, vyn = mf, vf + np.exp(2.0 * params['sn_emission'])
+        else:
+            my, vy0, vyn = emission_predict(params['C_emission'], params['R_emission'], mt, vt)
+            vf = np.diagonal(vy0, axis1=1, axis2=2)
+            vyn = np.diagonal(vyn, axis1=1, axis2=2)
+        out[0][t], out[1][t], out[2][t], out[3][t], out[4][t] = mt, vt, my, vf, vyn
+        m, v = mt, vt
+    return tuple(out)
+
+
+def sgpssm_predict_forward_mc(params, M, T_train, Din, T, no_samples, x_control=None,
+                              control_to_emi=True, gp_emi=False, prior_mean=0.0, prior_var=1.0):
+    """Base_SGPSSM.predict_forward_mc (base_models.py:1504-1542): samples of the latent
+    path with draws from the legacy global RNG (no_samples x Din per step, one set before
+    the first step); returns (x, my, vy) with a leading time axis."""
+    dyn = layer_update(params, '_dynamic', M, T_train - 1, True)
+    emi = layer_update(params, '_emission', M, T_train, True) if gp_emi else None
+    Dcon = 0 if x_control is None else x_control.shape[1]
+    if Dcon > 0 and control_to_emi:
+        # the reference's control-to-emission branch does not run (:1531, a misplaced
+        # parenthesis in np.tile)
+        raise NotImplementedError('Monte Carlo rollout with control inputs to the emission')
+    Dout = params['C_emission'].shape[0] if not gp_emi else params['eta2_emission'].shape[0]
+    x = np.zeros((T, no_samples, Din))
+    my = np.zeros((T, no_samples, Dout))
+    vy = np.zeros((T, no_samples, Dout))
+    pm, pv = sgpssm_posterior_x(params, prior_mean, prior_var)
+    eps = np.random.randn(no_samples, Din)
+    xs = eps * np.sqrt(pv[[-1], :]) + pm[[-1], :]
+    for t in range(T):
+        xc = np.hstack((xs, np.tile(x_control[[t], :], [no_samples, 1]))) if Dcon > 0 else xs
+        mt, vt = fwd_det_post(dyn, xc)
+        eps = np.random.randn(no_samples, Din)
+        xs = eps * np.sqrt(vt) + mt
+        if gp_emi:
+            mf, vf = fwd_det_post(emi, xs)
+            myt, vyt = mf, vf + np.exp(2.0 * params['sn_emission'])
+        else:
+            myt, _, vyt = emission_predict(params['C_emission'], params['R_emission'], xs,
+                                           np.zeros_like(xs))
+            vyt = np.diagonal(vyt, axis1=1, axis2=2)
+        x[t], my[t], vy[t] = xs, myt, vyt
+    return x, my, vy
 
 
 # ----------------------------------------------------------------------------
