@@ -435,9 +435,12 @@ class SDGPR(Base_Model):
             xb, yb = self.x_dev.index_select(0, idx), self.y_dev.index_select(0, idx)
         B = yb.shape[0]
         scale_logZ = -N * 1.0 / B / alpha
-        phis = self._prepare_layers(params, alpha)
+        phis, ready = self._prepare_layers(params, alpha)
         ms, vs, p1, p2 = [], [], [], []
         for i, layer in enumerate(self.sgp_layers):
+            if ready is not None:
+                # layer i's forward needs only its own q(u) set-up
+                torch.cuda.current_stream().wait_event(ready[i])
             if i == 0:
                 m, v, kfu = layer.forward_prop_thru_cav(xb)
                 ms.append(m); vs.append(v); p1.append(kfu); p2.append(None)
@@ -457,6 +460,8 @@ class SDGPR(Base_Model):
             if i == 0:
                 if streams:
                     torch.cuda.current_stream().wait_stream(streams[0])
+                # (forking this layer's adjoint over more streams measured slower when
+                # replayed as a graph: the extra branches add dependencies, not overlap)
                 g = layer.backprop_grads_reg(ms[0], vs[0], dmi, dvi, p1[0], xb, alpha)
             else:
                 g, gin = layer.backprop_grads_lvm_mm(ms[i], vs[i], dmi, dvi, p1[i], p2[i],
@@ -480,7 +485,10 @@ class SDGPR(Base_Model):
     def _prepare_layers(self, params, alpha):
         """update_hypers, compute_cavity and compute_phi of every layer.  The layers'
         chains are independent, so each runs on a stream of its own forked from the
-        current one and joined back (parallel branches when captured in a hipGraph)."""
+        current one (parallel branches when captured in a hipGraph).  Returns the phi
+        terms and, on the multi-stream path, one event per layer marking its q(u) set-up;
+        the caller waits for event i before layer i's forward and joins the streams at
+        the end."""
         dp = self._device_params(params)
         self.lik_layer.update_hypers(dp)
         cur = torch.cuda.current_stream()
@@ -489,7 +497,7 @@ class SDGPR(Base_Model):
             for i, layer in enumerate(self.sgp_layers):
                 layer.prepare(dp, alpha, key_suffix='_%d' % i)
                 phis.append(layer.compute_phi(alpha))
-            return phis
+            return phis, None
         if getattr(self, '_streams', None) is None:
             self._streams = [torch.cuda.Stream() for _ in range(self.L)]
         phis, ready = [], []
@@ -504,9 +512,7 @@ class SDGPR(Base_Model):
                 # the adjoint's q(u)-only terms follow on the same stream; the forward
                 # waits only for the q(u) set-up above, the layer's backward for these
                 layer.adjoint_constants('reg' if i == 0 else 'mm', alpha)
-        for ev in ready:
-            cur.wait_event(ev)
-        return phis
+        return phis, ready
 
     def predict_f(self, inputs, prop_mode=PROP_MM, no_samples=200):
         """Base_SDGPR.predict_f (base_models.py:1132-1138): moment-matched propagation
