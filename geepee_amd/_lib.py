@@ -50,6 +50,17 @@ SIGNATURES = {
     'gpx_emission_workspace': (_l, [_i, _i, _i]),
     'gpx_emission_tilted': (_i, [_i, _i, _i, _p, _p, _p, _i, _p, _i, _p, _i, _d, _d, _p, _p, _p,
                                  _p, _p, _p, _p]),
+    'gpx_vfe_kl': (_i, [_i, _i, _p, _p, _d, _p, _p, _p, _p]),
+    'gpx_gauss_lik_exp': (_i, [_i, _i, _i, _i, _p, _p, _p, _p, _d, _p, _p, _p, _p]),
+    'gpx_probit_lik_exp': (_i, [_i, _i, _i, _i, _p, _p, _p, _d, _p, _p, _p, _p]),
+    'gpx_transition_lik_exp': (_i, [_i, _i, _i, _i, _p, _p, _p, _p, _p, _d, _p, _p, _p, _p, _p, _p]),
+    'gpx_emission_lik_exp_workspace': (_l, [_i, _i]),
+    'gpx_emission_lik_exp': (_i, [_i, _i, _i, _p, _p, _p, _i, _p, _i, _p, _i, _d, _p, _p, _p, _p, _p,
+                                  _p, _p]),
+    'gpx_lvm_x_post': (_i, [_i, _i, _p, _p, _p, _d, _d, _i, _p, _p, _p]),
+    'gpx_lvm_vfe_x_grads': (_i, [_i, _i, _p, _p, _p, _p, _p, _d, _d, _i, _d, _p, _p, _p, _p]),
+    'gpx_ssm_vfe_x_grads': (_i, [_i, _i, _i, _i, _p, _p, _d, _d, _p, _p, _i, _p, _p, _p, _p, _i, _d,
+                                 _p, _p, _p, _p]),
     'gpx_rowscale': (_i, [_i, _i, _i, _p, _p, _p, _p]),
     'gpx_wsum': (_i, [_i, _i, _i, _d, _p, _p, _d, _p, _p]),
     'gpx_dmall': (_i, [_l, _p, _p, _p, _p, _p]),

@@ -255,6 +255,71 @@ int gpx_emission_tilted(int Nb, int Dout, int Din, const double *C, const double
 int gpx_emission_predict(int Nb, int Dout, int Din, const double *C, const double *Rp, const double *mf,
                          const double *vf, double *my, double *vy, void *stream);
 
+/* ---- variational free energy (VFE) family -------------------------------------- */
+
+/* KL(q(u) || p(u)) = (D logdet Kuu - sum_d logdet Su_d - D M + sum_d <Kuu^-1, Su_d + mu_d mu_d^T>) / 2
+ * into out[0], with logdet Su_d = sgnS * ldS[d] (ldS = logdet Su^-1, sgnS = -1, in the natural
+ * parameterisation).  Replaces SGP_Layer.compute_KL (geepee/vfe_models.py:309-326). */
+int gpx_vfe_kl(int D, int M, const double *ldK, const double *ldS, double sgnS, const double *Kuuinv,
+               const double *Splusmm, double *out, void *stream);
+
+/* Expected Gaussian log-likelihood E log N(y | f, exp(2 sn)) of (B, D) propagations
+ * (three = 0, S = 1) or the mean over S Monte Carlo samples (S, B, D) (three = 1):
+ * scal[0] = scale * E, scal[1] = dsn * scale, dm and dv times scale (scal holds 130 doubles:
+ * the rest are block partials).  Replaces Gauss_Layer.compute_log_lik_exp and
+ * backprop_grads_log_lik_exp (geepee/lik_layers.py:183-236). */
+int gpx_gauss_lik_exp(int S, int three, int B, int D, const double *m, const double *v, const double *y,
+                      const double *sn, double scale, double *dm, double *dv, double *scal, void *stream);
+
+/* Expected probit log-likelihood E log Phi(y f) by 10-point Gauss-Hermite quadrature, (B, D) or
+ * averaged over (S, B, D) samples: scal[0] = scale * E; dm, dv times scale (130 doubles).
+ * Replaces Probit_Layer.compute_log_lik_exp (geepee/lik_layers.py:412-455). */
+int gpx_probit_lik_exp(int S, int three, int B, int D, const double *m, const double *v, const double *y,
+                       double scale, double *dm, double *dv, double *scal, void *stream);
+
+/* GPSSM expected transition log-likelihood E log N(x_t | f(x_{t-1}), exp(2 sn)) of the
+ * propagations mp, vp ((B, D), or (S, B, D) averaged) against the posterior of x_t (mt, vt):
+ * scal[0] = scale * E, scal[1] = dsn; dmp, dvp, dmt, dvt scaled (130 doubles).
+ * Replaces SGPSSM.compute_transition_log_lik_exp (geepee/vfe_models.py:1080-1120). */
+int gpx_transition_lik_exp(int S, int three, int B, int D, const double *mp, const double *vp, const double *mt,
+                           const double *vt, const double *sn, double scale, double *dmp, double *dvp,
+                           double *dmt, double *dvt, double *scal, void *stream);
+
+/* Linear-Gaussian emission expected log-likelihood E log N(y | C x, diag R), R = exp(2 Rp), under
+ * x ~ N(mx, diag vx) (Dout, Din <= 16): scal[0] = scale * E; dmx, dvx, dC, dR scaled; `work`
+ * holds gpx_emission_lik_exp_workspace(Dout, Din) doubles.  Replaces
+ * Gauss_Emis.compute_emission_log_lik_exp (geepee/lik_layers.py:629-676). */
+long gpx_emission_lik_exp_workspace(int Dout, int Din);
+int gpx_emission_lik_exp(int Nb, int Dout, int Din, const double *C, const double *Rp, const double *y, int ldy,
+                         const double *mx, int ldm, const double *vx, int ldv, double scale, double *dmx,
+                         double *dvx, double *dC, double *dR, double *scal, double *work, void *stream);
+
+/* SGPLVM posterior q(x) of the rows idx (int64, NULL = rows 0..B-1) from the factors x1, x2:
+ * natural parameters prior + (x1, exp(2 x2)) when nat_param, else mean x1 / exp(2 x2) and
+ * variance 1 / exp(2 x2).  Replaces Base_SGPLVM.get_posterior_x (geepee/base_models.py:895-911)
+ * as used by the VFE SGPLVM (geepee/vfe_models.py:772-780). */
+int gpx_lvm_x_post(int B, int Q, const long *idx, const double *x1, const double *x2, double prior_mean,
+                   double prior_var, int nat_param, double *mx, double *vx, void *stream);
+
+/* SGPLVM: scal[0] = scale_x * KL(q(x_idx) || N(prior_mean, prior_var)) and the gradients of
+ * the rows idx w.r.t. x1, x2 from dmx, dvx plus scale_x times the KL's (rows outside idx are
+ * not written).  Replaces SGPLVM.compute_KL_x (geepee/vfe_models.py:847-864) and
+ * Base_SGPLVM.compute_posterior_grad_x (geepee/base_models.py:913-929); 130 doubles in scal. */
+int gpx_lvm_vfe_x_grads(int B, int Q, const long *idx, const double *x1, const double *x2, const double *dmx,
+                        const double *dvx, double prior_mean, double prior_var, int nat_param, double scale_x,
+                        double *g1, double *g2, double *scal, void *stream);
+
+/* SGPSSM: entropy term scal[0] = scale_ent (Be Q (1 + log 2 pi) / 2 + sum log(var) / 2) of the
+ * window [s, e) of the latent path and the gradients w.r.t. its factors (rows s..e-1 written):
+ * dup / vup the emission-input grads (Be rows, leading dim ldu), dmt / dvt the
+ * transition-target grads (Be-1 rows, leading dim Q), dmn / dvn the dynamics-input grads
+ * (Be-1 rows, leading dim ldn).  Replaces geepee/vfe_models.py:1030-1060 with
+ * Base_SGPSSM.compute_posterior_grad_x (geepee/base_models.py:1730-1752); 130 doubles in scal. */
+int gpx_ssm_vfe_x_grads(int N, int Q, int s, int e, const double *xf1, const double *x2, double prior_mean,
+                        double prior_var, const double *dup, const double *vup, int ldu, const double *dmt,
+                        const double *dvt, const double *dmn, const double *dvn, int ldn, double scale_ent,
+                        double *g1, double *g2, double *scal, void *stream);
+
 /* ---- fused moment-matched layer (psi2 never written to HBM) -------------------- */
 
 /* Symmetrised Bhat in pair-major layout, out[a, b, d] = (B[d,a,b] + B[d,b,a]) / 2: the operand
