@@ -50,86 +50,118 @@ __global__ __launch_bounds__(256) void psi1_kernel(int N, int M, int Q, const do
   psi1[e] = sf2 * exp(acc + lc);
 }
 
-// One block per (tile pair ta >= tb of 32 x 32 entries, group of PSI2_PPB points): the
-// z tiles are staged once and reused for every point of the group, the per-point
-// constants are formed in parallel, and an off-diagonal tile is mirrored through LDS
-// (two buffers, one barrier per point) so both triangles are stored as contiguous rows.
-constexpr int PSI2_PPB = 8;
+// One block per (32 x 32 tile pair ta >= tb, chunk of PSI2_CH points).  Wave w computes
+// the 16 x 16 block (w >> 1, w & 1) of the tile for every point of the chunk, lane l the
+// entries (row (l >> 4) + 4r, column l & 15), r < 4.  The per-pair terms zh_q = (z_aq +
+// z_bq)/2 and P_ab = sum_q (z_aq - z_bq)^2 / (4 l_q) stay in registers across the points,
+// the points' constants are formed once per block in LDS, and an off-diagonal tile's
+// mirrored block goes out through a wave-private LDS transpose: after the set-up no wave
+// waits for another, and every store writes 128-byte row segments.
+// QC: compile-time bound on Q (register arrays).
+constexpr int PSI2_CH = 16;
 
+template <int QC>
 __global__ __launch_bounds__(256) void psi2_kernel(int N, int M, int Q, const double *__restrict__ mx,
                                                     const double *__restrict__ vx,
                                                     const double *__restrict__ z,
                                                     const double *__restrict__ ls,
                                                     const double *__restrict__ sf,
                                                     double *__restrict__ psi2) {
-  __shared__ double za[32][GPX_QMAX + 1], zb[32][GPX_QMAX + 1];
-  __shared__ double mu[PSI2_PPB][GPX_QMAX], dd[PSI2_PPB][GPX_QMAX], lq[PSI2_PPB][GPX_QMAX];
-  __shared__ double lam[GPX_QMAX], i4l[GPX_QMAX];
-  __shared__ double tile[2][32][33];
-  // decode the tile pair index p -> (ta, tb), ta >= tb
+  __shared__ double pmu[PSI2_CH][QC], pdd[PSI2_CH][QC], plc[PSI2_CH];
+  __shared__ double lam[QC], i4l[QC];
+  __shared__ double tr[4][16][17];
   int p = blockIdx.x, ta = 0;
   while (p > ta) { p -= ta + 1; ++ta; }
   const int tb = p;
   const int a0 = ta * 32, b0 = tb * 32, tid = threadIdx.x;
-  for (int e = tid; e < 32 * Q; e += 256) {
-    const int i = e / Q, q = e % Q;
-    za[i][q] = (a0 + i < M) ? z[(a0 + i) * Q + q] : 0.0;
-    zb[i][q] = (b0 + i < M) ? z[(b0 + i) * Q + q] : 0.0;
-  }
+  const int n0 = blockIdx.y * PSI2_CH, nend = min(N, n0 + PSI2_CH);
   if (tid < Q) {
     const double l = exp(2.0 * ls[tid]);
     lam[tid] = l;
     i4l[tid] = 1.0 / (4.0 * l);
   }
   __syncthreads();
-  const int j = tid & 31;
+  // point constants: mu, d = 1/(2v + l) (zero beyond Q, so the loops below run to QC
+  // without branches) and lc = sum_q log(l / (l + 2v)) / 2
+  for (int e = tid; e < PSI2_CH * QC; e += 256) {
+    const int pt = e / QC, q = e % QC, n = min(n0 + pt, N - 1);
+    const bool in = q < Q;
+    pmu[pt][q] = in ? mx[(long)n * Q + q] : 0.0;
+    pdd[pt][q] = in ? 1.0 / (2.0 * vx[(long)n * Q + q] + lam[q]) : 0.0;
+  }
+  if (tid < PSI2_CH) {
+    const int n = min(n0 + tid, N - 1);
+    double lc = 0.0;
+    for (int q = 0; q < Q; ++q) lc += 0.5 * log(lam[q] / (lam[q] + 2.0 * vx[(long)n * Q + q]));
+    plc[tid] = lc;
+  }
+  __syncthreads();
+  const int lane = tid & 63, w = tid >> 6;
+  const int ra = (w >> 1) * 16 + (lane >> 4), cb = (w & 1) * 16 + (lane & 15);
+  const int b = b0 + cb, bl = min(b, M - 1);
+  double zh[4][QC], P[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int al = min(a0 + ra + 4 * r, M - 1);  // padding rows read a real z row; never stored
+    double pp = 0.0;
+#pragma unroll
+    for (int q = 0; q < QC; ++q) {
+      if (q < Q) {
+        const double x = z[al * Q + q], y = z[bl * Q + q];
+        const double dz = x - y;
+        zh[r][q] = 0.5 * (x + y);
+        pp += dz * dz * i4l[q];
+      } else {
+        zh[r][q] = 0.0;
+      }
+    }
+    P[r] = pp;
+  }
   const double sf2 = exp(2.0 * sf[0]), sf4 = sf2 * sf2;
   const bool mirror = ta != tb;
-  const int ngroups = (N + PSI2_PPB - 1) / PSI2_PPB;
-  int buf = 0;
-  for (int g = blockIdx.y; g < ngroups; g += gridDim.y) {
-    const int nb = g * PSI2_PPB;
-    if (tid < PSI2_PPB * Q) {
-      const int pt = tid / Q, q = tid % Q, n = nb + pt;
-      if (n < N) {
-        const double v = vx[(long)n * Q + q], l = lam[q];
-        mu[pt][q] = mx[(long)n * Q + q];
-        dd[pt][q] = 1.0 / (2.0 * v + l);
-        lq[pt][q] = 0.5 * log(l / (l + 2.0 * v));
-      }
-    }
-    __syncthreads();
-    for (int pt = 0; pt < PSI2_PPB; ++pt) {
-      const int n = nb + pt;
-      if (n >= N) break;
-      double lc = 0.0;
-      for (int q = 0; q < Q; ++q) lc += lq[pt][q];
-      double *out = psi2 + (long)n * M * M;
+  // mirrored block: lane l writes row b0 + (w & 1) * 16 + (l >> 4) + 4r, columns
+  // a0 + (w >> 1) * 16 + (l & 15)
+  const int mb = b0 + (w & 1) * 16 + (lane >> 4), ma = a0 + (w >> 1) * 16 + (lane & 15);
+  for (int n = n0; n < nend; ++n) {
+    const int pt = n - n0;
+    const double lc = plc[pt];
+    double *out = psi2 + (long)n * M * M;
+    double val[4], m[QC], d[QC];
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const int i = (tid >> 5) + 8 * s;
-        double e = lc;
-        for (int q = 0; q < Q; ++q) {
-          const double x = za[i][q], y = zb[j][q];
-          const double dz = x - y, h = mu[pt][q] - 0.5 * (x + y);
-          e -= dz * dz * i4l[q] + h * h * dd[pt][q];
-        }
-        const double val = sf4 * exp(e);
-        tile[buf][i][j] = val;
-        if (a0 + i < M && b0 + j < M) __builtin_nontemporal_store(val, out + (long)(a0 + i) * M + b0 + j);
-      }
-      if (mirror) {
-        __syncthreads();
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          const int i = (tid >> 5) + 8 * s;  // row within the mirrored tile (b index)
-          if (b0 + i < M && a0 + j < M)
-            __builtin_nontemporal_store(tile[buf][j][i], out + (long)(b0 + i) * M + a0 + j);
-        }
-        buf ^= 1;
-      }
+    for (int q = 0; q < QC; ++q) {
+      m[q] = pmu[pt][q];
+      d[q] = pdd[pt][q];
     }
-    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      double e = lc - P[r];
+#pragma unroll
+      for (int q = 0; q < QC; ++q) {
+        const double h = m[q] - zh[r][q];
+        e -= h * h * d[q];
+      }
+      val[r] = sf4 * exp(e);
+      const int a = a0 + ra + 4 * r;
+      if (a < M && b < M) __builtin_nontemporal_store(val[r], out + (long)a * M + b);
+    }
+    if (mirror) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) tr[w][(lane >> 4) + 4 * r][lane & 15] = val[r];
+      // the wave reads what its other lanes wrote: order the LDS accesses at wave scope
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int bb = mb + 4 * r;
+        const double v = tr[w][lane & 15][(lane >> 4) + 4 * r];
+        if (bb < M && ma < M) __builtin_nontemporal_store(v, out + (long)bb * M + ma);
+      }
+      // and before the next point overwrites the buffer
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
   }
 }
 
@@ -374,10 +406,15 @@ int gpx_psi_fwd(int N, int M, int Q, const double *mx, const double *vx, const d
   GPX_CHECK_LAUNCH();
   if (psi2) {
     const int T = (M + 31) / 32;
-    unsigned groups = gpx_cdiv(N, PSI2_PPB);
-    if (groups > 65535) groups = 65535;
-    hipLaunchKernelGGL(psi2_kernel, dim3(T * (T + 1) / 2, groups), dim3(256), 0, st, N, M, Q, mx, vx, z,
-                       ls, sf, psi2);
+    const unsigned chunks = gpx_cdiv(N, PSI2_CH);
+    if (chunks > 65535) return GPX_EINVAL;
+    const dim3 grid(T * (T + 1) / 2, chunks);
+    if (Q <= 4)
+      hipLaunchKernelGGL(psi2_kernel<4>, grid, dim3(256), 0, st, N, M, Q, mx, vx, z, ls, sf, psi2);
+    else if (Q <= 8)
+      hipLaunchKernelGGL(psi2_kernel<8>, grid, dim3(256), 0, st, N, M, Q, mx, vx, z, ls, sf, psi2);
+    else
+      hipLaunchKernelGGL(psi2_kernel<GPX_QMAX>, grid, dim3(256), 0, st, N, M, Q, mx, vx, z, ls, sf, psi2);
     GPX_CHECK_LAUNCH();
   }
   return GPX_OK;

@@ -172,7 +172,9 @@ __global__ __launch_bounds__(NW * 64) void spd_gj_kernel(int M, int batch, const
   __shared__ SpdLds<MT> sm;
   const int b = blockIdx.x;
   if (b >= batch) return;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  // w in a scalar register: the tile indices and their LDS offsets are then wave-uniform
+  // scalars rather than a few vector registers per tile
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lr = lane >> 4, lc = lane & 15;
   int fail = 0;
   const double *Ab = A + b * sA;
@@ -186,6 +188,7 @@ __global__ __launch_bounds__(NW * 64) void spd_gj_kernel(int M, int batch, const
       acc[u][r] = (t < T && i < M && j < M) ? Ab[(long)i * lda + j] : (i == j ? 1.0 : 0.0);
     }
   }
+#pragma unroll 1
   for (int K = 0; K < MT; ++K) {
     // (1) block row and column K to LDS
 #pragma unroll
@@ -201,23 +204,30 @@ __global__ __launch_bounds__(NW * 64) void spd_gj_kernel(int M, int batch, const
       fail = sm.failk[K & 1];
       break;
     }
-    // (3) the tiles
+    // (3) the tiles, one tile's operands at a time (hoisting the LDS loads of later tiles
+    // above this one's MFMAs would hold all of them in registers at once)
 #pragma unroll
     for (int u = 0; u < TPW; ++u) {
       const int t = w + NW * u;
       if (t < T) spd_update<MT>(sm, K, t / MT, t % MT, acc[u], lr, lc);
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
   spd_finish<MT, NW>(sm, M, b, fail, logdet, info);
   if (!out) return;
   double *Ob = out + b * sO;
+  // the store indices are formed afresh from an opaque copy of the lane: otherwise the
+  // compiler keeps the load's per-tile indices live through the whole elimination
+  int lane2 = lane;
+  asm volatile("" : "+v"(lane2));
+  const int lr2 = lane2 >> 4, lc2 = lane2 & 15;
 #pragma unroll
   for (int u = 0; u < TPW; ++u) {
     const int t = w + NW * u, I = t / MT, J = t % MT;
     if (t >= T) continue;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int i = I * 16 + lr + 4 * r, j = J * 16 + lc;
+      const int i = I * 16 + lr2 + 4 * r, j = J * 16 + lc2;
       if (i < M && j < M) Ob[(long)i * ldo + j] = fail ? NAN : acc[u][r];
     }
   }
