@@ -303,12 +303,32 @@ def run_copy(args):
             'bytes_read_per_launch': 8 * n, 'bytes_written_per_launch': 8 * n, 'ms_per_step': dt * 1e3}
 
 
+def run_fill(args):
+    """HBM write ceiling: gpx_lincomb broadcasting one value into 64 Mi doubles (512 MiB
+    written per launch, one double read), the reference rate for write-bound kernels."""
+    import torch
+    from geepee_amd import dops
+    n = 64 << 20
+    one = torch.ones(1, dtype=torch.float64, device='cuda')
+    out = torch.empty(n, dtype=torch.float64, device='cuda')
+    for _ in range(args.warmup):
+        dops.lincomb(out, 1.0, one)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dops.lincomb(out, 1.0, one)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    return {'metric': 'fill GB/s (HBM write ceiling)', 'value': 8.0 * n / dt / 1e9, 'unit': 'GB/s',
+            'bytes_written_per_launch': 8 * n, 'ms_per_step': dt * 1e3}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=10)
     ap.add_argument('--warmup', type=int, default=2)
-    ap.add_argument('--workload', default='sgplvm', choices=sorted(WORKLOADS) + ['serving', 'copy'])
+    ap.add_argument('--workload', default='sgplvm', choices=sorted(WORKLOADS) + ['serving', 'copy', 'fill'])
     ap.add_argument('--path', default='fused', choices=['plain', 'fused', 'graph'])
     ap.add_argument('--probe', default=None,
                     help='C-ABI entry point timed for the roofline (default: the path\'s dominant one)')
@@ -323,8 +343,8 @@ def main():
         print('bench.py measures one GPU only: this build runs the hot path on a single MI355X '
               '(the multi-GPU sharding is a design note); rerun with --gpus 1', file=sys.stderr)
         sys.exit(2)
-    if args.workload == 'copy':
-        print(json.dumps(run_copy(args)), flush=True)
+    if args.workload in ('copy', 'fill'):
+        print(json.dumps(run_copy(args) if args.workload == 'copy' else run_fill(args)), flush=True)
         return
     if args.workload == 'serving':
         r = run_serving(args)

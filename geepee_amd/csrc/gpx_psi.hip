@@ -29,25 +29,45 @@ namespace {
 // forward
 // ---------------------------------------------------------------------------
 
+// PSI1_PTS points per block: their constants (w = 1 / (2 (v + l)) per q, and
+// lc = sum_q log(l / (l + v)) / 2) are formed once in LDS, then thread (point, m) sums
+// its Q terms; a row of M entries is written contiguously.
+constexpr int PSI1_PTS = 4;
+
 __global__ __launch_bounds__(256) void psi1_kernel(int N, int M, int Q, const double *__restrict__ mx,
                                                     const double *__restrict__ vx,
                                                     const double *__restrict__ z,
                                                     const double *__restrict__ ls,
                                                     const double *__restrict__ sf,
                                                     double *__restrict__ psi1) {
-  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (e >= (long)N * M) return;
-  const int n = (int)(e / M), m = (int)(e % M);
-  const double sf2 = exp(2.0 * sf[0]);
-  double acc = 0.0, lc = 0.0;
-  for (int q = 0; q < Q; ++q) {
-    const double l = exp(2.0 * ls[q]);
+  __shared__ double lam[GPX_QMAX], pw[PSI1_PTS][GPX_QMAX], pm[PSI1_PTS][GPX_QMAX], plc[PSI1_PTS];
+  const int tid = threadIdx.x, n0 = blockIdx.x * PSI1_PTS;
+  if (tid < Q) lam[tid] = exp(2.0 * ls[tid]);
+  __syncthreads();
+  if (tid < PSI1_PTS * Q) {
+    const int pt = tid / Q, q = tid % Q, n = min(n0 + pt, N - 1);
     const double v = vx[(long)n * Q + q];
-    const double d = mx[(long)n * Q + q] - z[m * Q + q];
-    acc += -d * d / (2.0 * (v + l));
-    lc += 0.5 * log(l / (l + v));
+    pw[pt][q] = 1.0 / (2.0 * (v + lam[q]));
+    pm[pt][q] = mx[(long)n * Q + q];
   }
-  psi1[e] = sf2 * exp(acc + lc);
+  if (tid < PSI1_PTS) {
+    const int n = min(n0 + tid, N - 1);
+    double lc = 0.0;
+    for (int q = 0; q < Q; ++q) lc += 0.5 * log(lam[q] / (lam[q] + vx[(long)n * Q + q]));
+    plc[tid] = lc;
+  }
+  __syncthreads();
+  const double sf2 = exp(2.0 * sf[0]);
+  for (int e = tid; e < PSI1_PTS * M; e += 256) {
+    const int pt = e / M, m = e % M, n = n0 + pt;
+    if (n >= N) break;
+    double acc = 0.0;
+    for (int q = 0; q < Q; ++q) {
+      const double d = pm[pt][q] - z[m * Q + q];
+      acc -= d * d * pw[pt][q];
+    }
+    psi1[(long)n * M + m] = sf2 * exp(acc + plc[pt]);
+  }
 }
 
 // One block per (32 x 32 tile pair ta >= tb, chunk of PSI2_CH points).  Wave w computes
@@ -66,8 +86,10 @@ __global__ __launch_bounds__(256) void psi2_kernel(int N, int M, int Q, const do
                                                     const double *__restrict__ z,
                                                     const double *__restrict__ ls,
                                                     const double *__restrict__ sf,
+                                                    double *__restrict__ psi1,
                                                     double *__restrict__ psi2) {
   __shared__ double pmu[PSI2_CH][QC], pdd[PSI2_CH][QC], plc[PSI2_CH];
+  __shared__ double pw1[PSI2_CH][QC], plc1[PSI2_CH];  // psi1 constants (tile pair 0 only)
   __shared__ double lam[QC], i4l[QC];
   __shared__ double tr[4][16][17];
   int p = blockIdx.x, ta = 0;
@@ -95,7 +117,34 @@ __global__ __launch_bounds__(256) void psi2_kernel(int N, int M, int Q, const do
     for (int q = 0; q < Q; ++q) lc += 0.5 * log(lam[q] / (lam[q] + 2.0 * vx[(long)n * Q + q]));
     plc[tid] = lc;
   }
+  // the blocks of tile pair 0 also write psi1 for their points (one launch for both)
+  const bool do_psi1 = blockIdx.x == 0;
+  if (do_psi1) {
+    for (int e = tid; e < PSI2_CH * Q; e += 256) {
+      const int pt = e / Q, q = e % Q, n = min(n0 + pt, N - 1);
+      pw1[pt][q] = 1.0 / (2.0 * (vx[(long)n * Q + q] + lam[q]));
+    }
+    if (tid >= 64 && tid < 64 + PSI2_CH) {
+      const int pt = tid - 64, n = min(n0 + pt, N - 1);
+      double lc = 0.0;
+      for (int q = 0; q < Q; ++q) lc += 0.5 * log(lam[q] / (lam[q] + vx[(long)n * Q + q]));
+      plc1[pt] = lc;
+    }
+  }
   __syncthreads();
+  const double sf2 = exp(2.0 * sf[0]), sf4 = sf2 * sf2;
+  if (do_psi1) {
+    for (int e = tid; e < PSI2_CH * M; e += 256) {
+      const int pt = e / M, m = e % M, n = n0 + pt;
+      if (n >= nend) break;
+      double acc = 0.0;
+      for (int q = 0; q < Q; ++q) {
+        const double d = pmu[pt][q] - z[m * Q + q];
+        acc -= d * d * pw1[pt][q];
+      }
+      psi1[(long)n * M + m] = sf2 * exp(acc + plc1[pt]);
+    }
+  }
   const int lane = tid & 63, w = tid >> 6;
   const int ra = (w >> 1) * 16 + (lane >> 4), cb = (w & 1) * 16 + (lane & 15);
   const int b = b0 + cb, bl = min(b, M - 1);
@@ -117,7 +166,6 @@ __global__ __launch_bounds__(256) void psi2_kernel(int N, int M, int Q, const do
     }
     P[r] = pp;
   }
-  const double sf2 = exp(2.0 * sf[0]), sf4 = sf2 * sf2;
   const bool mirror = ta != tb;
   // mirrored block: lane l writes row b0 + (w & 1) * 16 + (l >> 4) + 4r, columns
   // a0 + (w >> 1) * 16 + (l & 15)
@@ -401,20 +449,21 @@ int gpx_psi_fwd(int N, int M, int Q, const double *mx, const double *vx, const d
   if (N == 0 || M == 0) return GPX_OK;
   if (!mx || !vx || !z || !ls || !sf || !psi1) return GPX_EINVAL;
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(psi1_kernel, dim3(gpx_cdiv((long)N * M, 256)), dim3(256), 0, st, N, M, Q, mx, vx,
-                     z, ls, sf, psi1);
-  GPX_CHECK_LAUNCH();
-  if (psi2) {
+  if (!psi2) {
+    hipLaunchKernelGGL(psi1_kernel, dim3(gpx_cdiv(N, PSI1_PTS)), dim3(256), 0, st, N, M, Q, mx, vx, z, ls, sf,
+                       psi1);
+    GPX_CHECK_LAUNCH();
+  } else {
     const int T = (M + 31) / 32;
     const unsigned chunks = gpx_cdiv(N, PSI2_CH);
     if (chunks > 65535) return GPX_EINVAL;
     const dim3 grid(T * (T + 1) / 2, chunks);
     if (Q <= 4)
-      hipLaunchKernelGGL(psi2_kernel<4>, grid, dim3(256), 0, st, N, M, Q, mx, vx, z, ls, sf, psi2);
+      hipLaunchKernelGGL(psi2_kernel<4>, grid, dim3(256), 0, st, N, M, Q, mx, vx, z, ls, sf, psi1, psi2);
     else if (Q <= 8)
-      hipLaunchKernelGGL(psi2_kernel<8>, grid, dim3(256), 0, st, N, M, Q, mx, vx, z, ls, sf, psi2);
+      hipLaunchKernelGGL(psi2_kernel<8>, grid, dim3(256), 0, st, N, M, Q, mx, vx, z, ls, sf, psi1, psi2);
     else
-      hipLaunchKernelGGL(psi2_kernel<GPX_QMAX>, grid, dim3(256), 0, st, N, M, Q, mx, vx, z, ls, sf, psi2);
+      hipLaunchKernelGGL(psi2_kernel<GPX_QMAX>, grid, dim3(256), 0, st, N, M, Q, mx, vx, z, ls, sf, psi1, psi2);
     GPX_CHECK_LAUNCH();
   }
   return GPX_OK;
