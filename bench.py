@@ -37,7 +37,7 @@ FP64_PEAK_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
 # shapes.  On MI355X a plain copy of 512 MiB reads as FETCH_SIZE 262,200 KiB and
 # WRITE_SIZE 524,288 KiB, so FETCH_SIZE is scaled by 2.0 and WRITE_SIZE by 1.0.
 TRAFFIC = {
-    ('sgplvm', 'gpx_psi_fwd'): 2.0 * 2452 * 1024 + 1.0 * 528384 * 1024,
+    ('sgplvm', 'gpx_psi_fwd'): 2.0 * 2185 * 1024 + 1.0 * 528384 * 1024,
     ('sgplvm', 'gpx_fused_mm_bwd'): 2.0 * 66138 * 1024 + 1.0 * 78483 * 1024,
 }
 
