@@ -74,6 +74,21 @@ __device__ __forceinline__ void gemm_store(const double *ra, const double *rb, d
   }
 }
 
+// Block -> (column tile, row tile, z) of a batched product (z = batch element, or batch
+// element and split).  With xcd the launch is 1-D and all the tiles of one z run on one
+// XCD: blocks are dealt to the eight XCDs round-robin by id, so block p runs on XCD p % 8;
+// z takes XCD z % 8 and its tiles are the blocks p = 8 (tpz (z / 8) + t) + z % 8.  The
+// operand panels that a z's tiles re-read then stay in that XCD's L2.
+struct TileIdx {
+  int x, y, z;
+};
+__device__ __forceinline__ TileIdx gemm_tile(int xcd, int tx, int ty) {
+  if (!xcd) return {(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z};
+  const int tpz = tx * ty, p = blockIdx.x, j = p >> 3;
+  const int t = j % tpz;
+  return {t % tx, t / tx, (p & 7) + 8 * (j / tpz)};
+}
+
 // 4 waves as 2x2; each wave owns (BM/2) x (BN/2) of 16x16 MFMA tiles.  K advances in
 // slices of BK through two LDS buffers; the next slice is fetched from global memory into
 // registers while the current one feeds the matrix cores (one barrier per slice).
@@ -83,17 +98,18 @@ __global__ __launch_bounds__(256) void gemm_kernel(
     const double *__restrict__ A, int lda, long sA,
     const double *__restrict__ B, int ldb, long sB,
     GemmEpi ep, double *__restrict__ C, int ldc, long sC,
-    int batch, int splits, int kchunk, double *__restrict__ part) {
+    int batch, int splits, int kchunk, double *__restrict__ part, int xcd) {
   constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
   constexpr int LA = BM * BK / 256, LB = BN * BK / 256;
   __shared__ double As[2][BK][BM + 1];
   __shared__ double Bs[2][BK][BN + 1];
-  const int z = blockIdx.z;
+  const TileIdx ti = gemm_tile(xcd, (n + BN - 1) / BN, (m + BM - 1) / BM);
+  const int z = ti.z;
   const int bidx = z / splits, sidx = z % splits;
   if (bidx >= batch) return;
   A += bidx * sA;
   B += bidx * sB;
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int m0 = ti.y * BM, n0 = ti.x * BN;
   const int kbeg = sidx * kchunk, kend = min(k, kbeg + kchunk);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -181,14 +197,15 @@ __global__ __launch_bounds__(256) void gemm_fullk_kernel(
     int m, int n, int k, int tA, int tB, double alpha,
     const double *__restrict__ A, int lda, long sA,
     const double *__restrict__ B, int ldb, long sB,
-    GemmEpi ep, double *__restrict__ C, int ldc, long sC, int batch) {
+    GemmEpi ep, double *__restrict__ C, int ldc, long sC, int batch, int xcd) {
   __shared__ double As[FULLK_MAX][33];
   __shared__ double Bs[FULLK_MAX][33];
-  const int bidx = blockIdx.z;
+  const TileIdx ti = gemm_tile(xcd, (n + 31) / 32, (m + 31) / 32);
+  const int bidx = ti.z;
   if (bidx >= batch) return;
   A += bidx * sA;
   B += bidx * sB;
-  const int m0 = blockIdx.y * 32, n0 = blockIdx.x * 32;
+  const int m0 = ti.y * 32, n0 = ti.x * 32;
   const int kp = (k + 3) & ~3;
   const int tid = threadIdx.x;
   // all loads first (16 + 16 per thread in flight), then the LDS stores.  Thread t covers
@@ -319,9 +336,11 @@ int gpx_gemm_batched_ex(int transA, int transB, int m, int n, int k, double alph
   if (splits == 1 && k <= FULLK_MAX && tile != 64 &&
       (tile == 32 || (long)gpx_cdiv(m, 64) * gpx_cdiv(n, 64) * batch < 256)) {
     if (batch > 65535) return GPX_EINVAL;
-    dim3 grid(gpx_cdiv(n, 32), gpx_cdiv(m, 32), batch);
+    const int xcd = batch >= 8;
+    const unsigned tpz = gpx_cdiv(n, 32) * gpx_cdiv(m, 32);
+    dim3 grid = xcd ? dim3(8 * gpx_cdiv(batch, 8) * tpz) : dim3(gpx_cdiv(n, 32), gpx_cdiv(m, 32), batch);
     hipLaunchKernelGGL(gemm_fullk_kernel, grid, dim3(256), 0, st, m, n, k, transA, transB, alpha, A, lda, strideA,
-                       B, ldb, strideB, ep, C, ldc, strideC, batch);
+                       B, ldb, strideB, ep, C, ldc, strideC, batch, xcd);
     GPX_CHECK_LAUNCH();
     return GPX_OK;
   }
@@ -330,14 +349,17 @@ int gpx_gemm_batched_ex(int transA, int transB, int m, int n, int k, double alph
   splits = (k + kchunk - 1) / kchunk;
   double *part = splits > 1 ? work : nullptr;
   if ((long)batch * splits > 65535L) return GPX_EINVAL;
+  const int nz = batch * splits, xcd = nz >= 8;
   if (tile == 32 || (tile == 0 && (long)gpx_cdiv(m, 64) * gpx_cdiv(n, 64) * batch * splits < 256)) {
-    dim3 grid(gpx_cdiv(n, 32), gpx_cdiv(m, 32), batch * splits);
+    const unsigned tpz = gpx_cdiv(n, 32) * gpx_cdiv(m, 32);
+    dim3 grid = xcd ? dim3(8 * gpx_cdiv(nz, 8) * tpz) : dim3(gpx_cdiv(n, 32), gpx_cdiv(m, 32), nz);
     hipLaunchKernelGGL((gemm_kernel<32, 32>), grid, dim3(256), 0, st, m, n, k, transA, transB, alpha,
-                       A, lda, strideA, B, ldb, strideB, ep, C, ldc, strideC, batch, splits, kchunk, part);
+                       A, lda, strideA, B, ldb, strideB, ep, C, ldc, strideC, batch, splits, kchunk, part, xcd);
   } else {
-    dim3 grid(gpx_cdiv(n, 64), gpx_cdiv(m, 64), batch * splits);
+    const unsigned tpz = gpx_cdiv(n, 64) * gpx_cdiv(m, 64);
+    dim3 grid = xcd ? dim3(8 * gpx_cdiv(nz, 8) * tpz) : dim3(gpx_cdiv(n, 64), gpx_cdiv(m, 64), nz);
     hipLaunchKernelGGL((gemm_kernel<64, 64>), grid, dim3(256), 0, st, m, n, k, transA, transB, alpha,
-                       A, lda, strideA, B, ldb, strideB, ep, C, ldc, strideC, batch, splits, kchunk, part);
+                       A, lda, strideA, B, ldb, strideB, ep, C, ldc, strideC, batch, splits, kchunk, part, xcd);
   }
   GPX_CHECK_LAUNCH();
   if (part) {
