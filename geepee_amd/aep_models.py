@@ -500,11 +500,15 @@ class SDGPR(Base_Model):
             return phis, None
         if getattr(self, '_streams', None) is None:
             self._streams = [torch.cuda.Stream() for _ in range(self.L)]
+        # The SPD inverses of the layers that share an M run batched: one launch for their
+        # Kuu, one for their [posterior; cavity] precisions (each launch takes about as
+        # long for one matrix as for many, one workgroup per matrix).
+        precs = self._batched_inverses(dp, alpha)
         phis, ready = [], []
         for i, (layer, st) in enumerate(zip(self.sgp_layers, self._streams)):
             st.wait_stream(cur)
             with torch.cuda.stream(st):
-                layer.prepare(dp, alpha, key_suffix='_%d' % i)
+                layer.finish_prepare(*precs[i], alpha=alpha)
                 phis.append(layer.compute_phi(alpha))
                 ev = torch.cuda.Event()
                 ev.record(st)
@@ -513,6 +517,35 @@ class SDGPR(Base_Model):
                 # waits only for the q(u) set-up above, the layer's backward for these
                 layer.adjoint_constants('reg' if i == 0 else 'mm', alpha)
         return phis, ready
+
+    def _batched_inverses(self, dp, alpha):
+        """prepare's Kuu and precision inverses for every layer, batched over the layers
+        of each M; returns per layer (prec, cov, logdet, info) views for finish_prepare."""
+        layers = self.sgp_layers
+        for i, layer in enumerate(layers):
+            layer.prepare_params(dp, key_suffix='_%d' % i)
+        groups = {}
+        for i, layer in enumerate(layers):
+            groups.setdefault(layer.M, []).append(i)
+        precs = {}
+        for M, idx in groups.items():
+            K = dops.empty(len(idx), M, M, like=layers[idx[0]].zu)
+            for g, i in enumerate(idx):
+                layers[i].kuu_into(K[g])
+            Ki, ld, info = dops.spd_inv(K)
+            for g, i in enumerate(idx):
+                layers[i].set_kuu_inverse(Ki[g], ld[g:g + 1], info[g:g + 1])
+            sizes = [2 * layers[i].Dout for i in idx]
+            prec = dops.empty(sum(sizes), M, M, like=K)
+            o, spans = 0, []
+            for i, n in zip(idx, sizes):
+                layers[i].precisions_into(prec[o:o + n], alpha)
+                spans.append((i, o, n))
+                o += n
+            cov, ldp, infop = dops.spd_inv(prec)
+            for i, o, n in spans:
+                precs[i] = (prec[o:o + n], cov[o:o + n], ldp[o:o + n], infop[o:o + n])
+        return precs
 
     def predict_f(self, inputs, prop_mode=PROP_MM, no_samples=200):
         """Base_SDGPR.predict_f (base_models.py:1132-1138): moment-matched propagation

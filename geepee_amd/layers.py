@@ -87,21 +87,44 @@ class SGP_Layer(object):
             self.update_hypers(params, key_suffix)
             self.compute_cavity(alpha)
             return
+        self.prepare_params(params, key_suffix)
+        self.compute_kuu()
+        prec = dops.empty(2 * self.Dout, self.M, self.M, like=self.Kuuinv)
+        self.precisions_into(prec, alpha)
+        self.finish_prepare(prec, *dops.spd_inv(prec), alpha=alpha)
+
+    # the parts of prepare (natural parameterisation), so that a model with several layers
+    # of one M can invert all their Kuu in one batched SPD launch and all their precisions
+    # in another (SDGPR._prepare_layers)
+    def prepare_params(self, params, key_suffix=''):
         self.ls = params['ls' + key_suffix]
         self.sf = params['sf' + key_suffix]
         self.zu = params['zu' + key_suffix]
         self.theta_2 = params['eta2' + key_suffix]
-        R = self.theta_1_R = dops.eta_to_R(params['eta1_R' + key_suffix], self.M)
+        self.theta_1_R = dops.eta_to_R(params['eta1_R' + key_suffix], self.M)
         self.theta_1 = None
         self._adj = None
-        self.compute_kuu()
-        D, M = self.Dout, self.M
+
+    def kuu_into(self, out):
+        """Kuu (with the jitter) into out (M, M)."""
+        self.Kuu = dops.se(self.zu, self.zu, self.ls, self.sf, jitter=JITTER, out=out)
+
+    def set_kuu_inverse(self, Kuuinv, ld, info):
+        self.Kuuinv, self.ld_Kuu, self.info_Kuu = Kuuinv, ld, info
+
+    def precisions_into(self, prec, alpha):
+        """[Kuu^-1 + theta1; Kuu^-1 + beta theta1] into prec (2D, M, M)."""
+        D = self.Dout
         beta = (self.N - alpha) * 1.0 / self.N
-        Ki = self.Kuuinv
-        prec = dops.empty(2 * D, M, M, like=Ki)
+        R, Ki = self.theta_1_R, self.Kuuinv
         mm(R, R, prec[:D], tA=True, E=Ki, gamma=1.0)             # Kuu^-1 + theta1
         mm(R, R, prec[D:], tA=True, alpha=beta, E=Ki, gamma=1.0)  # Kuu^-1 + beta theta1
-        cov, ld, info = dops.spd_inv(prec)
+
+    def finish_prepare(self, prec, cov, ld, info, alpha):
+        """The rest of prepare from the inverted precisions (views of batched buffers
+        are fine)."""
+        D, M = self.Dout, self.M
+        beta = (self.N - alpha) * 1.0 / self.N
         self.Suinv, self.Suhatinv = prec[:D], prec[D:]
         self.Su, self.Suhat = cov[:D], cov[D:]
         self.ld_Suinv, self.ld_Suhatinv = ld[:D], ld[D:]
