@@ -433,12 +433,14 @@ __global__ __launch_bounds__(256) void fused_point_feat_kernel(int N, int Q, con
 
 // ---------------------------------------------------------------------------
 // backward, per-pair pass ([n x pair] orientation): each wave owns one row block
-// (a, B-tile) and sums over the points of its range, reading the precomputed point
-// features and dv straight from memory (no LDS, no barriers):
+// (a, B-tile) and sums over the points of its range:
 //   dBhat[d, a, b] += sum_n dv_nd psi2_nab,  LW[a, b, q'] += sum_n [1 | d_nq'] L_nab
+// Two forms: for D <= 16 the waves read the precomputed point features and dv straight
+// from memory (no LDS, no barriers); for D > 16 a workgroup stages each 16-point block
+// in LDS once for its four waves (fused_bwd_pairs_staged).
 // ---------------------------------------------------------------------------
 template <int DB, int DS, int KSC, int NR>
-__global__ __launch_bounds__(256) void fused_bwd_pairs_kernel(
+__device__ __forceinline__ void bwd_pairs_body(
     int N, int M, int Q, int D, const double *__restrict__ z, const double *__restrict__ ls,
     const double *__restrict__ sf, const double *__restrict__ Bp, const double *__restrict__ dv,
     const double *__restrict__ pf, int nsplit, int nchunk, double *__restrict__ dBpart,
@@ -573,6 +575,170 @@ __global__ __launch_bounds__(256) void fused_bwd_pairs_kernel(
     }
   }
 }
+
+#define GPX_PAIRS_ARGS                                                                                   \
+  int N, int M, int Q, int D, const double *__restrict__ z, const double *__restrict__ ls,                \
+      const double *__restrict__ sf, const double *__restrict__ Bp, const double *__restrict__ dv,       \
+      const double *__restrict__ pf, int nsplit, int nchunk, double *__restrict__ dBpart,                \
+      double *__restrict__ LWpart
+
+template <int DB, int DS, int KSC, int NR>
+__global__ __launch_bounds__(256) void fused_bwd_pairs_kernel(GPX_PAIRS_ARGS) {
+  bwd_pairs_body<DB, DS, KSC, NR>(N, M, Q, D, z, ls, sf, Bp, dv, pf, nsplit, nchunk, dBpart, LWpart);
+}
+
+// The same pass held to 128 registers (four waves per SIMD, no accumulation registers):
+// faster over long point ranges (>= 16 blocks of 16 points per workgroup), slower over
+// short ones, where the lower latency of the unconstrained build wins.
+template <int DB, int DS, int KSC, int NR>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void fused_bwd_pairs_occ4_kernel(
+    GPX_PAIRS_ARGS) {
+  bwd_pairs_body<DB, DS, KSC, NR>(N, M, Q, D, z, ls, sf, Bp, dv, pf, nsplit, nchunk, dBpart, LWpart);
+}
+
+// Row k of a staged 16 x 32 dv block keeps column c at c ^ dv_swizzle(k).  The two read
+// patterns (16 rows x 4 columns, and 4 rows x 16 columns) then fall on distinct LDS banks
+// two lanes a bank: rows k and k+8 share their column permutation, and of every four
+// consecutive rows two keep the halves of the row in place and two swap them.
+__device__ __forceinline__ int dv_swizzle(int k) { return 4 * (((k & 1) << 2) | ((k >> 1) & 3)); }
+
+// Per-pair pass for 16 < D <= 32 with the point operands staged in LDS: each 16-point
+// block of pf (16 x PF_W, contiguous) and dv (16 x D) is read from memory once per
+// workgroup, coalesced, one block ahead into registers, and written to the other half
+// of a double buffer while the waves work on the current one (one barrier per block).
+// Rows past the range end are staged as zeros (weight 0 through p).  Held to three waves
+// per SIMD, which leaves no accumulation registers and fits 168 VGPRs.
+// Same operands, sums and order as bwd_pairs_body with NR = 1, hence the same results.
+template <int DB, int DS, int KSC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void fused_bwd_pairs_staged(
+    GPX_PAIRS_ARGS) {
+  constexpr int PA_W = 4 * FKS_MAX;  // exponent-operand columns of a pf row
+  __shared__ double pas[2][16][PA_W];
+  __shared__ double pqs[2][16][16];
+  __shared__ double dvs[2][16][32];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int T = (M + 15) / 16;
+  const int t = blockIdx.x >> 2, h = blockIdx.x & 3;
+  int A, B;
+  decode_tile(t, T, A, B);
+  if (16 * A + 4 * h >= M) return;  // the whole workgroup: no barrier is skipped by part of it
+  const int a = 16 * A + 4 * h + wave;
+  const bool act = a < M;           // inactive waves still stage and meet the barriers
+  const int ns = blockIdx.y;
+  const int nbeg = ns * nchunk, nend = min(N, nbeg + nchunk);
+  constexpr int KSN = KSC ? KSC : FKS_MAX;
+  const int KS = KSC ? KSC : (2 * Q + 2 + 3) / 4;
+  const int jb = 16 * B + (lane & 15);
+  const bool bvalid = jb < M && act;
+  double af[KSN], bs[DS];
+  {
+    double il4[FQ_MAX], zb[FQ_MAX], za[FQ_MAX];
+#pragma unroll
+    for (int q = 0; q < FQ_MAX; ++q) {
+      il4[q] = (q < Q) ? 0.25 * exp(-2.0 * ls[q]) : 0.0;
+      zb[q] = (q < Q && bvalid) ? z[jb * Q + q] : 0.0;
+      za[q] = (q < Q && act) ? z[a * Q + q] : 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < KSN; ++s)
+      af[s] = (s < KS && bvalid) ? pair_feature((lane >> 4) + 4 * s, Q, za, zb, il4) : 0.0;
+#pragma unroll
+    for (int s = 0; s < DS; ++s) {
+      const int d = (lane >> 4) + 4 * s;
+      bs[s] = (bvalid && d < D) ? Bp[((long)a * M + jb) * D + d] : 0.0;
+    }
+  }
+  const double sf2 = exp(2.0 * sf[0]), sf4 = sf2 * sf2;
+  gpx_d4 dacc[DB], lacc = gpx_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int db = 0; db < DB; ++db) dacc[db] = gpx_d4{0.0, 0.0, 0.0, 0.0};
+  constexpr int NPF = (16 * PF_W + 255) / 256;
+  const int sk = tid >> 4, sc = tid & 15;  // staging: dv row and column of this thread
+  double rp[NPF], rd[DB];
+  auto fetch = [&](int n0) {
+#pragma unroll
+    for (int i = 0; i < NPF; ++i) {
+      const int e = tid + 256 * i;
+      rp[i] = (e < 16 * PF_W && n0 + e / PF_W < nend) ? pf[(long)n0 * PF_W + e] : 0.0;
+    }
+#pragma unroll
+    for (int db = 0; db < DB; ++db)
+      rd[db] = (n0 + sk < nend && 16 * db + sc < D) ? dv[(long)(n0 + sk) * D + 16 * db + sc] : 0.0;
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < NPF; ++i) {
+      const int e = tid + 256 * i;
+      if (e < 16 * PF_W) {
+        const int k = e / PF_W, c = e % PF_W;
+        if (c < PA_W) pas[buf][k][c] = rp[i]; else pqs[buf][k][c - PA_W] = rp[i];
+      }
+    }
+#pragma unroll
+    for (int db = 0; db < DB; ++db) dvs[buf][sk][(16 * db + sc) ^ dv_swizzle(sk)] = rd[db];
+  };
+  fetch(nbeg);
+  stage(0);
+  __syncthreads();
+  const int ka = lane & 15, kq = lane >> 4;
+  const int swa = dv_swizzle(ka);
+  int buf = 0;
+  for (int n0 = nbeg; n0 < nend; n0 += 16) {
+    const bool more = n0 + 16 < nend;  // uniform over the workgroup
+    if (more) fetch(n0 + 16);
+    if (act) {
+      double pa[KSN], da[DS];
+#pragma unroll
+      for (int s = 0; s < KSN; ++s) pa[s] = (s < KS) ? pas[buf][ka][kq + 4 * s] : 0.0;
+#pragma unroll
+      for (int s = 0; s < DS; ++s) da[s] = dvs[buf][ka][(kq + 4 * s) ^ swa];
+      gpx_d4 e = gpx_d4{0.0, 0.0, 0.0, 0.0}, g = gpx_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < KSN; ++s)
+        if (s < KS) e = gpx_mfma16(pa[s], af[s], e);
+#pragma unroll
+      for (int s = 0; s < DS; ++s) g = gpx_mfma16(da[s], bs[s], g);
+      double dvv[4][DB], dq[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = kq + 4 * r;
+#pragma unroll
+        for (int db = 0; db < DB; ++db) dvv[r][db] = dvs[buf][k][(16 * db + ka) ^ dv_swizzle(k)];
+        dq[r] = pqs[buf][k][ka];
+      }
+      double p[4], L[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        p[r] = sf4 * exp(e[r]) * ((n0 + kq + 4 * r < nend && bvalid) ? 1.0 : 0.0);
+        L[r] = g[r] * p[r];
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+#pragma unroll
+        for (int db = 0; db < DB; ++db) dacc[db] = gpx_mfma16(dvv[r][db], p[r], dacc[db]);
+        lacc = gpx_mfma16(dq[r], L[r], lacc);
+      }
+    }
+    if (more) stage(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+  if (!act) return;
+#pragma unroll
+  for (int db = 0; db < DB; ++db)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int d = 16 * db + kq + 4 * r;
+      if (d < D && jb < M) dBpart[(((long)ns * D + d) * M + a) * M + jb] = dacc[db][r];
+    }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int qq = kq + 4 * r;
+    if (qq <= Q && jb < M) LWpart[(((long)ns * M + a) * M + jb) * (Q + 1) + qq] = lacc[r];
+  }
+}
+#undef GPX_PAIRS_ARGS
 
 // Sum of the per-pair pass's point-range partials.  One workgroup per stored 16 x 16
 // tile pair (A <= B, the half the pair kernel writes) and output slice (k < D: dBhat[k],
@@ -909,9 +1075,18 @@ int gpx_fused_mm_bwd_pairs(int N, int M, int Q, int D, const double *z, const do
   constexpr int NR = 1;  // rows per wave (one: three waves per SIMD at D = 32; two: one)
   dim3 gp(ntiles * 4 / NR, nsplit);
   const int ks = (2 * Q + 2 + 3) / 4;
-#define GPX_PAIRS(DBV, DSV, KSV)                                                                               \
-  hipLaunchKernelGGL((fused_bwd_pairs_kernel<DBV, DSV, KSV, NR>), gp, dim3(256), 0, st, N, M, Q, D, z, ls, sf, \
-                     Bp, dv, \
+  const bool occ4 = nchunk >= 256;  // long point ranges: the four-waves-per-SIMD build
+#define GPX_PAIRS(DBV, DSV, KSV)                                                                                 \
+  do {                                                                                                           \
+    if (occ4)                                                                                                    \
+      hipLaunchKernelGGL((fused_bwd_pairs_occ4_kernel<DBV, DSV, KSV, NR>), gp, dim3(256), 0, st, N, M, Q, D, z, \
+                         ls, sf, Bp, dv, (const double *)W.pf, nsplit, nchunk, W.dBpart, W.LWpart);             \
+    else                                                                                                         \
+      hipLaunchKernelGGL((fused_bwd_pairs_kernel<DBV, DSV, KSV, NR>), gp, dim3(256), 0, st, N, M, Q, D, z, ls,  \
+                         sf, Bp, dv, (const double *)W.pf, nsplit, nchunk, W.dBpart, W.LWpart);                 \
+  } while (0)
+#define GPX_PAIRS_STAGED(KSV)                                                                                   \
+  hipLaunchKernelGGL((fused_bwd_pairs_staged<2, 8, KSV>), gp, dim3(256), 0, st, N, M, Q, D, z, ls, sf, Bp, dv, \
                      (const double *)W.pf, nsplit, nchunk, W.dBpart, W.LWpart)
   // DS = ceil(D / 4) rounded up to a power of two: slices of the D contraction
   if (D <= 4) {
@@ -921,9 +1096,10 @@ int gpx_fused_mm_bwd_pairs(int N, int M, int Q, int D, const double *z, const do
   } else if (D <= 16) {
     if (ks == 3) GPX_PAIRS(1, 4, 3); else if (ks == 5) GPX_PAIRS(1, 4, 5); else GPX_PAIRS(1, 4, 0);
   } else {
-    if (ks == 3) GPX_PAIRS(2, 8, 3); else if (ks == 5) GPX_PAIRS(2, 8, 5); else GPX_PAIRS(2, 8, 0);
+    if (ks == 3) GPX_PAIRS_STAGED(3); else if (ks == 5) GPX_PAIRS_STAGED(5); else GPX_PAIRS_STAGED(0);
   }
 #undef GPX_PAIRS
+#undef GPX_PAIRS_STAGED
   GPX_CHECK_LAUNCH();
   hipLaunchKernelGGL(fused_pairs_finish, dim3(ntiles, D + 1), dim3(256), 0, st, M, Q, D, nsplit,
                      (const double *)W.dBpart, (const double *)W.LWpart, dBhat, W.LW);

@@ -38,7 +38,11 @@ def _spread(M, Q):
                                      (70, 24, 2, 8), (50, 16, 2, 16),
                                      # the corners fused_fits admits: M = 256 with Q = 15 and
                                      # D = 32 (the largest LDS request), M in (200, 256]
-                                     (70, 256, 15, 32), (45, 210, 9, 3)])
+                                     (70, 256, 15, 32), (45, 210, 9, 3),
+                                     # point ranges of >= 256 points per split (the per-pair
+                                     # pass's four-waves-per-SIMD build) and, at D > 16, 8 staged
+                                     # blocks a range with a ragged last one
+                                     (1000, 200, 4, 4), (999, 120, 2, 20)])
 def test_fused_layer_matches_plain(N, M, Q, D):
     from geepee_amd import dops
     from geepee_amd.aep_models import DeviceParams, _shapes
