@@ -37,6 +37,27 @@ __device__ __forceinline__ void decode_tile(int t, int T, int &A, int &B) {
   B = A + rem;
 }
 
+// Work items (point block, split) of the forward and per-point passes on a 1-D grid.
+// Workgroups are dealt to the 8 XCDs round-robin by linear index; when the split count S
+// divides 8, every block on XCD x serves split x mod S, so that XCD's L2 holds only that
+// split's share of the pair operands (B-hat, P_ab) rather than all of them.  Returns false
+// for the blocks past the last point block (a whole workgroup, before any barrier).
+__device__ __forceinline__ bool split_block(int nblk, int S, int &pb, int &split) {
+  const int id = blockIdx.x;
+  if (8 % S == 0) {
+    const int x = id & 7, per = 8 / S;
+    split = x % S;
+    pb = (id >> 3) * per + x / S;
+  } else {
+    split = id / nblk;
+    pb = id - split * nblk;
+  }
+  return pb < nblk;
+}
+inline unsigned split_grid(long nblk, int S) {
+  return (8 % S == 0) ? 8u * gpx_cdiv(nblk, 8 / S) : (unsigned)(nblk * S);
+}
+
 // Per-point exponent feature k (B operand of the exponent GEMM).
 __device__ __forceinline__ double point_feature(int k, int Q, const double *mu, const double *d, double c) {
   if (k < Q) return 2.0 * d[k] * mu[k];
@@ -156,7 +177,9 @@ __global__ __launch_bounds__(256) void fused_fwd_kernel(int N, int M, int Q, int
   __shared__ double zs[FM_MAX][FQ_MAX + 1];
   __shared__ double lam[FQ_MAX];
   __shared__ double red[4][DB * 16][17];
-  const int n0 = blockIdx.x * 16 * NSUB, split = blockIdx.y;
+  int pb, split;
+  if (!split_block((N + 16 * NSUB - 1) / (16 * NSUB), S, pb, split)) return;
+  const int n0 = pb * 16 * NSUB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int e = tid; e < M * Q; e += 256) zs[e / Q][e % Q] = z[e];
   if (tid < Q) lam[tid] = exp(2.0 * ls[tid]);
@@ -285,6 +308,8 @@ __global__ __launch_bounds__(256) void fused_bwd_points_kernel(
   double *Tacc = dyn + M * ZS;
   __shared__ double lam[FQ_MAX];
   __shared__ double cst[4][16][17];  // column-sum staging; reused for the V reduction
+  // (split_block's XCD-aware order measured slower here: its splits hold unequal work,
+  // column tiles B = split, split + S, ..., and whole XCDs then wait on the larger one)
   const int n0 = blockIdx.x * 16, split = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int e = tid; e < M * Q; e += 256) zs[(e / Q) * ZS + e % Q] = z[e];
@@ -888,24 +913,22 @@ __global__ __launch_bounds__(256) void fused_fwd_finish(int N, int M, int Q, int
                                                         double *__restrict__ psi1, double *__restrict__ mout,
                                                         double *__restrict__ vout) {
   __shared__ double p1[FIN_PTS][FM_MAX + 1];
-  __shared__ double den[FIN_PTS][FQ_MAX], mus[FIN_PTS][FQ_MAX], lcs[FIN_PTS];
+  __shared__ double rden[FIN_PTS][FQ_MAX], mus[FIN_PTS][FQ_MAX], lts[FIN_PTS][FQ_MAX], lcs[FIN_PTS];
   const int n0 = blockIdx.x * FIN_PTS, tid = threadIdx.x;
   const double sf2 = exp(2.0 * sf[0]);
-  // per point: 2 (v + l) and mx per dimension, and sum_q log(l / (l + v)) / 2 (the
-  // logs do not depend on the inducing point)
+  // per point and dimension (one thread each): 1 / (2 (v + l)), mx, and log(l / (l + v)) / 2,
+  // whose sum over q (in q order) is the point's constant (independent of the inducing point)
   if (tid < FIN_PTS * Q) {
     const int nn = tid / Q, q = tid % Q, n = n0 + nn;
     const double l = exp(2.0 * ls[q]), v = n < N ? vx[(long)n * Q + q] : 0.0;
-    den[nn][q] = 2.0 * (v + l);
+    rden[nn][q] = 1.0 / (2.0 * (v + l));
     mus[nn][q] = n < N ? mx[(long)n * Q + q] : 0.0;
+    lts[nn][q] = 0.5 * log(l / (l + v));
   }
+  __syncthreads();
   if (tid < FIN_PTS) {
-    const int n = n0 + tid;
     double lc = 0.0;
-    for (int q = 0; q < Q && n < N; ++q) {
-      const double l = exp(2.0 * ls[q]), v = vx[(long)n * Q + q];
-      lc += 0.5 * log(l / (l + v));
-    }
+    for (int q = 0; q < Q; ++q) lc += lts[tid][q];
     lcs[tid] = lc;
   }
   __syncthreads();
@@ -916,7 +939,7 @@ __global__ __launch_bounds__(256) void fused_fwd_finish(int N, int M, int Q, int
       double acc = 0.0;
       for (int q = 0; q < Q; ++q) {
         const double dlt = mus[nn][q] - z[m * Q + q];
-        acc += -dlt * dlt / den[nn][q];
+        acc -= dlt * dlt * rden[nn][q];
       }
       val = sf2 * exp(acc + lcs[nn]);
       psi1[(long)n * M + m] = val;
@@ -927,9 +950,18 @@ __global__ __launch_bounds__(256) void fused_fwd_finish(int N, int M, int Q, int
   for (int e = tid; e < FIN_PTS * D; e += 256) {
     const int nn = e / D, d = e % D, n = n0 + nn;
     if (n >= N) continue;
-    double m = 0.0;
-#pragma unroll 8
-    for (int k = 0; k < M; ++k) m += p1[nn][k] * Ahat[(long)d * M + k];
+    // four independent partial sums (k mod 4): a 128-long chain of dependent FMAs was
+    // this loop's latency
+    const double *arow = Ahat + (long)d * M;
+    double m4[4] = {0.0, 0.0, 0.0, 0.0};
+    int k = 0;
+#pragma unroll 2
+    for (; k + 4 <= M; k += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) m4[u] += p1[nn][k + u] * arow[k + u];
+    }
+    for (int u = 0; k < M; ++k, ++u) m4[u] += p1[nn][k] * arow[k];
+    const double m = (m4[0] + m4[1]) + (m4[2] + m4[3]);
     double s = 0.0;
     for (int sp = 0; sp < S; ++sp) s += part[((long)sp * N + n) * D + d];
     mout[(long)n * D + d] = m;
@@ -1180,7 +1212,7 @@ static int fused_fwd_launch(int N, int M, int Q, int D, const double *mx, const 
   double *Pab = work + (long)splits * N * D;
   hipLaunchKernelGGL(pair_p_kernel, dim3(gpx_cdiv((long)M * M, 256)), dim3(256), 0, st, M, Q, z, ls, Pab);
   GPX_CHECK_LAUNCH();
-  dim3 grid(gpx_cdiv(N, 16 * nsub), splits);
+  dim3 grid(split_grid(gpx_cdiv(N, 16 * nsub), splits));
   const int ks = (2 * Q + 2 + 3) / 4;  // compile-time slice counts for Q = 4, 5 (3) and 7, 8 (5)
 #define GPX_FWD1(DBV, NS, KSV)                                                                                 \
   hipLaunchKernelGGL((fused_fwd_kernel<DBV, NS, KSV>), grid, dim3(256), 0, st, N, M, Q, D, mx, vx, z, ls, sf, Bp, \

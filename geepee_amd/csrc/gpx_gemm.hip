@@ -172,18 +172,34 @@ __global__ __launch_bounds__(256) void gemm_kernel(
       }
 }
 
-__global__ void splitk_reduce(int m, int n, int batch, int splits, double alpha,
-                              const double *__restrict__ part, GemmEpi ep,
-                              double *__restrict__ C, int ldc, long sC) {
-  const long total = (long)batch * m * n;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
-       e += (long)gridDim.x * blockDim.x) {
-    const int b = (int)(e / ((long)m * n));
-    const long r = e % ((long)m * n);
+// Sum of the split-K partials part[q][b][i][j] (q < splits) and the epilogue.  A workgroup
+// takes 32 consecutive outputs; its 8 groups of 32 threads each sum the splits q = g,
+// g + 8, ... of them (coalesced rows, loads in flight together), and the 8 group sums are
+// added in group order, so every output's order of summation is fixed.  splits = 0 (k = 0)
+// stores the epilogue of a zero product.  Grid: ceil(batch*m*n / 32) workgroups of 256.
+constexpr int SPLITK_OUT = 32;
+__global__ __launch_bounds__(256) void splitk_reduce(int m, int n, int batch, int splits, double alpha,
+                                                     const double *__restrict__ part, GemmEpi ep,
+                                                     double *__restrict__ C, int ldc, long sC) {
+  __shared__ double red[8][SPLITK_OUT + 1];
+  const long mn = (long)m * n, total = (long)batch * mn;
+  const int o = threadIdx.x & (SPLITK_OUT - 1), g = threadIdx.x / SPLITK_OUT;
+  const long e = (long)blockIdx.x * SPLITK_OUT + o;
+  double s = 0.0;
+  if (e < total) {
+#pragma unroll 4
+    for (int q = g; q < splits; q += 8) s += part[(long)q * total + e];
+  }
+  red[g][o] = s;
+  __syncthreads();
+  if (g == 0 && e < total) {
+    double t = red[0][o];
+#pragma unroll
+    for (int h = 1; h < 8; ++h) t += red[h][o];
+    const int b = (int)(e / mn);
+    const long r = e % mn;
     const int i = (int)(r / n), j = (int)(r % n);
-    double s = 0.0;
-    for (int q = 0; q < splits; ++q) s += part[((long)q * batch + b) * (long)m * n + r];
-    epi_store(C + b * sC + (long)i * ldc + j, alpha * s, ep, b, i, j);
+    epi_store(C + b * sC + (long)i * ldc + j, alpha * t, ep, b, i, j);
   }
 }
 
@@ -314,8 +330,8 @@ int gpx_gemm_batched_ex(int transA, int transB, int m, int n, int k, double alph
   if (splits > 1 && !work) return GPX_EINVAL;
   hipStream_t st = (hipStream_t)stream;
   if (k == 0) {  // C = beta * C
-    hipLaunchKernelGGL(splitk_reduce, dim3(gpx_cdiv((long)batch * m * n, 256) > 4096 ? 4096 : gpx_cdiv((long)batch * m * n, 256)),
-                       dim3(256), 0, st, m, n, batch, 0, 0.0, (const double *)nullptr, ep, C, ldc, strideC);
+    hipLaunchKernelGGL(splitk_reduce, dim3(gpx_cdiv((long)batch * m * n, SPLITK_OUT)), dim3(256), 0, st, m, n, batch,
+                       0, 0.0, (const double *)nullptr, ep, C, ldc, strideC);
     GPX_CHECK_LAUNCH();
     return GPX_OK;
   }
@@ -363,10 +379,8 @@ int gpx_gemm_batched_ex(int transA, int transB, int m, int n, int k, double alph
   }
   GPX_CHECK_LAUNCH();
   if (part) {
-    long total = (long)batch * m * n;
-    unsigned g = gpx_cdiv(total, 256);
-    if (g > 4096) g = 4096;
-    hipLaunchKernelGGL(splitk_reduce, dim3(g), dim3(256), 0, st, m, n, batch, splits, alpha,
+    const long total = (long)batch * m * n;
+    hipLaunchKernelGGL(splitk_reduce, dim3(gpx_cdiv(total, SPLITK_OUT)), dim3(256), 0, st, m, n, batch, splits, alpha,
                        (const double *)part, ep, C, ldc, strideC);
     GPX_CHECK_LAUNCH();
   }

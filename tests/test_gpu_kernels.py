@@ -59,7 +59,11 @@ def say(*a):
     (5, 1, 7, 0, 0, 4, 1), (1, 300, 1, 0, 0, 1, 1),
     # matrix-vector (n = 1) and outer-product (k = 1) kernels, every transpose
     (128, 1, 128, 1, 0, 3, 1), (100, 1, 4096, 0, 1, 1, 1), (70, 1, 65, 1, 1, 2, 1),
-    (128, 128, 1, 0, 1, 4, 1), (33, 70, 1, 1, 0, 2, 1), (9, 5, 1, 1, 1, 3, 1)])
+    (128, 128, 1, 0, 1, 4, 1), (33, 70, 1, 1, 0, 2, 1), (9, 5, 1, 1, 1, 3, 1),
+    # split-K reductions: splits not a multiple of the reduction's 8 groups, more splits
+    # than groups, batched, and a ragged last output workgroup
+    (32, 128, 4096, 1, 0, 1, 32), (45, 7, 2000, 0, 1, 3, 13), (128, 128, 8192, 1, 0, 2, 64),
+    (3, 5, 700, 1, 1, 2, 5)])
 def test_gemm(m, n, k, tA, tB, batch, splits):
     L = _lib()
     say('gemm', m, n, k, tA, tB, batch, splits)
