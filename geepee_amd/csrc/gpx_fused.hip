@@ -287,7 +287,7 @@ __global__ void sym_pairmajor_kernel(int D, int M, const double *__restrict__ B,
 
 // ---------------------------------------------------------------------------
 // backward, per-point pass ([pair x n] orientation): for 16 points per block and the
-// column tiles B of split s (B = s, s + S, ...), visited column-major with A <= B:
+// column tiles B of split s, visited column-major with A <= B:
 // T_na (row sums, plus column sums of the off-diagonal tiles) and V_nq.  Row a is
 // owned by wave (a % 16) % 4, so row sums need no barrier; a column tile's sums are
 // kept in registers over its A tiles and merged once, in wave order.
@@ -308,9 +308,9 @@ __global__ __launch_bounds__(256) void fused_bwd_points_kernel(
   double *Tacc = dyn + M * ZS;
   __shared__ double lam[FQ_MAX];
   __shared__ double cst[4][16][17];  // column-sum staging; reused for the V reduction
-  // (split_block's XCD-aware order measured slower here: its splits hold unequal work,
-  // column tiles B = split, split + S, ..., and whole XCDs then wait on the larger one)
-  const int n0 = blockIdx.x * 16, split = blockIdx.y;
+  int pb, split;
+  if (!split_block((N + 15) / 16, S, pb, split)) return;
+  const int n0 = pb * 16;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int e = tid; e < M * Q; e += 256) zs[(e / Q) * ZS + e % Q] = z[e];
   for (int e = tid; e < 16 * TS; e += 256) Tacc[e] = 0.0;
@@ -346,7 +346,13 @@ __global__ __launch_bounds__(256) void fused_bwd_points_kernel(
   const double sf2 = exp(2.0 * sf[0]), sf4 = sf2 * sf2;
   gpx_d4 vacc = gpx_d4{0.0, 0.0, 0.0, 0.0};
   const int T = (M + 15) / 16;
-  for (int B = split; B < T; B += S) {
+  // Column tiles of this split in a reversed snake order: round r takes the tiles
+  // T-1-rS ... T-S-rS, split s the s-th of them in even rounds and the (S-1-s)-th in odd
+  // ones.  Tile B holds B + 1 row blocks, so the splits get (nearly) equal work, largest
+  // tiles first.
+  for (int r = 0; r * S < T; ++r) {
+    const int B = T - 1 - (r * S + ((r & 1) ? S - 1 - split : split));
+    if (B < 0) continue;  // uniform over the workgroup
     double colp[4] = {0.0, 0.0, 0.0, 0.0};
     const int ja = 16 * B + (lane & 15);
     // loads use the column clamped to M - 1: a padding column's exponent is then that of
@@ -1061,7 +1067,7 @@ int gpx_fused_mm_bwd_points(int N, int M, int Q, int D, const double *mx, const 
   GPX_CHECK_LAUNCH();
   const int ks = (2 * Q + 2 + 3) / 4;  // compile-time slice counts for Q = 4, 5 (3) and 7, 8 (5)
   // per-point pass over tile splits
-  dim3 gq(gpx_cdiv(N, 16), splits);
+  dim3 gq(split_grid(gpx_cdiv(N, 16), splits));
 #define GPX_POINTS1(DSV, KSV, FULL)                                                                             \
   hipLaunchKernelGGL((fused_bwd_points_kernel<DSV, KSV, FULL>), gq, dim3(256), points_lds_bytes(M, Q), st, N, M, Q, \
                      D, mx, vx, z, ls, sf, Bp, dv, (const double *)W.Pab, splits, W.Tpart, W.Vpart)
