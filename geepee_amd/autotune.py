@@ -131,6 +131,10 @@ def fused_candidates(N, M):
            if -(-N // (16 * ns)) * s >= 64]
     bwd_points = [s for s in _pow2_upto(1, min(T, 64)) if -(-N // 16) * s >= 64]
     bwd_pairs = [s for s in _pow2_upto(1, min(-(-N // 16), 64)) if ntiles * 4 * s >= 64]
+    # point-range counts that fill whole rounds of 256 workgroups over the tiles (the
+    # merged adjoint kernel of D > 16 runs one workgroup per CU)
+    bwd_pairs = sorted(set(bwd_pairs) | {k * 256 // ntiles for k in (1, 2, 3, 4)
+                                         if 1 <= k * 256 // ntiles <= min(-(-N // 16), 64)})
     return fwd or [{'nsub': 1, 'splits_f': min(ntiles, 64)}], bwd_points or [min(T, 64)], \
         bwd_pairs or [min(-(-N // 16), 64)]
 

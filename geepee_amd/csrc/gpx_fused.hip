@@ -58,6 +58,16 @@ inline unsigned split_grid(long nblk, int S) {
   return (8 % S == 0) ? 8u * gpx_cdiv(nblk, 8 / S) : (unsigned)(nblk * S);
 }
 
+// For 16 < D <= 32 the adjoint's two passes run as one kernel (fused_bwd_merged_kernel),
+// whose per-point partials have T slots (T) and one slot per tile (V) instead of one per
+// column-tile split.
+inline bool fused_merged(int D) { return D > 16; }
+inline int fused_t_slots(int M, int D, int splits) { return fused_merged(D) ? (M + 15) / 16 : splits; }
+inline int fused_v_slots(int M, int D, int splits) {
+  const int T = (M + 15) / 16;
+  return fused_merged(D) ? T * (T + 1) / 2 : splits;
+}
+
 // Per-point exponent feature k (B operand of the exponent GEMM).
 __device__ __forceinline__ double point_feature(int k, int Q, const double *mu, const double *d, double c) {
   if (k < Q) return 2.0 * d[k] * mu[k];
@@ -771,6 +781,251 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 }
 #undef GPX_PAIRS_ARGS
 
+// Wave-scope ordering of LDS accesses (a wave reading what its other lanes wrote).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------------------
+// backward, both passes in one (16 < D <= 32): the per-pair sums of
+// fused_bwd_pairs_staged and the per-point sums of fused_bwd_points_kernel from one
+// evaluation of p and G per (point, pair).  A workgroup of MG_NW = 8 waves owns the 16
+// rows of one tile (A, B) over one point range; wave w holds rows a = 16A + w + 8u
+// (u < 2).  Per row and 16-point block the exponent, G, p, L and the dB-hat / LW
+// contractions are those of the staged per-pair kernel (same operands and order, so
+// dBpart and LWpart are the same bits).  Added per row:
+//   U[n][q'] = sum_b L[n][b] [1 | z_b][q']  (L transposed through the wave's LDS, 4 MFMAs)
+//   row sums  T[n][a] = U[n][0]                      -> Tpart slot B, column a
+//   V[n][q]  += w z_aq U[n][1+q]  (w = 2 off the diagonal tile, 1 on it)
+// and per workgroup (A < B) the column sums sum_a L[n][a][b] -> Tpart slot A, column b.
+// Column c of tile C thus has exactly T slots (row sums from tiles (C, s), s >= C; column
+// sums from tiles (s, C), s < C), each written once; V has one slot per tile.  The
+// 8-wave sums of the column sums and V go through LDS in wave order (double-buffered
+// by block parity, so one barrier per block serves the staging and both sums).
+// ---------------------------------------------------------------------------
+constexpr int MG_NW = 8;
+template <int KSC>
+__global__ __launch_bounds__(64 * MG_NW) void fused_bwd_merged_kernel(
+    int N, int M, int Q, int D, const double *__restrict__ z, const double *__restrict__ ls,
+    const double *__restrict__ sf, const double *__restrict__ Bp, const double *__restrict__ dv,
+    const double *__restrict__ pf, int nsplit, int nchunk, double *__restrict__ dBpart,
+    double *__restrict__ LWpart, double *__restrict__ Tpart, double *__restrict__ Vpart) {
+  constexpr int DB = 2, DS = 8, NR = 2, NT = 64 * MG_NW;
+  constexpr int PA_W = 4 * FKS_MAX;
+  __shared__ double pas[2][16][PA_W];
+  __shared__ double pqs[2][16][16];
+  __shared__ double dvs[2][16][32];
+  __shared__ double trn[MG_NW][16][17];        // per-wave transpose of L
+  __shared__ double csum[2][MG_NW][16][17];    // per-wave column sums  [n][b]
+  __shared__ double vsum[2][MG_NW][16][17];    // per-wave V terms      [n][q']
+  // pair features of the wave's rows (B operand of the exponent), kept in LDS when the
+  // slice count is fixed at compile time: in registers they push the kernel past 256
+  constexpr bool AF_LDS = KSC != 0;
+  constexpr int KSL = AF_LDS ? KSC : 1;
+  __shared__ double afs[MG_NW][2][KSL][64];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int T = (M + 15) / 16;
+  const int tile = blockIdx.x;
+  int A, B;
+  decode_tile(tile, T, A, B);
+  const bool off = A != B;
+  const int ns = blockIdx.y;
+  const int nbeg = ns * nchunk, nend = min(N, nbeg + nchunk);
+  constexpr int KSN = KSC ? KSC : FKS_MAX;
+  const int KS = KSC ? KSC : (2 * Q + 2 + 3) / 4;
+  const int ka = lane & 15, kq = lane >> 4;
+  const int jb = 16 * B + ka;
+  int a[NR];
+  bool act[NR];
+#pragma unroll
+  for (int u = 0; u < NR; ++u) {
+    a[u] = 16 * A + wave + 8 * u;
+    act[u] = a[u] < M;
+  }
+  double af[AF_LDS ? 1 : NR][AF_LDS ? 1 : KSN], bs[NR][DS], zw[NR];
+  {
+    double il4[FQ_MAX], zb[FQ_MAX];
+#pragma unroll
+    for (int q = 0; q < FQ_MAX; ++q) {
+      il4[q] = (q < Q) ? 0.25 * exp(-2.0 * ls[q]) : 0.0;
+      zb[q] = (q < Q && jb < M) ? z[jb * Q + q] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < NR; ++u) {
+      const bool bv = jb < M && act[u];
+      double za[FQ_MAX];
+#pragma unroll
+      for (int q = 0; q < FQ_MAX; ++q) za[q] = (q < Q && act[u]) ? z[a[u] * Q + q] : 0.0;
+#pragma unroll
+      for (int s = 0; s < KSN; ++s) {
+        const double f = (s < KS && bv) ? pair_feature((lane >> 4) + 4 * s, Q, za, zb, il4) : 0.0;
+        if constexpr (AF_LDS) afs[wave][u][s][lane] = f; else af[u][s] = f;
+      }
+#pragma unroll
+      for (int s = 0; s < DS; ++s) {
+        const int d = kq + 4 * s;
+        bs[u][s] = (bv && d < D) ? Bp[((long)a[u] * M + jb) * D + d] : 0.0;
+      }
+      // V weight of this lane's column q' = ka of U: w z_a[q'-1]
+      zw[u] = (act[u] && ka >= 1 && ka <= Q) ? (off ? 2.0 : 1.0) * z[a[u] * Q + ka - 1] : 0.0;
+    }
+  }
+  // B operand of U: [1 | z_b] for b = 16B + kq + 4s (rows), q' = ka (columns); padding
+  // columns b >= M give zeros (their L is 0 as well)
+  double zx[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int b = 16 * B + kq + 4 * s;
+    zx[s] = (b < M && ka <= Q) ? (ka == 0 ? 1.0 : z[b * Q + ka - 1]) : 0.0;
+  }
+  const double sf2 = exp(2.0 * sf[0]), sf4 = sf2 * sf2;
+  gpx_d4 dacc[NR][DB], lacc[NR];
+#pragma unroll
+  for (int u = 0; u < NR; ++u) {
+    lacc[u] = gpx_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int db = 0; db < DB; ++db) dacc[u][db] = gpx_d4{0.0, 0.0, 0.0, 0.0};
+  }
+  constexpr int NPF = (16 * PF_W + NT - 1) / NT;
+  double rp[NPF], rd;
+  const int sk = tid >> 5, sc = tid & 31;  // dv staging: row sk (< 16), column sc (< 32)
+  auto fetch = [&](int n0) {
+#pragma unroll
+    for (int i = 0; i < NPF; ++i) {
+      const int e = tid + NT * i;
+      rp[i] = (e < 16 * PF_W && n0 + e / PF_W < nend) ? pf[(long)n0 * PF_W + e] : 0.0;
+    }
+    rd = (n0 + sk < nend && sc < D) ? dv[(long)(n0 + sk) * D + sc] : 0.0;
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < NPF; ++i) {
+      const int e = tid + NT * i;
+      if (e < 16 * PF_W) {
+        const int k = e / PF_W, c = e % PF_W;
+        if (c < PA_W) pas[buf][k][c] = rp[i]; else pqs[buf][k][c - PA_W] = rp[i];
+      }
+    }
+    dvs[buf][sk][sc ^ dv_swizzle(sk)] = rd;
+  };
+  // the 8-wave sums of block n0's column sums and V terms (buffer cb), written out
+  auto reduce = [&](int n0, int cb) {
+    if (tid < 256) {
+      const int nn = tid >> 4, bb = tid & 15, n = n0 + nn, b = 16 * B + bb;
+      if (off && n < nend && b < M) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < MG_NW; ++w) t += csum[cb][w][nn][bb];
+        Tpart[((long)A * N + n) * M + b] = t;
+      }
+    } else {
+      const int e = tid - 256, nn = e >> 4, qq = e & 15, n = n0 + nn;
+      if (qq >= 1 && qq <= Q && n < nend) {
+        double v = 0.0;
+#pragma unroll
+        for (int w = 0; w < MG_NW; ++w) v += vsum[cb][w][nn][qq];
+        Vpart[((long)tile * N + n) * Q + qq - 1] = v;
+      }
+    }
+  };
+  fetch(nbeg);
+  stage(0);
+  __syncthreads();
+  const int swa = dv_swizzle(ka);
+  int buf = 0;
+  for (int n0 = nbeg; n0 < nend; n0 += 16) {
+    const bool more = n0 + 16 < nend;  // uniform over the workgroup
+    if (more) fetch(n0 + 16);
+    double pa[KSN], da[DS], dvv[4][DB], dq[4];
+#pragma unroll
+    for (int s = 0; s < KSN; ++s) pa[s] = (s < KS) ? pas[buf][ka][kq + 4 * s] : 0.0;
+#pragma unroll
+    for (int s = 0; s < DS; ++s) da[s] = dvs[buf][ka][(kq + 4 * s) ^ swa];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int k = kq + 4 * r;
+#pragma unroll
+      for (int db = 0; db < DB; ++db) dvv[r][db] = dvs[buf][k][(16 * db + ka) ^ dv_swizzle(k)];
+      dq[r] = pqs[buf][k][ka];
+    }
+    double cs[4] = {0.0, 0.0, 0.0, 0.0}, vw[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int u = 0; u < NR; ++u) {
+      if (!act[u]) continue;  // wave-uniform
+      gpx_d4 e = gpx_d4{0.0, 0.0, 0.0, 0.0}, g = gpx_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < KSN; ++s)
+        if (s < KS) {
+          double f;
+          if constexpr (AF_LDS) f = afs[wave][u][s][lane]; else f = af[u][s];
+          e = gpx_mfma16(pa[s], f, e);
+        }
+#pragma unroll
+      for (int s = 0; s < DS; ++s) g = gpx_mfma16(da[s], bs[u][s], g);
+      double p[4], L[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        p[r] = sf4 * exp(e[r]) * ((n0 + kq + 4 * r < nend && jb < M) ? 1.0 : 0.0);
+        L[r] = g[r] * p[r];
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+#pragma unroll
+        for (int db = 0; db < DB; ++db) dacc[u][db] = gpx_mfma16(dvv[r][db], p[r], dacc[u][db]);
+        lacc[u] = gpx_mfma16(dq[r], L[r], lacc[u]);
+      }
+      // U = L [1 | z_b]: L (lane: n = kq + 4r, b = ka) transposed to the A-operand
+      // layout (lane: n = ka, b = kq + 4s) through the wave's LDS
+#pragma unroll
+      for (int r = 0; r < 4; ++r) trn[wave][kq + 4 * r][ka] = L[r];
+      wave_lds_sync();
+      double lt[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) lt[s] = trn[wave][ka][kq + 4 * s];
+      wave_lds_sync();
+      gpx_d4 U = gpx_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < 4; ++s) U = gpx_mfma16(lt[s], zx[s], U);
+      // lane holds U[n = kq + 4r][q' = ka]
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = n0 + kq + 4 * r;
+        if (ka == 0 && n < nend) Tpart[((long)B * N + n) * M + a[u]] = U[r];
+        vw[r] += zw[u] * U[r];
+        cs[r] += L[r];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      csum[buf][wave][kq + 4 * r][ka] = cs[r];
+      vsum[buf][wave][kq + 4 * r][ka] = vw[r];
+    }
+    if (more) stage(buf ^ 1);
+    __syncthreads();
+    reduce(n0, buf);
+    buf ^= 1;
+  }
+#pragma unroll
+  for (int u = 0; u < NR; ++u) {
+    if (!act[u]) continue;
+#pragma unroll
+    for (int db = 0; db < DB; ++db)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int d = 16 * db + kq + 4 * r;
+        if (d < D && jb < M) dBpart[(((long)ns * D + d) * M + a[u]) * M + jb] = dacc[u][db][r];
+      }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int qq = kq + 4 * r;
+      if (qq <= Q && jb < M) LWpart[(((long)ns * M + a[u]) * M + jb) * (Q + 1) + qq] = lacc[u][r];
+    }
+  }
+}
+
 // Sum of the per-pair pass's point-range partials.  One workgroup per stored 16 x 16
 // tile pair (A <= B, the half the pair kernel writes) and output slice (k < D: dBhat[k],
 // k == D: LW); rows are read coalesced and the mirrored tile (B, A) is written through an
@@ -819,14 +1074,15 @@ __global__ __launch_bounds__(256) void fused_pairs_finish(int M, int Q, int D, i
 }
 
 
-// Per-point finish of the fused adjoint: T = sum_s Tpart, V = sum_s Vpart, Lp = dpsi1 psi1,
+// Per-point finish of the fused adjoint: T = sum of the S slots of Tpart, V = sum of the
+// SV slots of Vpart, Lp = dpsi1 psi1,
 // then the same per-point formulas (and outputs) as the materialising path's row pass.
 // One wave per point: its lanes stride over the M columns and the sums are wave
 // reductions, so no workgroup barrier is needed.
 // QC: Q fixed at compile time for the common shapes (0: read at run time).
 template <int QC>
 __global__ __launch_bounds__(256) void fused_rows_finish(
-    int N, int M, int Q, int S, const double *__restrict__ mx, const double *__restrict__ vx,
+    int N, int M, int Q, int S, int SV, const double *__restrict__ mx, const double *__restrict__ vx,
     const double *__restrict__ z, const double *__restrict__ ls, const double *__restrict__ Tpart,
     const double *__restrict__ Vpart, const double *__restrict__ dpsi1, const double *__restrict__ psi1,
     double *__restrict__ dmx, double *__restrict__ dvx, double *__restrict__ Tout,
@@ -845,7 +1101,8 @@ __global__ __launch_bounds__(256) void fused_rows_finish(
   for (int i = 0; i < FRED; ++i) c[i] = 0.0;
   for (int m = lane; m < M; m += 64) {
     double T = 0.0;
-    for (int s = 0; s < S; ++s) T += Tpart[((long)s * N + n) * M + m];
+#pragma unroll 8
+    for (int s = 0; s < S; ++s) T += Tpart[((long)s * N + n) * M + m];  // loads issue ahead of the adds
     const double Lp = dpsi1[(long)n * M + m] * psi1[(long)n * M + m];
     Tout[(long)n * M + m] = T;
     Lpout[(long)n * M + m] = Lp;
@@ -878,7 +1135,8 @@ __global__ __launch_bounds__(256) void fused_rows_finish(
   if (lane < Q) {
     const int q = lane;
     double V = 0.0;
-    for (int s = 0; s < S; ++s) V += Vpart[((long)s * N + n) * Q + q];
+#pragma unroll 8
+    for (int s = 0; s < SV; ++s) V += Vpart[((long)s * N + n) * Q + q];
     const double S0 = c[0], A1 = c[1];
     const double S1 = out_S1, TZ2 = out_TZ2, B1 = out_B1, C1 = out_C1;
     const double l = exp(2.0 * ls[q]);
@@ -993,7 +1251,7 @@ extern "C" {
 long gpx_fused_bwd_workspace(int N, int M, int Q, int D, int splits, int nsplit) {
   const long MM = (long)M * M;
   long s = (long)nsplit * D * MM + (long)nsplit * MM * (Q + 1) + MM * (Q + 1);  // dBpart, LWpart, LW
-  s += (long)splits * N * M + (long)splits * N * Q;                           // Tpart, Vpart
+  s += (long)fused_t_slots(M, D, splits) * N * M + (long)fused_v_slots(M, D, splits) * N * Q;  // Tpart, Vpart
   s += 3L * N * M + (long)N * Q + 2L * N;                                     // dpsi1, T, Lp, H, s0
   s += (long)N * (Q + 1) + 4L * N * Q + 4L * M * Q + 2;                       // Dn, Pn, En, TP, LE, cols0
   s += (long)N * PF_W;                                                        // point features
@@ -1016,8 +1274,8 @@ struct FusedBwdWork {
     LWpart = dBpart + (long)nsplit * D * MM;
     LW = LWpart + (long)nsplit * MM * (Q + 1);
     Tpart = LW + MM * (Q + 1);
-    Vpart = Tpart + (long)splits * N * M;
-    dpsi1 = Vpart + (long)splits * N * Q;
+    Vpart = Tpart + (long)fused_t_slots(M, D, splits) * N * M;
+    dpsi1 = Vpart + (long)fused_v_slots(M, D, splits) * N * Q;
     Tn = dpsi1 + (long)N * M;
     Lp = Tn + (long)N * M;
     H = Lp + (long)N * M;
@@ -1063,9 +1321,22 @@ int gpx_fused_mm_bwd_points(int N, int M, int Q, int D, const double *mx, const 
   FusedBwdWork W(work, N, M, Q, D, splits, nsplit);
   hipLaunchKernelGGL(fused_point_feat_kernel, dim3(gpx_cdiv(N, 256)), dim3(256), 0, st, N, Q, mx, vx, ls, W.pf);
   GPX_CHECK_LAUNCH();
+  const int ks = (2 * Q + 2 + 3) / 4;  // compile-time slice counts for Q = 4, 5 (3) and 7, 8 (5)
+  if (fused_merged(D)) {
+    // both passes in one kernel: the per-pair partials (dBpart, LWpart) are written here
+    // too, and gpx_fused_mm_bwd_pairs only sums them
+    const int T = (M + 15) / 16, ntiles = T * (T + 1) / 2;
+    const int nchunk = ((N + nsplit - 1) / nsplit + 15) / 16 * 16;
+    dim3 gm(ntiles, nsplit);
+#define GPX_MERGED(KSV)                                                                                        \
+  hipLaunchKernelGGL((fused_bwd_merged_kernel<KSV>), gm, dim3(64 * MG_NW), 0, st, N, M, Q, D, z, ls, sf, Bp, dv, \
+                     (const double *)W.pf, nsplit, nchunk, W.dBpart, W.LWpart, W.Tpart, W.Vpart)
+    if (ks == 3) GPX_MERGED(3); else if (ks == 5) GPX_MERGED(5); else GPX_MERGED(0);
+#undef GPX_MERGED
+    GPX_CHECK_LAUNCH();
+  } else {
   hipLaunchKernelGGL(pair_p_kernel, dim3(gpx_cdiv((long)M * M, 256)), dim3(256), 0, st, M, Q, z, ls, W.Pab);
   GPX_CHECK_LAUNCH();
-  const int ks = (2 * Q + 2 + 3) / 4;  // compile-time slice counts for Q = 4, 5 (3) and 7, 8 (5)
   // per-point pass over tile splits
   dim3 gq(split_grid(gpx_cdiv(N, 16), splits));
 #define GPX_POINTS1(DSV, KSV, FULL)                                                                             \
@@ -1086,12 +1357,14 @@ int gpx_fused_mm_bwd_points(int N, int M, int Q, int D, const double *mx, const 
 #undef GPX_POINTS
 #undef GPX_POINTS1
   GPX_CHECK_LAUNCH();
+  }
   // dpsi1 = dm_all Ahat
   int rc = gpx_gemm_batched(0, 0, N, M, D, 1.0, dm_all, D, 0, Ahat, M, 0, 0.0, W.dpsi1, M, 0, 1, 1, nullptr, 0,
                             stream);
   if (rc) return rc;
 #define GPX_ROWS(QV)                                                                                           \
-  hipLaunchKernelGGL((fused_rows_finish<QV>), dim3(gpx_cdiv(N, 4)), dim3(256), 0, st, N, M, Q, splits, mx, vx, z, ls, \
+  hipLaunchKernelGGL((fused_rows_finish<QV>), dim3(gpx_cdiv(N, 4)), dim3(256), 0, st, N, M, Q,                 \
+                     fused_t_slots(M, D, splits), fused_v_slots(M, D, splits), mx, vx, z, ls,                    \
                      (const double *)W.Tpart, (const double *)W.Vpart, (const double *)W.dpsi1, psi1, dmx, dvx,   \
                      W.Tn, W.Lp, W.H, W.s0, W.Dn, W.Pn, W.En)
   if (Q == 4) GPX_ROWS(4); else if (Q == 5) GPX_ROWS(5); else if (Q == 8) GPX_ROWS(8); else GPX_ROWS(0);
@@ -1108,6 +1381,12 @@ int gpx_fused_mm_bwd_pairs(int N, int M, int Q, int D, const double *z, const do
   hipStream_t st = (hipStream_t)stream;
   const int T = (M + 15) / 16, ntiles = T * (T + 1) / 2;
   FusedBwdWork W(work, N, M, Q, D, splits, nsplit);
+  if (fused_merged(D)) {  // the partials came from the merged kernel of the per-point part
+    hipLaunchKernelGGL(fused_pairs_finish, dim3(ntiles, D + 1), dim3(256), 0, st, M, Q, D, nsplit,
+                       (const double *)W.dBpart, (const double *)W.LWpart, dBhat, W.LW);
+    GPX_CHECK_LAUNCH();
+    return GPX_OK;
+  }
   // per-pair pass over point ranges
   const int nchunk = ((N + nsplit - 1) / nsplit + 15) / 16 * 16;
   constexpr int NR = 1;  // rows per wave (one: three waves per SIMD at D = 32; two: one)
