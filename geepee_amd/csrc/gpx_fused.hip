@@ -58,14 +58,15 @@ inline unsigned split_grid(long nblk, int S) {
   return (8 % S == 0) ? 8u * gpx_cdiv(nblk, 8 / S) : (unsigned)(nblk * S);
 }
 
-// For 16 < D <= 32 the adjoint's two passes run as one kernel (fused_bwd_merged_kernel),
-// whose per-point partials have T slots (T) and one slot per tile (V) instead of one per
-// column-tile split.
-inline bool fused_merged(int D) { return D > 16; }
-inline int fused_t_slots(int M, int D, int splits) { return fused_merged(D) ? (M + 15) / 16 : splits; }
-inline int fused_v_slots(int M, int D, int splits) {
+// For D > 16, and for long point ranges (N >= 4096), the adjoint's two passes run as one
+// kernel (fused_bwd_merged_kernel), whose per-point partials have T slots (T) and one slot
+// per tile (V) instead of one per column-tile split.  Below that the two-kernel form is
+// kept: the deep GP's layers fork its per-pair half onto a second stream.
+inline bool fused_merged(int N, int D) { return D > 16 || N >= 4096; }
+inline int fused_t_slots(int N, int M, int D, int splits) { return fused_merged(N, D) ? (M + 15) / 16 : splits; }
+inline int fused_v_slots(int N, int M, int D, int splits) {
   const int T = (M + 15) / 16;
-  return fused_merged(D) ? T * (T + 1) / 2 : splits;
+  return fused_merged(N, D) ? T * (T + 1) / 2 : splits;
 }
 
 // Per-point exponent feature k (B operand of the exponent GEMM).
@@ -789,13 +790,14 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // ---------------------------------------------------------------------------
-// backward, both passes in one (16 < D <= 32): the per-pair sums of
-// fused_bwd_pairs_staged and the per-point sums of fused_bwd_points_kernel from one
+// backward, both passes in one: the per-pair sums of fused_bwd_pairs_staged /
+// bwd_pairs_body and the per-point sums of fused_bwd_points_kernel from one
 // evaluation of p and G per (point, pair).  A workgroup of MG_NW = 8 waves owns the 16
 // rows of one tile (A, B) over one point range; wave w holds rows a = 16A + w + 8u
 // (u < 2).  Per row and 16-point block the exponent, G, p, L and the dB-hat / LW
-// contractions are those of the staged per-pair kernel (same operands and order, so
-// dBpart and LWpart are the same bits).  Added per row:
+// contractions are those of the per-pair kernels (same operands and order).  DB, DS: the
+// 16-row and 4-row slices of the D contraction (2, 8 for D <= 32; 1, DS for D <= 16).
+// Added per row:
 //   U[n][q'] = sum_b L[n][b] [1 | z_b][q']  (L transposed through the wave's LDS, 4 MFMAs)
 //   row sums  T[n][a] = U[n][0]                      -> Tpart slot B, column a
 //   V[n][q]  += w z_aq U[n][1+q]  (w = 2 off the diagonal tile, 1 on it)
@@ -806,13 +808,13 @@ __device__ __forceinline__ void wave_lds_sync() {
 // by block parity, so one barrier per block serves the staging and both sums).
 // ---------------------------------------------------------------------------
 constexpr int MG_NW = 8;
-template <int KSC>
+template <int DB, int DS, int KSC>
 __global__ __launch_bounds__(64 * MG_NW) void fused_bwd_merged_kernel(
     int N, int M, int Q, int D, const double *__restrict__ z, const double *__restrict__ ls,
     const double *__restrict__ sf, const double *__restrict__ Bp, const double *__restrict__ dv,
     const double *__restrict__ pf, int nsplit, int nchunk, double *__restrict__ dBpart,
     double *__restrict__ LWpart, double *__restrict__ Tpart, double *__restrict__ Vpart) {
-  constexpr int DB = 2, DS = 8, NR = 2, NT = 64 * MG_NW;
+  constexpr int NR = 2, NT = 64 * MG_NW;
   constexpr int PA_W = 4 * FKS_MAX;
   __shared__ double pas[2][16][PA_W];
   __shared__ double pqs[2][16][16];
@@ -1251,7 +1253,7 @@ extern "C" {
 long gpx_fused_bwd_workspace(int N, int M, int Q, int D, int splits, int nsplit) {
   const long MM = (long)M * M;
   long s = (long)nsplit * D * MM + (long)nsplit * MM * (Q + 1) + MM * (Q + 1);  // dBpart, LWpart, LW
-  s += (long)fused_t_slots(M, D, splits) * N * M + (long)fused_v_slots(M, D, splits) * N * Q;  // Tpart, Vpart
+  s += (long)fused_t_slots(N, M, D, splits) * N * M + (long)fused_v_slots(N, M, D, splits) * N * Q;  // Tpart, Vpart
   s += 3L * N * M + (long)N * Q + 2L * N;                                     // dpsi1, T, Lp, H, s0
   s += (long)N * (Q + 1) + 4L * N * Q + 4L * M * Q + 2;                       // Dn, Pn, En, TP, LE, cols0
   s += (long)N * PF_W;                                                        // point features
@@ -1274,8 +1276,8 @@ struct FusedBwdWork {
     LWpart = dBpart + (long)nsplit * D * MM;
     LW = LWpart + (long)nsplit * MM * (Q + 1);
     Tpart = LW + MM * (Q + 1);
-    Vpart = Tpart + (long)fused_t_slots(M, D, splits) * N * M;
-    dpsi1 = Vpart + (long)fused_v_slots(M, D, splits) * N * Q;
+    Vpart = Tpart + (long)fused_t_slots(N, M, D, splits) * N * M;
+    dpsi1 = Vpart + (long)fused_v_slots(N, M, D, splits) * N * Q;
     Tn = dpsi1 + (long)N * M;
     Lp = Tn + (long)N * M;
     H = Lp + (long)N * M;
@@ -1322,16 +1324,22 @@ int gpx_fused_mm_bwd_points(int N, int M, int Q, int D, const double *mx, const 
   hipLaunchKernelGGL(fused_point_feat_kernel, dim3(gpx_cdiv(N, 256)), dim3(256), 0, st, N, Q, mx, vx, ls, W.pf);
   GPX_CHECK_LAUNCH();
   const int ks = (2 * Q + 2 + 3) / 4;  // compile-time slice counts for Q = 4, 5 (3) and 7, 8 (5)
-  if (fused_merged(D)) {
+  if (fused_merged(N, D)) {
     // both passes in one kernel: the per-pair partials (dBpart, LWpart) are written here
     // too, and gpx_fused_mm_bwd_pairs only sums them
     const int T = (M + 15) / 16, ntiles = T * (T + 1) / 2;
     const int nchunk = ((N + nsplit - 1) / nsplit + 15) / 16 * 16;
     dim3 gm(ntiles, nsplit);
-#define GPX_MERGED(KSV)                                                                                        \
-  hipLaunchKernelGGL((fused_bwd_merged_kernel<KSV>), gm, dim3(64 * MG_NW), 0, st, N, M, Q, D, z, ls, sf, Bp, dv, \
-                     (const double *)W.pf, nsplit, nchunk, W.dBpart, W.LWpart, W.Tpart, W.Vpart)
-    if (ks == 3) GPX_MERGED(3); else if (ks == 5) GPX_MERGED(5); else GPX_MERGED(0);
+#define GPX_MERGED(DBV, DSV, KSV)                                                                             \
+  hipLaunchKernelGGL((fused_bwd_merged_kernel<DBV, DSV, KSV>), gm, dim3(64 * MG_NW), 0, st, N, M, Q, D, z, ls, sf,  \
+                     Bp, dv, (const double *)W.pf, nsplit, nchunk, W.dBpart, W.LWpart, W.Tpart, W.Vpart)
+#define GPX_MERGED_KS(DBV, DSV) \
+  if (ks == 3) GPX_MERGED(DBV, DSV, 3); else if (ks == 5) GPX_MERGED(DBV, DSV, 5); else GPX_MERGED(DBV, DSV, 0);
+    if (D <= 4) { GPX_MERGED_KS(1, 1) }
+    else if (D <= 8) { GPX_MERGED_KS(1, 2) }
+    else if (D <= 16) { GPX_MERGED_KS(1, 4) }
+    else { GPX_MERGED_KS(2, 8) }
+#undef GPX_MERGED_KS
 #undef GPX_MERGED
     GPX_CHECK_LAUNCH();
   } else {
@@ -1364,7 +1372,7 @@ int gpx_fused_mm_bwd_points(int N, int M, int Q, int D, const double *mx, const 
   if (rc) return rc;
 #define GPX_ROWS(QV)                                                                                           \
   hipLaunchKernelGGL((fused_rows_finish<QV>), dim3(gpx_cdiv(N, 4)), dim3(256), 0, st, N, M, Q,                 \
-                     fused_t_slots(M, D, splits), fused_v_slots(M, D, splits), mx, vx, z, ls,                    \
+                     fused_t_slots(N, M, D, splits), fused_v_slots(N, M, D, splits), mx, vx, z, ls,              \
                      (const double *)W.Tpart, (const double *)W.Vpart, (const double *)W.dpsi1, psi1, dmx, dvx,   \
                      W.Tn, W.Lp, W.H, W.s0, W.Dn, W.Pn, W.En)
   if (Q == 4) GPX_ROWS(4); else if (Q == 5) GPX_ROWS(5); else if (Q == 8) GPX_ROWS(8); else GPX_ROWS(0);
@@ -1381,7 +1389,7 @@ int gpx_fused_mm_bwd_pairs(int N, int M, int Q, int D, const double *z, const do
   hipStream_t st = (hipStream_t)stream;
   const int T = (M + 15) / 16, ntiles = T * (T + 1) / 2;
   FusedBwdWork W(work, N, M, Q, D, splits, nsplit);
-  if (fused_merged(D)) {  // the partials came from the merged kernel of the per-point part
+  if (fused_merged(N, D)) {  // the partials came from the merged kernel of the per-point part
     hipLaunchKernelGGL(fused_pairs_finish, dim3(ntiles, D + 1), dim3(256), 0, st, M, Q, D, nsplit,
                        (const double *)W.dBpart, (const double *)W.LWpart, dBhat, W.LW);
     GPX_CHECK_LAUNCH();

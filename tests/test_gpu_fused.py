@@ -40,9 +40,11 @@ def _spread(M, Q):
                                      # D = 32 (the largest LDS request), M in (200, 256]
                                      (70, 256, 15, 32), (45, 210, 9, 3),
                                      # point ranges of >= 256 points per split (the per-pair
-                                     # pass's four-waves-per-SIMD build) and, at D > 16, 8 staged
+                                     # pass's four-waves-per-SIMD build) and, at D > 16, 8 point
                                      # blocks a range with a ragged last one
-                                     (1000, 200, 4, 4), (999, 120, 2, 20)])
+                                     (1000, 200, 4, 4), (999, 120, 2, 20),
+                                     # N >= 4096 with D <= 16: the merged adjoint kernel
+                                     (4100, 50, 3, 6)])
 def test_fused_layer_matches_plain(N, M, Q, D):
     from geepee_amd import dops
     from geepee_amd.aep_models import DeviceParams, _shapes
