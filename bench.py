@@ -38,7 +38,7 @@ FP64_PEAK_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
 # WRITE_SIZE 524,288 KiB, so FETCH_SIZE is scaled by 2.0 and WRITE_SIZE by 1.0.
 TRAFFIC = {
     ('sgplvm', 'gpx_psi_fwd'): 2.0 * 2185 * 1024 + 1.0 * 528384 * 1024,
-    ('sgplvm', 'gpx_fused_mm_bwd'): 2.0 * 66138 * 1024 + 1.0 * 78483 * 1024,
+    ('sgplvm', 'gpx_fused_mm_bwd'): 2.0 * 59085 * 1024 + 1.0 * 85058 * 1024,
 }
 
 WORKLOADS = {
@@ -174,11 +174,12 @@ def roofline_model(entry, name, w):
             pairs = M * (M + 1) / 2.0
             tot += N * pairs * (2.0 * (2 * Q + 2) + 1.0 + 2.0 * D)
         elif entry == 'gpx_fused_mm_bwd':
-            # both passes recompute the exponents; per-point pass contracts Bhat with dv (2D)
-            # and accumulates T and V (2 + 2Q); per-pair pass forms G, dBhat and LW (4D + 2(Q+1))
+            # per point and unordered pair, each product formed once: the exponent (2K) and
+            # its exp (1), G = <dv, Bhat> (2D), the dBhat and LW contractions (2D + 2(Q+1)),
+            # and the row sums and V terms of the per-point statistics (2 + 2Q)
             pairs = M * (M + 1) / 2.0
             K = 2 * Q + 2
-            tot += N * pairs * (2 * (2.0 * K + 1.0) + 2.0 * D + 2.0 + 2.0 * Q + 4.0 * D + 2.0 * (Q + 1))
+            tot += N * pairs * (2.0 * K + 1.0 + 4.0 * D + 2.0 * (Q + 1) + 2.0 + 2.0 * Q)
     bound = 'hbm' if entry == 'gpx_psi_fwd' else 'mfma'
     return bound, (tot / launches if launches else 0.0)
 
