@@ -1030,9 +1030,9 @@ __global__ __launch_bounds__(64 * MG_NW) void fused_bwd_merged_kernel(
 
 // Sum of the per-pair pass's point-range partials.  One workgroup per stored 16 x 16
 // tile pair (A <= B, the half the pair kernel writes) and output slice (k < D: dBhat[k],
-// k == D: LW); rows are read coalesced and the mirrored tile (B, A) is written through an
-// LDS transpose.  dBhat (D x M x M) and LW ((M*M) x (Q+1)) come out in the layouts of
-// the materialising path.
+// k >= D: column q = k - D of LW), 256 entries each; rows are read coalesced and the
+// mirrored tile (B, A) is written through an LDS transpose.  dBhat (D x M x M) and LW
+// ((M*M) x (Q+1)) come out in the layouts of the materialising path.
 __global__ __launch_bounds__(256) void fused_pairs_finish(int M, int Q, int D, int nsplit,
                                                           const double *__restrict__ dBpart,
                                                           const double *__restrict__ LWpart,
@@ -1046,8 +1046,8 @@ __global__ __launch_bounds__(256) void fused_pairs_finish(int M, int Q, int D, i
   }
   const int B = A + t, k = blockIdx.y, tid = threadIdx.x, i = tid >> 4, j = tid & 15;
   const long MM = (long)M * M;
+  const int a = A * 16 + i, b = B * 16 + j;
   if (k < D) {
-    const int a = A * 16 + i, b = B * 16 + j;
     double s = 0.0;
     if (a < M && b < M) {
 #pragma unroll 8
@@ -1061,11 +1061,8 @@ __global__ __launch_bounds__(256) void fused_pairs_finish(int M, int Q, int D, i
       if (a2 < M && b2 < M) dBhat[k * MM + (long)a2 * M + b2] = tile[j][i];
     }
   } else {
-    const int Q1 = Q + 1;
-    for (int e = tid; e < 256 * Q1; e += 256) {
-      const int ii = e / (16 * Q1), r = e % (16 * Q1), jj = r / Q1, q = r % Q1;
-      const int a = A * 16 + ii, b = B * 16 + jj;
-      if (a >= M || b >= M) continue;
+    const int Q1 = Q + 1, q = k - D;
+    if (a < M && b < M) {
       double s = 0.0;
 #pragma unroll 8
       for (int p = 0; p < nsplit; ++p) s += LWpart[(((long)p * M + a) * M + b) * Q1 + q];
@@ -1390,7 +1387,7 @@ int gpx_fused_mm_bwd_pairs(int N, int M, int Q, int D, const double *z, const do
   const int T = (M + 15) / 16, ntiles = T * (T + 1) / 2;
   FusedBwdWork W(work, N, M, Q, D, splits, nsplit);
   if (fused_merged(N, D)) {  // the partials came from the merged kernel of the per-point part
-    hipLaunchKernelGGL(fused_pairs_finish, dim3(ntiles, D + 1), dim3(256), 0, st, M, Q, D, nsplit,
+    hipLaunchKernelGGL(fused_pairs_finish, dim3(ntiles, D + Q + 1), dim3(256), 0, st, M, Q, D, nsplit,
                        (const double *)W.dBpart, (const double *)W.LWpart, dBhat, W.LW);
     GPX_CHECK_LAUNCH();
     return GPX_OK;
@@ -1426,7 +1423,7 @@ int gpx_fused_mm_bwd_pairs(int N, int M, int Q, int D, const double *z, const do
 #undef GPX_PAIRS
 #undef GPX_PAIRS_STAGED
   GPX_CHECK_LAUNCH();
-  hipLaunchKernelGGL(fused_pairs_finish, dim3(ntiles, D + 1), dim3(256), 0, st, M, Q, D, nsplit,
+  hipLaunchKernelGGL(fused_pairs_finish, dim3(ntiles, D + Q + 1), dim3(256), 0, st, M, Q, D, nsplit,
                      (const double *)W.dBpart, (const double *)W.LWpart, dBhat, W.LW);
   GPX_CHECK_LAUNCH();
   return GPX_OK;

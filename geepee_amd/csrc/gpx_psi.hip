@@ -21,6 +21,8 @@
 // MFMA GEMMs on the host side of this file's entry points).
 #include "gpx_common.h"
 
+#include <cstdint>
+
 #define GPX_QMAX 16
 
 namespace {
@@ -413,7 +415,7 @@ static int tail_splits(int N) {
   return s < 1 ? 1 : (s > 64 ? 64 : s);
 }
 
-long gpx_psi_tail_workspace(int N, int M, int Q) { return (long)tail_splits(N) * M * 2 * Q; }
+long gpx_psi_tail_workspace(int N, int M, int Q) { return 2L * tail_splits(N) * M * 2 * Q; }
 
 int gpx_psi_bwd_tail(int N, int M, int Q, const double *z, const double *ls, const double *sf,
                      const double *T, const double *Lp, const double *H, const double *s0,
@@ -423,11 +425,13 @@ int gpx_psi_bwd_tail(int N, int M, int Q, const double *z, const double *ls, con
   if (N < 1 || M < 1 || M > 256 || Q < 1 || Q > GPX_QMAX || !kwork) return GPX_EINVAL;
   hipStream_t st = (hipStream_t)stream;
   const int ks = tail_splits(N);
-  int rc = gpx_gemm_batched(1, 0, M, 2 * Q, N, 1.0, T, M, 0, Pn, 2 * Q, 0, 0.0, TP, 2 * Q, 0, 1, ks,
-                            kwork, 0, stream);
-  if (rc) return rc;
-  rc = gpx_gemm_batched(1, 0, M, 2 * Q, N, 1.0, Lp, M, 0, En, 2 * Q, 0, 0.0, LE, 2 * Q, 0, 1, ks,
-                        kwork, 0, stream);
+  // TP = T^T Pn and LE = Lp^T En as one batch of two split-K products: member 1's operands
+  // are addressed from member 0's at the element distances between the buffers
+  const long sA = ((intptr_t)Lp - (intptr_t)T) / (long)sizeof(double);
+  const long sB = ((intptr_t)En - (intptr_t)Pn) / (long)sizeof(double);
+  const long sC = ((intptr_t)LE - (intptr_t)TP) / (long)sizeof(double);
+  int rc = gpx_gemm_batched(1, 0, M, 2 * Q, N, 1.0, T, M, sA, Pn, 2 * Q, sB, 0.0, TP, 2 * Q, sC, 2, ks, kwork, 0,
+                            stream);
   if (rc) return rc;
   hipLaunchKernelGGL(colsum_kernel, dim3(Q), dim3(256), 0, st, N, Q, H, dell);
   GPX_CHECK_LAUNCH();
