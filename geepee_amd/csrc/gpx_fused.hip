@@ -58,11 +58,10 @@ inline unsigned split_grid(long nblk, int S) {
   return (8 % S == 0) ? 8u * gpx_cdiv(nblk, 8 / S) : (unsigned)(nblk * S);
 }
 
-// For D > 16, and for long point ranges (N >= 4096), the adjoint's two passes run as one
-// kernel (fused_bwd_merged_kernel), whose per-point partials have T slots (T) and one slot
-// per tile (V) instead of one per column-tile split.  Below that the two-kernel form is
-// kept: the deep GP's layers fork its per-pair half onto a second stream.
-inline bool fused_merged(int N, int D) { return D > 16 || N >= 4096; }
+// For D > 16, and for N >= 1000, the adjoint's two passes run as one kernel
+// (fused_bwd_merged_kernel), whose per-point partials have T slots (T) and one slot per
+// tile (V) instead of one per column-tile split.  Below that the two-kernel form is kept.
+inline bool fused_merged(int N, int D) { return D > 16 || N >= 1000; }
 inline int fused_t_slots(int N, int M, int D, int splits) { return fused_merged(N, D) ? (M + 15) / 16 : splits; }
 inline int fused_v_slots(int N, int M, int D, int splits) {
   const int T = (M + 15) / 16;

@@ -357,7 +357,7 @@ def fused_plan(N, M, Q=None, D=None, tuned=True):
     pt_blocks = -(-N // 16)
     splits_b = max(1, min(T, -(-1024 // pt_blocks)))
     nsplit = max(1, min(-(-N // 16), -(-1024 // (ntiles * 4))))
-    if D is not None and (D > 16 or N >= 4096):
+    if D is not None and (D > 16 or N >= 1000):
         # the merged adjoint kernel: one workgroup per CU, one round of 256 over the tiles
         nsplit = max(1, min(-(-N // 16), 256 // ntiles))
     return {'nsub': nsub, 'splits_f': splits_f, 'splits_b': splits_b, 'nsplit': nsplit}

@@ -39,11 +39,12 @@ def _spread(M, Q):
                                      # the corners fused_fits admits: M = 256 with Q = 15 and
                                      # D = 32 (the largest LDS request), M in (200, 256]
                                      (70, 256, 15, 32), (45, 210, 9, 3),
-                                     # point ranges of >= 256 points per split (the per-pair
-                                     # pass's four-waves-per-SIMD build) and, at D > 16, 8 point
-                                     # blocks a range with a ragged last one
-                                     (1000, 200, 4, 4), (999, 120, 2, 20),
-                                     # N >= 4096 with D <= 16: the merged adjoint kernel
+                                     # point ranges of >= 256 points per split (the two-pass
+                                     # form's four-waves-per-SIMD per-pair build at N = 900; the
+                                     # merged kernel at N = 1000) and, at D > 16, 8 point blocks
+                                     # a range with a ragged last one
+                                     (900, 200, 4, 4), (1000, 200, 4, 4), (999, 120, 2, 20),
+                                     # N >= 1000 with D <= 16: the merged adjoint kernel
                                      (4100, 50, 3, 6)])
 def test_fused_layer_matches_plain(N, M, Q, D):
     from geepee_amd import dops
