@@ -893,6 +893,19 @@ __global__ __launch_bounds__(64 * MG_NW) void fused_bwd_merged_kernel(
 #pragma unroll
     for (int db = 0; db < DB; ++db) dacc[u][db] = gpx_d4{0.0, 0.0, 0.0, 0.0};
   }
+  // D <= 4 with Q <= 5 (one D slice, three feature slices): the dB-hat and LW contractions
+  // as per-lane FMAs over the lane's four points (summed over the four lane rows once, at
+  // the end), where the MFMA forms would fill 4 and at most 6 of their 16 output rows
+  constexpr bool VSM = DB == 1 && DS == 1 && KSC == 3;
+  constexpr int VQ = 6;  // Q + 1 <= 6
+  double vd[VSM ? NR : 1][4], vl[VSM ? NR : 1][VQ];
+#pragma unroll
+  for (int u = 0; u < (VSM ? NR : 1); ++u) {
+#pragma unroll
+    for (int d = 0; d < 4; ++d) vd[u][d] = 0.0;
+#pragma unroll
+    for (int q = 0; q < VQ; ++q) vl[u][q] = 0.0;
+  }
   constexpr int NPF = (16 * PF_W + NT - 1) / NT;
   double rp[NPF], rd;
   const int sk = tid >> 5, sc = tid & 31;  // dv staging: row sk (< 16), column sc (< 32)
@@ -949,14 +962,17 @@ __global__ __launch_bounds__(64 * MG_NW) void fused_bwd_merged_kernel(
     for (int s = 0; s < KSN; ++s) pa[s] = (s < KS) ? pas[buf][ka][kq + 4 * s] : 0.0;
 #pragma unroll
     for (int s = 0; s < DS; ++s) da[s] = dvs[buf][ka][(kq + 4 * s) ^ swa];
+    if constexpr (!VSM) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int k = kq + 4 * r;
+      for (int r = 0; r < 4; ++r) {
+        const int k = kq + 4 * r;
 #pragma unroll
-      for (int db = 0; db < DB; ++db) dvv[r][db] = dvs[buf][k][(16 * db + ka) ^ dv_swizzle(k)];
-      dq[r] = pqs[buf][k][ka];
+        for (int db = 0; db < DB; ++db) dvv[r][db] = dvs[buf][k][(16 * db + ka) ^ dv_swizzle(k)];
+        dq[r] = pqs[buf][k][ka];
+      }
     }
     double cs[4] = {0.0, 0.0, 0.0, 0.0}, vw[4] = {0.0, 0.0, 0.0, 0.0};
+    double pp[VSM ? NR : 1][4], LL[VSM ? NR : 1][4];
 #pragma unroll
     for (int u = 0; u < NR; ++u) {
       if (!act[u]) continue;  // wave-uniform
@@ -976,11 +992,19 @@ __global__ __launch_bounds__(64 * MG_NW) void fused_bwd_merged_kernel(
         p[r] = sf4 * gpx_exp_tab(e[r], etab) * ((n0 + kq + 4 * r < nend && jb < M) ? 1.0 : 0.0);
         L[r] = g[r] * p[r];
       }
+      if constexpr (VSM) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
+        for (int r = 0; r < 4; ++r) {
+          pp[u][r] = p[r];
+          LL[u][r] = L[r];
+        }
+      } else {
 #pragma unroll
-        for (int db = 0; db < DB; ++db) dacc[u][db] = gpx_mfma16(dvv[r][db], p[r], dacc[u][db]);
-        lacc[u] = gpx_mfma16(dq[r], L[r], lacc[u]);
+        for (int r = 0; r < 4; ++r) {
+#pragma unroll
+          for (int db = 0; db < DB; ++db) dacc[u][db] = gpx_mfma16(dvv[r][db], p[r], dacc[u][db]);
+          lacc[u] = gpx_mfma16(dq[r], L[r], lacc[u]);
+        }
       }
       // U = L [1 | z_b]: L (lane: n = kq + 4r, b = ka) transposed to the A-operand
       // layout (lane: n = ka, b = kq + 4s) through the wave's LDS
@@ -1003,6 +1027,27 @@ __global__ __launch_bounds__(64 * MG_NW) void fused_bwd_merged_kernel(
         cs[r] += L[r];
       }
     }
+    if constexpr (VSM) {
+      // lane: pair column b = ka, points n = kq + 4r (r < 4): dBhat[d][b] += dv[n][d] p[n][b],
+      // LW[b][q'] += [1 | d_n][q'] L[n][b]
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = kq + 4 * r;
+        double dvl[4], dql[VQ];
+#pragma unroll
+        for (int d = 0; d < 4; ++d) dvl[d] = dvs[buf][k][d ^ dv_swizzle(k)];
+#pragma unroll
+        for (int q = 0; q < VQ; ++q) dql[q] = pqs[buf][k][q];
+#pragma unroll
+        for (int u = 0; u < NR; ++u) {
+          if (!act[u]) continue;
+#pragma unroll
+          for (int d = 0; d < 4; ++d) vd[u][d] = fma(dvl[d], pp[u][r], vd[u][d]);
+#pragma unroll
+          for (int q = 0; q < VQ; ++q) vl[u][q] = fma(dql[q], LL[u][r], vl[u][q]);
+        }
+      }
+    }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       csum[buf][wave][kq + 4 * r][ka] = cs[r];
@@ -1016,6 +1061,20 @@ __global__ __launch_bounds__(64 * MG_NW) void fused_bwd_merged_kernel(
 #pragma unroll
   for (int u = 0; u < NR; ++u) {
     if (!act[u]) continue;
+    if constexpr (VSM) {
+      // sum over the four lane rows (lanes ka + 16 kq), in a fixed order
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        const double t = gpx_swap_add<16>(gpx_swap_add<32>(vd[u][d]));
+        if (kq == 0 && d < D && jb < M) dBpart[(((long)ns * D + d) * M + a[u]) * M + jb] = t;
+      }
+#pragma unroll
+      for (int q = 0; q < VQ; ++q) {
+        const double t = gpx_swap_add<16>(gpx_swap_add<32>(vl[u][q]));
+        if (kq == 0 && q <= Q && jb < M) LWpart[(((long)ns * M + a[u]) * M + jb) * (Q + 1) + q] = t;
+      }
+      continue;
+    }
 #pragma unroll
     for (int db = 0; db < DB; ++db)
 #pragma unroll
