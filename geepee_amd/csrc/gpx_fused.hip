@@ -806,7 +806,8 @@ __device__ __forceinline__ void wave_lds_sync() {
 // Column c of tile C thus has exactly T slots (row sums from tiles (C, s), s >= C; column
 // sums from tiles (s, C), s < C), each written once; V has one slot per tile.  The
 // 8-wave sums of the column sums and V go through LDS in wave order (double-buffered
-// by block parity, so one barrier per block serves the staging and both sums).
+// by block parity, so one barrier per block serves the staging and both sums); a block's
+// sums are read and written out during the next block, beside its first MFMAs.
 // ---------------------------------------------------------------------------
 constexpr int MG_NW = 8;
 template <int DB, int DS, int KSC>
@@ -975,17 +976,22 @@ __global__ __launch_bounds__(64 * MG_NW) void fused_bwd_merged_kernel(
     double pp[VSM ? NR : 1][4], LL[VSM ? NR : 1][4];
 #pragma unroll
     for (int u = 0; u < NR; ++u) {
-      if (!act[u]) continue;  // wave-uniform
       gpx_d4 e = gpx_d4{0.0, 0.0, 0.0, 0.0}, g = gpx_d4{0.0, 0.0, 0.0, 0.0};
+      if (act[u]) {  // wave-uniform
 #pragma unroll
-      for (int s = 0; s < KSN; ++s)
-        if (s < KS) {
-          double f;
-          if constexpr (AF_LDS) f = afs[wave][u][s][lane]; else f = af[u][s];
-          e = gpx_mfma16(pa[s], f, e);
-        }
+        for (int s = 0; s < KSN; ++s)
+          if (s < KS) {
+            double f;
+            if constexpr (AF_LDS) f = afs[wave][u][s][lane]; else f = af[u][s];
+            e = gpx_mfma16(pa[s], f, e);
+          }
 #pragma unroll
-      for (int s = 0; s < DS; ++s) g = gpx_mfma16(da[s], bs[u][s], g);
+        for (int s = 0; s < DS; ++s) g = gpx_mfma16(da[s], bs[u][s], g);
+      }
+      // the previous block's 8-wave sums, while this block's first MFMAs run (its buffer
+      // half is not written again before the next barrier)
+      if (u == 0 && n0 > nbeg) reduce(n0 - 16, buf ^ 1);
+      if (!act[u]) continue;
       double p[4], L[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -1055,9 +1061,9 @@ __global__ __launch_bounds__(64 * MG_NW) void fused_bwd_merged_kernel(
     }
     if (more) stage(buf ^ 1);
     __syncthreads();
-    reduce(n0, buf);
     buf ^= 1;
   }
+  if (nend > nbeg) reduce(nbeg + (nend - nbeg - 1) / 16 * 16, buf ^ 1);  // the last block's
 #pragma unroll
   for (int u = 0; u < NR; ++u) {
     if (!act[u]) continue;
