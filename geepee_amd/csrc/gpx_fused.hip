@@ -184,9 +184,14 @@ __global__ __launch_bounds__(256) void fused_fwd_kernel(int N, int M, int Q, int
                                                         const double *__restrict__ Bp,
                                                         const double *__restrict__ Pab, int S,
                                                         double *__restrict__ part) {
+  // DB = 0: D <= 4, the contraction with Bp as per-lane FMAs over the lane's four pair
+  // columns (summed over the four lane rows once, at the end) instead of MFMAs that would
+  // fill 4 of their 16 output rows
+  constexpr bool VS = DB == 0;
+  constexpr int DBA = VS ? 1 : DB;
   __shared__ double zs[FM_MAX][FQ_MAX + 1];
   __shared__ double lam[FQ_MAX];
-  __shared__ double red[4][DB * 16][17];
+  __shared__ double red[4][DBA * 16][17];
   __shared__ double etab[32];
   int pb, split;
   if (!split_block((N + 16 * NSUB - 1) / (16 * NSUB), S, pb, split)) return;
@@ -214,11 +219,11 @@ __global__ __launch_bounds__(256) void fused_fwd_kernel(int N, int M, int Q, int
 #pragma unroll
   for (int s = 0; s < KSN; ++s) ps[s] = pair_sel((lane >> 4) + 4 * s, Q);
   const double sf2 = exp(2.0 * sf[0]), sf4 = sf2 * sf2;
-  gpx_d4 acc[NSUB][DB];
+  gpx_d4 acc[NSUB][DBA];
 #pragma unroll
   for (int u = 0; u < NSUB; ++u)
 #pragma unroll
-    for (int i = 0; i < DB; ++i) acc[u][i] = gpx_d4{0.0, 0.0, 0.0, 0.0};
+    for (int i = 0; i < DBA; ++i) acc[u][i] = gpx_d4{0.0, 0.0, 0.0, 0.0};
   const int T = (M + 15) / 16, ntiles = T * (T + 1) / 2;
   for (int t = split; t < ntiles; t += S) {
     int A, B;
@@ -233,14 +238,25 @@ __global__ __launch_bounds__(256) void fused_fwd_kernel(int N, int M, int Q, int
       // unconditional loads (column b and row d clamped; rows d >= D of acc are never
       // stored), issued together and waited for once
       const double pab = Pab[(long)a * M + jl];
-      double bv[DB][4];
-#pragma unroll
-      for (int db = 0; db < DB; ++db)
+      // MFMA form: lane (d = 16 db + lane%16, b = lane/16 + 4r); VS: lane (b = lane/16 + 4r,
+      // d < 4), the four rows of lane/16 reading the same values
+      double bv[DBA][4], bs4[VS ? 4 : 1][4];
+      if constexpr (VS) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int d = min(16 * db + (lane & 15), D - 1), b = min(16 * B + (lane >> 4) + 4 * r, M - 1);
-          bv[db][r] = Bp[((long)a * M + b) * D + d];
+          const int b = min(16 * B + (lane >> 4) + 4 * r, M - 1);
+#pragma unroll
+          for (int d = 0; d < 4; ++d) bs4[r][d] = Bp[((long)a * M + b) * D + min(d, D - 1)];
         }
+      } else {
+#pragma unroll
+        for (int db = 0; db < DB; ++db)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int d = min(16 * db + (lane & 15), D - 1), b = min(16 * B + (lane >> 4) + 4 * r, M - 1);
+            bv[db][r] = Bp[((long)a * M + b) * D + d];
+          }
+      }
       double af[KSN];
 #pragma unroll
       for (int s = 0; s < KSN; ++s) af[s] = (s < KS) ? pair_feature_sel(ps[s], zs[a], zs[jl], pab) : 0.0;
@@ -263,21 +279,38 @@ __global__ __launch_bounds__(256) void fused_fwd_kernel(int N, int M, int Q, int
           else
             p[r] = (b < M && nvalid[u]) ? w * gpx_exp_tab(e[r], etab) : 0.0;
         }
+        if constexpr (VS) {
+          // acc[u][0][d]: this lane's partial of part[n = lane%16][d] over its pair columns
 #pragma unroll
-        for (int db = 0; db < DB; ++db)
+          for (int r = 0; r < 4; ++r)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) acc[u][db] = gpx_mfma16(bv[db][r], p[r], acc[u][db]);
+            for (int d = 0; d < 4; ++d) acc[u][0][d] = fma(bs4[r][d], p[r], acc[u][0][d]);
+        } else {
+#pragma unroll
+          for (int db = 0; db < DB; ++db)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[u][db] = gpx_mfma16(bv[db][r], p[r], acc[u][db]);
+        }
       }
     }
   }
 #pragma unroll
   for (int u = 0; u < NSUB; ++u) {
+    if constexpr (VS) {
+      // the four lane rows summed (fixed order), then rows d < 4 of the block sum below
 #pragma unroll
-    for (int db = 0; db < DB; ++db)
+      for (int d = 0; d < 4; ++d) {
+        const double t = gpx_swap_add<16>(gpx_swap_add<32>(acc[u][0][d]));
+        if ((lane >> 4) == 0) red[wave][d][lane & 15] = t;
+      }
+    } else {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) red[wave][db * 16 + (lane >> 4) + 4 * r][lane & 15] = acc[u][db][r];
+      for (int db = 0; db < DB; ++db)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) red[wave][db * 16 + (lane >> 4) + 4 * r][lane & 15] = acc[u][db][r];
+    }
     __syncthreads();
-    for (int idx = tid; idx < DB * 16 * 16; idx += 256) {
+    for (int idx = tid; idx < (VS ? 4 : DB * 16) * 16; idx += 256) {
       const int dd = idx >> 4, nn = idx & 15, n = n0 + 16 * u + nn;
       const double s = red[0][dd][nn] + red[1][dd][nn] + red[2][dd][nn] + red[3][dd][nn];
       if (dd < D && n < N) part[((long)split * N + n) * D + dd] = s;
@@ -1584,7 +1617,9 @@ static int fused_fwd_launch(int N, int M, int Q, int D, const double *mx, const 
     else                                    \
       GPX_FWD1(DBV, NS, 0);                 \
   } while (0)
-  if (D <= 16) {
+  if (D <= 4) {
+    if (nsub == 1) GPX_FWD(0, 1); else if (nsub == 2) GPX_FWD(0, 2); else GPX_FWD(0, 4);
+  } else if (D <= 16) {
     if (nsub == 1) GPX_FWD(1, 1); else if (nsub == 2) GPX_FWD(1, 2); else GPX_FWD(1, 4);
   } else {
     if (nsub == 1) GPX_FWD(2, 1); else if (nsub == 2) GPX_FWD(2, 2); else GPX_FWD(2, 4);
