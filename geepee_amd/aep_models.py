@@ -522,11 +522,25 @@ class SDGPR(Base_Model):
         """prepare's Kuu and precision inverses for every layer, batched over the layers
         of each M; returns per layer (prec, cov, logdet, info) views for finish_prepare."""
         layers = self.sgp_layers
-        for i, layer in enumerate(layers):
-            layer.prepare_params(dp, key_suffix='_%d' % i)
         groups = {}
         for i, layer in enumerate(layers):
             groups.setdefault(layer.M, []).append(i)
+        # theta1's factors of a group in one launch when their packed parameters lie one after
+        # another in the parameter buffer (sorted keys: eta1_R_0, eta1_R_1, ...)
+        Rv, R_all = {}, {}
+        for M, idx in groups.items():
+            etas = [dp['eta1_R_%d' % i] for i in idx]
+            if len(idx) > 1 and all(etas[j + 1].data_ptr() == etas[j].data_ptr() + etas[j].numel() * 8
+                                    for j in range(len(idx) - 1)):
+                P = etas[0].shape[1]
+                Ds = [layers[i].Dout for i in idx]
+                R_all[M] = dops.eta_to_R(torch.as_strided(etas[0], (sum(Ds), P), (P, 1)), M)
+                o = 0
+                for i, d in zip(idx, Ds):
+                    Rv[i] = R_all[M][o:o + d]
+                    o += d
+        for i, layer in enumerate(layers):
+            layer.prepare_params(dp, key_suffix='_%d' % i, R=Rv.get(i))
         precs = {}
         for M, idx in groups.items():
             K = dops.empty(len(idx), M, M, like=layers[idx[0]].zu)
@@ -539,9 +553,19 @@ class SDGPR(Base_Model):
             prec = dops.empty(sum(sizes), M, M, like=K)
             o, spans = 0, []
             for i, n in zip(idx, sizes):
-                layers[i].precisions_into(prec[o:o + n], alpha)
                 spans.append((i, o, n))
                 o += n
+            # R^T R of every layer's output dims in one product, then both precisions of
+            # every layer (Kuu^-1 + theta1, Kuu^-1 + beta theta1) in one launch
+            if len(idx) == 1:
+                Rs = layers[idx[0]].theta_1_R
+            elif M in R_all:
+                Rs = R_all[M]
+            else:
+                Rs = torch.cat([layers[i].theta_1_R for i in idx])
+            X = mm(Rs, Rs, tA=True)
+            beta = (layers[idx[0]].N - alpha) * 1.0 / layers[idx[0]].N
+            dops.prec_form([layers[i].Dout for i in idx], X, Ki, beta, prec)
             cov, ldp, infop = dops.spd_inv(prec)
             for i, o, n in spans:
                 precs[i] = (prec[o:o + n], cov[o:o + n], ldp[o:o + n], infop[o:o + n])

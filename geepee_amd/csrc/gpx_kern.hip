@@ -707,6 +707,29 @@ __global__ void emission_predict_kernel(int Nb, int Dout, int Din, const double 
   vy[e] = v;
 }
 
+// Posterior and cavity precisions of several layers of one M from X = R^T R (the
+// layers' output dims stacked) and their Kuu^-1 (stacked): layer l with D_l outputs at
+// rows x_l of X writes prec[2 x_l + d] = X[x_l + d] + Ki[l] and prec[2 x_l + D_l + d] =
+// beta X[x_l + d] + Ki[l].  One thread per element of X.
+constexpr int kPrecLayers = 16;
+struct PrecLayers {
+  int D[kPrecLayers];
+};
+__global__ __launch_bounds__(256) void prec_form_kernel(int nl, PrecLayers L, int M, const double *__restrict__ X,
+                                                        const double *__restrict__ Ki, double beta,
+                                                        double *__restrict__ prec) {
+  const long MM = (long)M * M;
+  const long e = blockIdx.x * 256L + threadIdx.x;
+  int row = (int)(e / MM), l = 0, x0 = 0;
+  while (l < nl && row >= x0 + L.D[l]) x0 += L.D[l++];
+  if (l >= nl) return;  // past the last layer's rows
+  const long m = e - (long)row * MM;
+  const double x = X[e], k = Ki[l * MM + m];
+  const int d = row - x0;
+  prec[(2L * x0 + d) * MM + m] = x + k;
+  prec[(2L * x0 + L.D[l] + d) * MM + m] = beta * x + k;
+}
+
 }  // namespace
 
 extern "C" {
@@ -811,6 +834,22 @@ int gpx_eta_to_R(int D, int M, const double *eta, double *R, void *stream) {
   long n = (long)D * M * M;
   if (n == 0) return GPX_OK;
   hipLaunchKernelGGL(eta_to_R_kernel, dim3(gpx_cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream, D, M, eta, R);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+int gpx_prec_form(int nl, const int *D, int M, const double *X, const double *Ki, double beta, double *prec,
+                  void *stream) {
+  if (nl < 1 || nl > kPrecLayers || !D || M < 1 || !X || !Ki || !prec) return GPX_EINVAL;
+  PrecLayers L = {};
+  long rows = 0;
+  for (int l = 0; l < nl; ++l) {
+    if (D[l] < 1) return GPX_EINVAL;
+    L.D[l] = D[l];
+    rows += D[l];
+  }
+  hipLaunchKernelGGL(prec_form_kernel, dim3(gpx_cdiv(rows * M * M, 256)), dim3(256), 0, (hipStream_t)stream, nl, L,
+                     M, X, Ki, beta, prec);
   GPX_CHECK_LAUNCH();
   return GPX_OK;
 }

@@ -106,6 +106,16 @@ def eta_to_R(eta, M):
     return R
 
 
+def prec_form(Ds, X, Ki, beta, out):
+    """Per layer l (D = Ds[l] rows of X, stacked): out holds [X_l + Ki[l]; beta X_l + Ki[l]]
+    (2 D matrices), the layers one after another; X (sum Ds, M, M), Ki (len(Ds), M, M)."""
+    import ctypes
+    arr = (ctypes.c_int * len(Ds))(*[int(d) for d in Ds])
+    call('gpx_prec_form', len(Ds), ctypes.cast(arr, ctypes.c_void_p), X.shape[-1], P(X), P(Ki), float(beta),
+         P(out), S())
+    return out
+
+
 def pack_dR(R, G, out=None):
     D, M, _ = R.shape
     if out is None:

@@ -96,12 +96,13 @@ class SGP_Layer(object):
     # the parts of prepare (natural parameterisation), so that a model with several layers
     # of one M can invert all their Kuu in one batched SPD launch and all their precisions
     # in another (SDGPR._prepare_layers)
-    def prepare_params(self, params, key_suffix=''):
+    def prepare_params(self, params, key_suffix='', R=None):
+        """R: theta1's factor already unpacked (a view of a batch over layers), if given."""
         self.ls = params['ls' + key_suffix]
         self.sf = params['sf' + key_suffix]
         self.zu = params['zu' + key_suffix]
         self.theta_2 = params['eta2' + key_suffix]
-        self.theta_1_R = dops.eta_to_R(params['eta1_R' + key_suffix], self.M)
+        self.theta_1_R = dops.eta_to_R(params['eta1_R' + key_suffix], self.M) if R is None else R
         self.theta_1 = None
         self._adj = None
 

@@ -117,6 +117,14 @@ int gpx_dtrace_kzz(int M, int Q, const double *Mm, const double *z, const double
  * Replaces the loop of Base_SGP_Layer.update_hypers (geepee/base_models.py:645-653). */
 int gpx_eta_to_R(int D, int M, const double *eta, double *R, void *stream);
 
+/* The posterior and cavity precisions of nl (<= 16) layers of one M in one launch: from
+ * X = R^T R of the layers' output dims stacked (sum D[l] x M x M) and their Kuu^-1 (nl x M x M),
+ * prec holds per layer [X_l + Kuu^-1_l (D[l]); beta X_l + Kuu^-1_l (D[l])].  The
+ * Kuu^-1 + theta1 and Kuu^-1 + beta theta1 of compute_cavity and update_posterior
+ * (geepee/aep_models.py:523-533, base_models.py:462-470) batched over a deep GP's layers. */
+int gpx_prec_form(int nl, const int *D, int M, const double *X, const double *Ki, double beta, double *prec,
+                  void *stream);
+
 /* out[d, triu(i,j)] = (R (G + G^T))[i,j] * (i == j ? R_ii : 1).
  * Replaces the deta1_R packing (geepee/aep_models.py:267-280, 573-584;
  * base_models.py:504-514). */

@@ -364,3 +364,26 @@ def test_transition_and_emission(B):
     assert nrel(host(o[2]), gh['C']) < 1e-12
     assert nrel(host(o[3]), gh['R']) < 1e-12
     assert abs(host(o[4])[0] - lz) < 1e-12 * abs(lz)
+
+
+def test_prec_form_matches_numpy():
+    """Both precisions of several layers (Kuu^-1 + R^T R, Kuu^-1 + beta R^T R) from the
+    stacked R^T R in one launch, the layers of ragged output counts."""
+    from geepee_amd import dops
+    say('prec_form')
+    rs = np.random.RandomState(7)
+    Ds, M, beta = [3, 1, 2], 37, 0.75
+    X = rs.randn(sum(Ds), M, M)
+    Ki = rs.randn(len(Ds), M, M)
+    out = dev(np.full((2 * sum(Ds), M, M), np.nan))
+    dops.prec_form(Ds, dev(X), dev(Ki), beta, out)
+    got = host(out)
+    o = x = 0
+    for l, d in enumerate(Ds):
+        assert np.array_equal(got[o:o + d], X[x:x + d] + Ki[l])
+        # beta x + k may be one fused multiply-add on the device: within an ulp of the
+        # rounded product plus k
+        want = beta * X[x:x + d] + Ki[l]
+        assert np.all(np.abs(got[o + d:o + 2 * d] - want) <= 2.3e-16 * (np.abs(beta * X[x:x + d]) + np.abs(want)))
+        o += 2 * d
+        x += d
