@@ -36,6 +36,7 @@ SIGNATURES = {
     'gpx_eta_to_R': (_i, [_i, _i, _p, _p, _p]),
     'gpx_pack_dR': (_i, [_i, _i, _p, _p, _p, _p]),
     'gpx_prec_form': (_i, [_i, _p, _i, _p, _p, _d, _p, _p]),
+    'gpx_kuu_batched': (_i, [_i, _i, _p, _p, _p, _p, _d, _p, _p]),
     'gpx_lincomb': (_i, [_l, _d, _p, _l, _d, _p, _l, _d, _p, _l, _p, _p]),
     'gpx_sym': (_i, [_i, _i, _d, _p, _p, _p]),
     'gpx_det_var': (_i, [_i, _i, _i, _p, _p, _p, _p, _p]),

@@ -13,7 +13,7 @@ import numpy as np
 import torch
 
 from . import _lib, checkpoint, config, dops
-from .config import PROP_MM, PROP_MC
+from .config import PROP_MM, PROP_MC, JITTER
 from .dops import mm, lincomb, colsum
 from . import init
 from .layers import SGP_Layer
@@ -544,8 +544,13 @@ class SDGPR(Base_Model):
         precs = {}
         for M, idx in groups.items():
             K = dops.empty(len(idx), M, M, like=layers[idx[0]].zu)
-            for g, i in enumerate(idx):
-                layers[i].kuu_into(K[g])
+            if len(idx) > 1:
+                dops.kuu_batched([layers[i].zu for i in idx], [layers[i].ls for i in idx],
+                                 [layers[i].sf for i in idx], JITTER, K)
+                for g, i in enumerate(idx):
+                    layers[i].Kuu = K[g]
+            else:
+                layers[idx[0]].kuu_into(K[0])
             Ki, ld, info = dops.spd_inv(K)
             for g, i in enumerate(idx):
                 layers[i].set_kuu_inverse(Ki[g], ld[g:g + 1], info[g:g + 1])

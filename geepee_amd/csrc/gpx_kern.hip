@@ -707,6 +707,31 @@ __global__ void emission_predict_kernel(int Nb, int Dout, int Din, const double 
   vy[e] = v;
 }
 
+// Kuu of several layers of one M in one launch: K[l] = sf2_l exp(-r2 / 2) + jitter I over
+// the layer's inducing inputs z_l (M x Q_l), the arithmetic of se_kernel with x = z.
+constexpr int kKuuLayers = 16;
+struct KuuLayers {
+  const double *z[kKuuLayers], *ls[kKuuLayers], *sf[kKuuLayers];
+  int Q[kKuuLayers];
+};
+__global__ __launch_bounds__(256) void kuu_batched_kernel(int nl, KuuLayers L, int M, double jitter,
+                                                          double *__restrict__ K) {
+  const long MM = (long)M * M;
+  const long e = blockIdx.x * 256L + threadIdx.x;
+  if (e >= nl * MM) return;
+  const int l = (int)(e / MM), n = (int)((e % MM) / M), m = (int)(e % M);
+  const double *z = L.z[l], *ls = L.ls[l];
+  const int Q = L.Q[l];
+  double r2 = 0.0;
+  for (int q = 0; q < Q; ++q) {
+    const double d = z[n * Q + q] - z[m * Q + q];
+    r2 += d * d / exp(2.0 * ls[q]);
+  }
+  double k = exp(2.0 * L.sf[l][0]) * exp(-0.5 * r2);
+  if (n == m) k += jitter;
+  K[e] = k;
+}
+
 // Posterior and cavity precisions of several layers of one M from X = R^T R (the
 // layers' output dims stacked) and their Kuu^-1 (stacked): layer l with D_l outputs at
 // rows x_l of X writes prec[2 x_l + d] = X[x_l + d] + Ki[l] and prec[2 x_l + D_l + d] =
@@ -834,6 +859,23 @@ int gpx_eta_to_R(int D, int M, const double *eta, double *R, void *stream) {
   long n = (long)D * M * M;
   if (n == 0) return GPX_OK;
   hipLaunchKernelGGL(eta_to_R_kernel, dim3(gpx_cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream, D, M, eta, R);
+  GPX_CHECK_LAUNCH();
+  return GPX_OK;
+}
+
+int gpx_kuu_batched(int nl, int M, const double *const *z, const int *Q, const double *const *ls,
+                    const double *const *sf, double jitter, double *K, void *stream) {
+  if (nl < 1 || nl > kKuuLayers || M < 1 || !z || !Q || !ls || !sf || !K) return GPX_EINVAL;
+  KuuLayers L = {};
+  for (int l = 0; l < nl; ++l) {
+    if (!z[l] || !ls[l] || !sf[l] || Q[l] < 1) return GPX_EINVAL;
+    L.z[l] = z[l];
+    L.ls[l] = ls[l];
+    L.sf[l] = sf[l];
+    L.Q[l] = Q[l];
+  }
+  hipLaunchKernelGGL(kuu_batched_kernel, dim3(gpx_cdiv((long)nl * M * M, 256)), dim3(256), 0, (hipStream_t)stream, nl,
+                     L, M, jitter, K);
   GPX_CHECK_LAUNCH();
   return GPX_OK;
 }

@@ -106,6 +106,21 @@ def eta_to_R(eta, M):
     return R
 
 
+def kuu_batched(zs, lss, sfs, jitter, out):
+    """out[l] = se(zs[l], zs[l], lss[l], sfs[l]) + jitter I for layers of one M, one launch."""
+    import ctypes
+    nl = len(zs)
+    vp = lambda ts: (ctypes.c_void_p * nl)(*[t.data_ptr() for t in ts])
+    zp, lp, sp = vp(zs), vp(lss), vp(sfs)
+    qs = (ctypes.c_int * nl)(*[int(z.shape[1]) for z in zs])
+    for z in zs:
+        if not z.is_contiguous():
+            raise ValueError('kuu_batched needs contiguous inducing inputs')
+    call('gpx_kuu_batched', nl, zs[0].shape[0], ctypes.cast(zp, ctypes.c_void_p), ctypes.cast(qs, ctypes.c_void_p),
+         ctypes.cast(lp, ctypes.c_void_p), ctypes.cast(sp, ctypes.c_void_p), float(jitter), P(out), S())
+    return out
+
+
 def prec_form(Ds, X, Ki, beta, out):
     """Per layer l (D = Ds[l] rows of X, stacked): out holds [X_l + Ki[l]; beta X_l + Ki[l]]
     (2 D matrices), the layers one after another; X (sum Ds, M, M), Ki (len(Ds), M, M)."""

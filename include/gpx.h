@@ -117,6 +117,14 @@ int gpx_dtrace_kzz(int M, int Q, const double *Mm, const double *z, const double
  * Replaces the loop of Base_SGP_Layer.update_hypers (geepee/base_models.py:645-653). */
 int gpx_eta_to_R(int D, int M, const double *eta, double *R, void *stream);
 
+/* Kuu of nl (<= 16) layers of one M in one launch: K (nl x M x M), K[l] = the squared
+ * exponential kernel of layer l's inducing inputs z[l] (M x Q[l]) with its ls[l] (Q[l]) and
+ * sf[l] (1), plus jitter on the diagonal; the entries of gpx_se_kernel(z[l], z[l]).  The
+ * compute_kernel(zu, zu) + jitter of Base_SGP_Layer.update_hypers
+ * (geepee/base_models.py:456-461) batched over a deep GP's layers. */
+int gpx_kuu_batched(int nl, int M, const double *const *z, const int *Q, const double *const *ls,
+                    const double *const *sf, double jitter, double *K, void *stream);
+
 /* The posterior and cavity precisions of nl (<= 16) layers of one M in one launch: from
  * X = R^T R of the layers' output dims stacked (sum D[l] x M x M) and their Kuu^-1 (nl x M x M),
  * prec holds per layer [X_l + Kuu^-1_l (D[l]); beta X_l + Kuu^-1_l (D[l])].  The

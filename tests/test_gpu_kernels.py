@@ -387,3 +387,20 @@ def test_prec_form_matches_numpy():
         assert np.all(np.abs(got[o + d:o + 2 * d] - want) <= 2.3e-16 * (np.abs(beta * X[x:x + d]) + np.abs(want)))
         o += 2 * d
         x += d
+
+
+def test_kuu_batched_matches_se():
+    """Kuu of several layers (different input dims) in one launch: the bits of the
+    single-layer squared-exponential kernel plus jitter."""
+    from geepee_amd import dops
+    say('kuu_batched')
+    rs = np.random.RandomState(8)
+    M, Qs = 45, [10, 5, 1]
+    zs = [dev(rs.randn(M, q)) for q in Qs]
+    lss = [dev(0.3 * rs.randn(q)) for q in Qs]
+    sfs = [dev(0.2 * rs.randn(1)) for _ in Qs]
+    K = dev(np.full((len(Qs), M, M), np.nan))
+    dops.kuu_batched(zs, lss, sfs, 1e-5, K)
+    for l in range(len(Qs)):
+        want = dops.se(zs[l], zs[l], lss[l], sfs[l], jitter=1e-5)
+        assert torch.equal(K[l], want), l
