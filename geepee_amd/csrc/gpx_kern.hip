@@ -457,12 +457,15 @@ __global__ void transition_finish(int nblk, int B, int D, const double *__restri
   scal[1] = sdv * 2.0 * sn2 / alpha + scale * (double)B * D * (1.0 - alpha);
 }
 
-// Gauss_Emis.compute_emission_tilted (lik_layers.py:573-627) for Dout <= 16, one thread
-// per time step, Vy factorised by Cholesky in a padded LDS slice.  part[blk] collects
-// [logZ, dC (Dout x Din), dR (Dout)] partial sums of the block (unscaled).
+// Gauss_Emis.compute_emission_tilted (lik_layers.py:573-627) for Dout <= 8, one thread
+// per time step; DT = Dout fixed at compile time, so Vy, its Cholesky factor, the factor's
+// inverse and Vy^{-1} live in registers (fully unrolled, lower triangles packed).
+// part[blk] collects [logZ, dC (Dout x Din), dR (Dout)] sums over the block's 64 steps
+// (unscaled), summed with wave reductions.
 constexpr int EM_T = 64;
 constexpr int EM_DMAX = 8;
-__global__ __launch_bounds__(EM_T) void emission_kernel(int Nb, int Dout, int Din, const double *__restrict__ C,
+template <int DT>
+__global__ __launch_bounds__(EM_T) void emission_kernel(int Nb, int Din, const double *__restrict__ C,
                                                         const double *__restrict__ Rp,
                                                         const double *__restrict__ y, int ldy,
                                                         const double *__restrict__ mx, int ldm,
@@ -470,111 +473,132 @@ __global__ __launch_bounds__(EM_T) void emission_kernel(int Nb, int Dout, int Di
                                                         double alpha, double scale,
                                                         double *__restrict__ dmx, double *__restrict__ dvx,
                                                         double *__restrict__ part) {
-  __shared__ double Ls[EM_T][EM_DMAX * EM_DMAX + 1];
+  constexpr int NL = DT * (DT + 1) / 2;
   __shared__ double Cs[EM_DMAX * EM_DMAX];
   __shared__ double Rs[EM_DMAX];
-  __shared__ double acc[EM_T][EM_DMAX * EM_DMAX + EM_DMAX + 2];
   const int tid = threadIdx.x, t = blockIdx.x * EM_T + tid;
-  for (int e = tid; e < Dout * Din; e += EM_T) Cs[e] = C[e];
-  for (int e = tid; e < Dout; e += EM_T) Rs[e] = exp(2.0 * Rp[e]);
+  for (int e = tid; e < DT * Din; e += EM_T) Cs[e] = C[e];
+  if (tid < DT) Rs[tid] = exp(2.0 * Rp[tid]);
   __syncthreads();
-  const int W = Dout * Din + Dout + 1;
-  double *my = acc[tid];
-  for (int i = 0; i < W; ++i) my[i] = 0.0;
-  if (t < Nb) {
-    double *L = Ls[tid];
-    const double *vt = vx + (long)t * ldv, *mt = mx + (long)t * ldm;
-    // Vy = diag(R/alpha) + C diag(vx) C^T
-    for (int a = 0; a < Dout; ++a)
-      for (int b = 0; b <= a; ++b) {
-        double s = 0.0;
-        for (int c = 0; c < Din; ++c) s += Cs[a * Din + c] * vt[c] * Cs[b * Din + c];
-        if (a == b) s += Rs[a] / alpha;
-        L[a * Dout + b] = s;
-      }
-    // Cholesky (lower) in place
-    double logdetV = 0.0;
-    for (int k = 0; k < Dout; ++k) {
-      double s = L[k * Dout + k];
-      for (int j = 0; j < k; ++j) s -= L[k * Dout + j] * L[k * Dout + j];
-      const double lkk = sqrt(s);
-      L[k * Dout + k] = lkk;
-      logdetV += 2.0 * log(lkk);
-      for (int i = k + 1; i < Dout; ++i) {
-        double u = L[i * Dout + k];
-        for (int j = 0; j < k; ++j) u -= L[i * Dout + j] * L[k * Dout + j];
-        L[i * Dout + k] = u / lkk;
-      }
+  const bool ok = t < Nb;
+  const int tt = ok ? t : Nb - 1;  // padding steps compute a real step's values, weighted 0
+  const double wgt = ok ? 1.0 : 0.0;
+  const double *vt = vx + (long)tt * ldv, *mt = mx + (long)tt * ldm;
+#define LI(a, b) ((a) * ((a) + 1) / 2 + (b))
+  // Vy = diag(R/alpha) + C diag(vx) C^T (lower triangle)
+  double L[NL];
+#pragma unroll
+  for (int a = 0; a < DT; ++a)
+#pragma unroll
+    for (int b = 0; b <= a; ++b) {
+      double sum = 0.0;
+      for (int c = 0; c < Din; ++c) sum += Cs[a * Din + c] * vt[c] * Cs[b * Din + c];
+      if (a == b) sum += Rs[a] / alpha;
+      L[LI(a, b)] = sum;
     }
-    // r = y - C mx;  u = Vy^{-1} r
-    double r[EM_DMAX], u[EM_DMAX];
-    for (int a = 0; a < Dout; ++a) {
-      double s = y[(long)t * ldy + a];
-      for (int c = 0; c < Din; ++c) s -= Cs[a * Din + c] * mt[c];
-      r[a] = s;
+  // Cholesky (lower) in place
+  double logdetV = 0.0;
+#pragma unroll
+  for (int k = 0; k < DT; ++k) {
+    double sum = L[LI(k, k)];
+#pragma unroll
+    for (int j = 0; j < k; ++j) sum -= L[LI(k, j)] * L[LI(k, j)];
+    const double lkk = sqrt(sum);
+    L[LI(k, k)] = lkk;
+    logdetV += 2.0 * log(lkk);
+#pragma unroll
+    for (int i = k + 1; i < DT; ++i) {
+      double v = L[LI(i, k)];
+#pragma unroll
+      for (int j = 0; j < k; ++j) v -= L[LI(i, j)] * L[LI(k, j)];
+      L[LI(i, k)] = v / lkk;
     }
-    for (int a = 0; a < Dout; ++a) {  // forward
-      double s = r[a];
-      for (int j = 0; j < a; ++j) s -= L[a * Dout + j] * u[j];
-      u[a] = s / L[a * Dout + a];
+  }
+  // r = y - C mx;  u = Vy^{-1} r
+  double r[DT], u[DT];
+#pragma unroll
+  for (int a = 0; a < DT; ++a) {
+    double v = y[(long)tt * ldy + a];
+    for (int c = 0; c < Din; ++c) v -= Cs[a * Din + c] * mt[c];
+    r[a] = v;
+  }
+#pragma unroll
+  for (int a = 0; a < DT; ++a) {  // forward
+    double v = r[a];
+#pragma unroll
+    for (int j = 0; j < a; ++j) v -= L[LI(a, j)] * u[j];
+    u[a] = v / L[LI(a, a)];
+  }
+#pragma unroll
+  for (int a = DT - 1; a >= 0; --a) {  // backward
+    double v = u[a];
+#pragma unroll
+    for (int j = a + 1; j < DT; ++j) v -= L[LI(j, a)] * u[j];
+    u[a] = v / L[LI(a, a)];
+  }
+  // Li = L^{-1} (lower), then Vinv = Li^T Li (lower triangle; symmetric)
+  double Li[NL];
+#pragma unroll
+  for (int j = 0; j < DT; ++j)
+#pragma unroll
+    for (int i = j; i < DT; ++i) {
+      double v = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+      for (int k = j; k < i; ++k) v -= L[LI(i, k)] * Li[LI(k, j)];
+      Li[LI(i, j)] = v / L[LI(i, i)];
     }
-    for (int a = Dout - 1; a >= 0; --a) {  // backward
-      double s = u[a];
-      for (int j = a + 1; j < Dout; ++j) s -= L[j * Dout + a] * u[j];
-      u[a] = s / L[a * Dout + a];
+  double Vi[NL];
+#pragma unroll
+  for (int a = 0; a < DT; ++a)
+#pragma unroll
+    for (int b = 0; b <= a; ++b) {
+      double v = 0.0;
+#pragma unroll
+      for (int k = a; k < DT; ++k) v += Li[LI(k, a)] * Li[LI(k, b)];
+      Vi[LI(a, b)] = v;
     }
-    // Linv (lower) into the strictly-upper part is awkward; invert L into a local array.
-    double Li[EM_DMAX * EM_DMAX];
-    for (int j = 0; j < Dout; ++j)
-      for (int i = 0; i < Dout; ++i) {
-        if (i < j) { Li[i * Dout + j] = 0.0; continue; }
-        double s = (i == j) ? 1.0 : 0.0;
-        for (int k = j; k < i; ++k) s -= L[i * Dout + k] * Li[k * Dout + j];
-        Li[i * Dout + j] = s / L[i * Dout + i];
-      }
-    // Vinv = Li^T Li, overwrite L storage with it
-    for (int a = 0; a < Dout; ++a)
-      for (int b = 0; b < Dout; ++b) {
-        double s = 0.0;
-        for (int k = (a > b ? a : b); k < Dout; ++k) s += Li[k * Dout + a] * Li[k * Dout + b];
-        L[a * Dout + b] = s;
-      }
-    double quad = 0.0, sumlogR = 0.0;
-    for (int a = 0; a < Dout; ++a) {
-      quad += r[a] * u[a];
-      sumlogR += log(Rs[a]);
+#define VI(a, b) ((a) >= (b) ? Vi[LI(a, b)] : Vi[LI(b, a)])
+  double quad = 0.0, sumlogR = 0.0;
+#pragma unroll
+  for (int a = 0; a < DT; ++a) {
+    quad += r[a] * u[a];
+    sumlogR += log(Rs[a]);
+  }
+  // logdet(I + alpha C V C^T / R) = logdet(Vy) + Dout log(alpha) - sum log R
+  const double vlog = logdetV + DT * log(alpha) - sumlogR;
+  double part0 = -0.5 * quad - 0.5 * vlog - 0.5 * DT * alpha * log(6.283185307179586) - 0.5 * alpha * sumlogR;
+  part0 = gpx_wave_sum(wgt * part0);
+  if (tid == 0) part[(long)blockIdx.x * (DT * Din + DT + 1)] = part0;
+  // dSigma = -Vinv/2 + u u^T / 2
+  const long pb = (long)blockIdx.x * (DT * Din + DT + 1);
+#pragma unroll
+  for (int a = 0; a < DT; ++a) {
+    const double dr = gpx_wave_sum(wgt * (-0.5 * VI(a, a) + 0.5 * u[a] * u[a]));
+    if (tid == 0) part[pb + 1 + DT * Din + a] = dr;
+    for (int c = 0; c < Din; ++c) {
+      double v = 0.0;
+#pragma unroll
+      for (int b = 0; b < DT; ++b) v += (-0.5 * VI(a, b) + 0.5 * u[a] * u[b]) * Cs[b * Din + c];
+      const double dc = gpx_wave_sum(wgt * (u[a] * mt[c] + 2.0 * v * vt[c]));
+      if (tid == 0) part[pb + 1 + a * Din + c] = dc;
     }
-    // logdet(I + alpha C V C^T / R) = logdet(Vy) + Dout log(alpha) - sum log R
-    const double vlog = logdetV + Dout * log(alpha) - sumlogR;
-    my[0] = -0.5 * quad - 0.5 * vlog - 0.5 * Dout * alpha * log(6.283185307179586) -
-            0.5 * alpha * sumlogR;
-    // dSigma = -Vinv/2 + u u^T / 2
-    for (int a = 0; a < Dout; ++a) {
-      my[1 + Dout * Din + a] = (-0.5 * L[a * Dout + a] + 0.5 * u[a] * u[a]);
-      for (int c = 0; c < Din; ++c) {
-        double s = 0.0;
-        for (int b = 0; b < Dout; ++b)
-          s += (-0.5 * L[a * Dout + b] + 0.5 * u[a] * u[b]) * Cs[b * Din + c];
-        my[1 + a * Din + c] = u[a] * mt[c] + 2.0 * s * vt[c];
-      }
-    }
+  }
+  if (ok) {
     for (int c = 0; c < Din; ++c) {
       double gm = 0.0, gv = 0.0;
-      for (int a = 0; a < Dout; ++a) {
+#pragma unroll
+      for (int a = 0; a < DT; ++a) {
         gm += u[a] * Cs[a * Din + c];
-        for (int b = 0; b < Dout; ++b)
-          gv += (-0.5 * L[a * Dout + b] + 0.5 * u[a] * u[b]) * Cs[a * Din + c] * Cs[b * Din + c];
+#pragma unroll
+        for (int b = 0; b < DT; ++b)
+          gv += (-0.5 * VI(a, b) + 0.5 * u[a] * u[b]) * Cs[a * Din + c] * Cs[b * Din + c];
       }
       dmx[(long)t * Din + c] = gm * scale;
       dvx[(long)t * Din + c] = gv * scale;
     }
   }
-  __syncthreads();
-  for (int i = tid; i < W; i += EM_T) {
-    double s = 0.0;
-    for (int k = 0; k < EM_T; ++k) s += acc[k][i];
-    part[(long)blockIdx.x * W + i] = s;
-  }
+#undef VI
+#undef LI
 }
 
 // logZ, dC, dR from the block partials (the dR chain of lik_layers.py:610-613).
@@ -1010,8 +1034,20 @@ int gpx_emission_tilted(int Nb, int Dout, int Din, const double *C, const double
     return GPX_EINVAL;
   hipStream_t st = (hipStream_t)stream;
   const unsigned nblk = gpx_cdiv(Nb, EM_T);
-  hipLaunchKernelGGL(emission_kernel, dim3(nblk), dim3(EM_T), 0, st, Nb, Dout, Din, C, Rp, y, ldy, mx, ldm, vx,
-                     ldv, alpha, scale, dmx, dvx, work);
+#define GPX_EMIS(DTV)                                                                                         \
+  hipLaunchKernelGGL((emission_kernel<DTV>), dim3(nblk), dim3(EM_T), 0, st, Nb, Din, C, Rp, y, ldy, mx, ldm, vx, \
+                     ldv, alpha, scale, dmx, dvx, work)
+  switch (Dout) {
+    case 1: GPX_EMIS(1); break;
+    case 2: GPX_EMIS(2); break;
+    case 3: GPX_EMIS(3); break;
+    case 4: GPX_EMIS(4); break;
+    case 5: GPX_EMIS(5); break;
+    case 6: GPX_EMIS(6); break;
+    case 7: GPX_EMIS(7); break;
+    default: GPX_EMIS(8); break;
+  }
+#undef GPX_EMIS
   GPX_CHECK_LAUNCH();
   hipLaunchKernelGGL(emission_final, dim3(1), dim3(256), 0, st, (int)nblk, Nb, Dout, Din, (const double *)work,
                      Rp, alpha, scale, dC, dR, scal);
