@@ -1199,10 +1199,23 @@ __global__ __launch_bounds__(256) void fused_rows_finish(
   double c[FRED];
 #pragma unroll
   for (int i = 0; i < FRED; ++i) c[i] = 0.0;
-  for (int m = lane; m < M; m += 64) {
-    double T = 0.0;
-#pragma unroll 8
-    for (int s = 0; s < S; ++s) T += Tpart[((long)s * N + n) * M + m];  // loads issue ahead of the adds
+  // T = sum of the S slots of Tpart for this lane's columns m = lane + 64 j (j < 4, M <= 256),
+  // the four columns' loads of a slot issued together (each column summed in slot order)
+  constexpr int MJ = (FM_MAX + 63) / 64;
+  double Tm[MJ];
+#pragma unroll
+  for (int j = 0; j < MJ; ++j) Tm[j] = 0.0;
+  for (int s = 0; s < S; ++s) {
+    const double *row = Tpart + ((long)s * N + n) * M;
+#pragma unroll
+    for (int j = 0; j < MJ; ++j)
+      if (lane + 64 * j < M) Tm[j] += row[lane + 64 * j];
+  }
+#pragma unroll
+  for (int j = 0; j < MJ; ++j) {
+    const int m = lane + 64 * j;
+    if (m >= M) break;
+    const double T = Tm[j];
     const double Lp = dpsi1[(long)n * M + m] * psi1[(long)n * M + m];
     Tout[(long)n * M + m] = T;
     Lpout[(long)n * M + m] = Lp;
@@ -1232,11 +1245,27 @@ __global__ __launch_bounds__(256) void fused_rows_finish(
       out_B1 = c[2 + 2 * QN + q];
       out_C1 = c[2 + 3 * QN + q];
     }
+  // V = sum of the SV slots of Vpart: lane l sums the Q values of slots l, l + 64, ...
+  // (one row of Q contiguous values each), then a wave sum per q
+  double Vq[QN];
+#pragma unroll
+  for (int q = 0; q < QN; ++q) Vq[q] = 0.0;
+  for (int s = lane; s < SV; s += 64) {
+    const double *row = Vpart + ((long)s * N + n) * Q;
+#pragma unroll
+    for (int q = 0; q < QN; ++q)
+      if (q < Q) Vq[q] += row[q];
+  }
+  double Vsel = 0.0;
+#pragma unroll
+  for (int q = 0; q < QN; ++q) {
+    if (q >= Q) continue;
+    const double t = gpx_wave_sum(Vq[q]);
+    if (q == lane) Vsel = t;
+  }
   if (lane < Q) {
     const int q = lane;
-    double V = 0.0;
-#pragma unroll 8
-    for (int s = 0; s < SV; ++s) V += Vpart[((long)s * N + n) * Q + q];
+    const double V = Vsel;
     const double S0 = c[0], A1 = c[1];
     const double S1 = out_S1, TZ2 = out_TZ2, B1 = out_B1, C1 = out_C1;
     const double l = exp(2.0 * ls[q]);
