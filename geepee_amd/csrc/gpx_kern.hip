@@ -24,9 +24,10 @@ __global__ void se_kernel(int N, int M, int Q, const double *__restrict__ x, int
   K[e] = k;
 }
 
-// kfucompDer (kernels.py:381-399) with L = dK * K, pass 1 over chunks of KFU_NB rows:
-// partial[blk][m] = [colsum_m, sum_n L x_nq (Q), sum_n L (z_mq - x_nq)^2 (Q)], one
-// thread per (m, q) entry, the chunk's L staged in LDS;
+// kfucompDer (kernels.py:381-399) with L = dK * K, pass 1 over slabs of KFU_NB rows, one
+// slab per workgroup staged in LDS (rows past N as zeros): part[blk][m] = [colsum_m,
+// sum_n L x_nq (Q), sum_n L (z_mq - x_nq)^2 (Q)], one thread per (m, q') entry with the
+// slab's rows unrolled so their LDS reads issue together;
 // dx_nq = sum_m L_nm (z_mq - x_nq) / l_q.
 constexpr int KFU_NB = 16;
 __global__ __launch_bounds__(256) void kfu_bwd_partial(int N, int M, int Q, const double *__restrict__ dK,
@@ -39,10 +40,14 @@ __global__ __launch_bounds__(256) void kfu_bwd_partial(int N, int M, int Q, cons
   __shared__ double xs[KFU_NB][17];
   __shared__ double Ls[KFU_NB][257];
   const int n0 = blockIdx.x * KFU_NB, nb = min(KFU_NB, N - n0), tid = threadIdx.x;
-  for (int e = tid; e < nb * Q; e += 256) xs[e / Q][e % Q] = x[(long)(n0 + e / Q) * Q + e % Q];
-  for (int e = tid; e < nb * M; e += 256) {
-    const long o = (long)(n0 + e / M) * M + e % M;
-    Ls[e / M][e % M] = dK[o] * K[o];
+  for (int e = tid; e < KFU_NB * Q; e += 256) {
+    const int i = e / Q, q = e % Q;
+    xs[i][q] = i < nb ? x[(long)(n0 + i) * Q + q] : 0.0;
+  }
+  for (int e = tid; e < KFU_NB * M; e += 256) {
+    const int i = e / M, m = e % M;
+    const long o = (long)(n0 + i) * M + m;
+    Ls[i][m] = i < nb ? dK[o] * K[o] : 0.0;
   }
   __syncthreads();
   const int W = 2 * Q + 1;
@@ -51,13 +56,15 @@ __global__ __launch_bounds__(256) void kfu_bwd_partial(int N, int M, int Q, cons
     double *P = part + ((long)blockIdx.x * M + m) * W;
     if (q1 == 0) {
       double cs = 0.0;
-      for (int i = 0; i < nb; ++i) cs += Ls[i][m];
+#pragma unroll
+      for (int i = 0; i < KFU_NB; ++i) cs += Ls[i][m];
       P[0] = cs;
     } else {
       const int q = q1 - 1;
       const double zq = z[m * Q + q];
       double lx = 0.0, ld2 = 0.0;
-      for (int i = 0; i < nb; ++i) {
+#pragma unroll
+      for (int i = 0; i < KFU_NB; ++i) {
         const double L = Ls[i][m], xv = xs[i][q], d = zq - xv;
         lx += L * xv;
         ld2 += L * d * d;
@@ -75,46 +82,57 @@ __global__ __launch_bounds__(256) void kfu_bwd_partial(int N, int M, int Q, cons
   }
 }
 
-// pass 2, one thread per (m, q'): the chunk sums in chunk order; dZ_mq =
-// -(z_mq colsum_m - Lx_mq)/l_q; colsum_m and Ld2_mq go to `red` for pass 3.
+// Pass 2: workgroup g sums the slabs' partials (in slab order) for the rows
+// m in [g mb, (g + 1) mb), mb = 256 / (2Q + 1), one thread per partial column with its
+// loads issued KFU_LB at a time; then dZ_mq = -(z_mq colsum_m - Lx_mq) / l_q, and
+// colsum_m, Ld2_mq go to `red` (M x (Q + 1)) for pass 3.
+constexpr int KFU_LB = 16;
 __global__ __launch_bounds__(256) void kfu_bwd_final(int nblk, int M, int Q, const double *__restrict__ part,
                                                      const double *__restrict__ z,
                                                      const double *__restrict__ ls, double *__restrict__ dZ,
                                                      double *__restrict__ red) {
-  const int e = blockIdx.x * 256 + threadIdx.x;
-  if (e >= M * (Q + 1)) return;
-  const int m = e / (Q + 1), q1 = e % (Q + 1), W = 2 * Q + 1;
-  double cs = 0.0;
-#pragma unroll 8
-  for (int b = 0; b < nblk; ++b) cs += part[((long)b * M + m) * W];
-  if (q1 == 0) {
-    red[e] = cs;
-    return;
+  __shared__ double sm[256];
+  const int W = 2 * Q + 1, mb = 256 / W, m0 = blockIdx.x * mb, tid = threadIdx.x;
+  const int ncol = min(mb, M - m0) * W;
+  const long bs = (long)M * W;
+  if (tid < ncol) {
+    const double *P = part + (long)m0 * W + tid;
+    double s = 0.0;
+    for (int b0 = 0; b0 < nblk; b0 += KFU_LB) {
+      double v[KFU_LB];
+#pragma unroll
+      for (int u = 0; u < KFU_LB; ++u) v[u] = b0 + u < nblk ? P[(b0 + u) * bs] : 0.0;
+#pragma unroll
+      for (int u = 0; u < KFU_LB; ++u)
+        if (b0 + u < nblk) s += v[u];
+    }
+    sm[tid] = s;
   }
-  const int q = q1 - 1;
-  double lx = 0.0, ld2 = 0.0;
-#pragma unroll 8
-  for (int b = 0; b < nblk; ++b) {
-    const double *P = part + ((long)b * M + m) * W;
-    lx += P[1 + q];
-    ld2 += P[1 + Q + q];
+  __syncthreads();
+  for (int e = tid; e < min(mb, M - m0) * Q; e += 256) {
+    const int ml = e / Q, q = e % Q, m = m0 + ml;
+    dZ[m * Q + q] = -(z[m * Q + q] * sm[ml * W] - sm[ml * W + 1 + q]) / exp(2.0 * ls[q]);
+    red[m * (Q + 1) + 1 + q] = sm[ml * W + 1 + Q + q];
   }
-  dZ[m * Q + q] = -(z[m * Q + q] * cs - lx) / exp(2.0 * ls[q]);
-  red[e] = ld2;
+  if (tid < min(mb, M - m0)) red[(m0 + tid) * (Q + 1)] = sm[tid * W];
 }
 
-// pass 3: dsf2 = sum_m colsum_m / sf2, dell_q = sum_m Ld2_mq / ell^3 (m in order).
-__global__ void kfu_bwd_scalars(int M, int Q, const double *__restrict__ red, const double *__restrict__ ls,
-                                const double *__restrict__ sf, double *__restrict__ dsf2,
-                                double *__restrict__ dell) {
-  const int q1 = threadIdx.x;
-  if (q1 > Q) return;
+// pass 3, one wave per output j <= Q, lanes over m and a fixed-order wave reduction:
+// dsf2 = sum_m colsum_m / sf2, dell_q = sum_m Ld2_mq / ell^3.
+__global__ __launch_bounds__(64) void kfu_bwd_scalars(int M, int Q, const double *__restrict__ red,
+                                                      const double *__restrict__ ls,
+                                                      const double *__restrict__ sf, double *__restrict__ dsf2,
+                                                      double *__restrict__ dell) {
+  const int j = blockIdx.x, lane = threadIdx.x;
   double s = 0.0;
-  for (int m = 0; m < M; ++m) s += red[m * (Q + 1) + q1];
-  if (q1 == 0)
-    dsf2[0] = s / exp(2.0 * sf[0]);
-  else
-    dell[q1 - 1] = s / exp(3.0 * ls[q1 - 1]);
+  for (int m = lane; m < M; m += 64) s += red[m * (Q + 1) + j];
+  s = gpx_wave_sum(s);
+  if (lane == 0) {
+    if (j == 0)
+      dsf2[0] = s / exp(2.0 * sf[0]);
+    else
+      dell[j - 1] = s / exp(3.0 * ls[j - 1]);
+  }
 }
 
 // d_trace_MKzz_dhypers (kernels.py:447-475) with Kzz = SE(z, z) (no jitter):
@@ -810,10 +828,10 @@ int gpx_kfu_bwd(int N, int M, int Q, const double *dK, const double *K, const do
   double *red = work + (long)nblk * M * (2 * Q + 1);
   hipLaunchKernelGGL(kfu_bwd_partial, dim3(nblk), dim3(256), 0, st, N, M, Q, dK, K, x, z, ls, work, dx);
   GPX_CHECK_LAUNCH();
-  hipLaunchKernelGGL(kfu_bwd_final, dim3(gpx_cdiv((long)M * (Q + 1), 256)), dim3(256), 0, st, (int)nblk, M, Q,
+  hipLaunchKernelGGL(kfu_bwd_final, dim3(gpx_cdiv(M, 256 / (2 * Q + 1))), dim3(256), 0, st, (int)nblk, M, Q,
                      (const double *)work, z, ls, dZ, red);
   GPX_CHECK_LAUNCH();
-  hipLaunchKernelGGL(kfu_bwd_scalars, dim3(1), dim3(64), 0, st, M, Q, (const double *)red, ls, sf, dsf2, dell);
+  hipLaunchKernelGGL(kfu_bwd_scalars, dim3(Q + 1), dim3(64), 0, st, M, Q, (const double *)red, ls, sf, dsf2, dell);
   GPX_CHECK_LAUNCH();
   return GPX_OK;
 }

@@ -431,7 +431,9 @@ class SGP_Layer(object):
             -sp/2 theta2 theta2^T and dmu theta2^T = sp theta2 theta2^T combined),
           deta2 = beta Sigma-hat dmu_cav + sp mu  (Sigma dmu_post = sp Sigma theta2 = sp mu),
           deta1 = pack(R, beta dSuinv_cav + dSuinv_post) (pack is linear),
-          2 KiS^T dBhat - sum_d dBhat = (2 KiS - I)^T dBhat in one product."""
+          2 KiS^T dBhat - sum_d dBhat = (2 KiS - I)^T dBhat in one product,
+          the cavity's dm^T (kfu Ki) = dAhat Ki and sum_n dv_nd (kfu Ki)_n^T (kfu Ki)_n =
+            Ki^T dBhat Ki (M x M products in place of two contractions over the batch)."""
         N = self.N
         D, M = self.Dout, self.M
         B = kfu.shape[0]
@@ -440,15 +442,16 @@ class SGP_Layer(object):
         beta = self._beta
         scale_post = N * 1.0 / alpha - 1.0
         scale_cav = -N * 1.0 / alpha
-        kfuKi = mm(kfu, Ki)
         # grads wrt kfu: dm Ahat + 2 sum_d dv_nd (kfu Bhat_det[d])_n
         dkfu = mm(dm, self.Ahat)
         dops.wsum(dv, self._T_det, dkfu, alpha=2.0, beta=1.0)
         dsf2, dell, dzu, _ = dops.kfu_bwd(dkfu, kfu, x, self.zu, self.ls, self.sf)
         sumdv = colsum(dv, B * D, 1)
+        dAhat = mm(dm, kfu, tA=True)
+        dBhat = mm(dops.rowscale(dv, kfu), kfu, tA=True)
         # cavity: dmu_cav, dSu_cav (+ the normalising factor), then dSuinv_cav
-        dmucav = mm(dm, kfuKi, tA=True, E=th, gamma=scale_cav * beta)
-        dSucav = mm(dops.rowscale(dv, kfuKi), kfuKi, tA=True, E=self.Suhatinv, gamma=0.5 * scale_cav)
+        dmucav = mm(dAhat, Ki, E=th, gamma=scale_cav * beta)
+        dSucav = mm(Ki, mm(dBhat, Ki), tA=True, E=self.Suhatinv, gamma=0.5 * scale_cav)
         _add_outer(dSucav, th, th, -0.5 * scale_cav * beta * beta)
         _add_outer(dSucav, dmucav, th, beta)
         c = self._adjoint('reg', alpha)
@@ -457,9 +460,7 @@ class SGP_Layer(object):
         deta1 = dops.pack_dR(self.theta_1_R, lincomb(dSucav, beta, dS_cav, 1.0, c['dS_post']))
         deta2 = mm(self.Suhat, _col(dmucav), alpha=beta, E=_col(self.mu), gamma=scale_post).reshape(D, M)
         # Kuuinv: phi prior, Ahat, Bhat_det, cavity, posterior
-        dAhat = mm(dm, kfu, tA=True)
         dKi = mm(dAhat, self.muhat, tA=True, E=c['Kc'], gamma=1.0)
-        dBhat = mm(dops.rowscale(dv, kfu), kfu, tA=True)
         mm(c['KiS2'].reshape(D * M, M), dBhat.reshape(D * M, M), dKi, tA=True, beta=1.0)
         colsum(dS_cav, D, M * M, out=dKi.reshape(-1), beta=1.0)
         M_inner = _sandwich(Ki, dKi, Ki, alpha=-1.0)

@@ -260,11 +260,11 @@ def test_psi_bwd(N, M, Q):
         assert nrel(host(got), want) < 1e-12, name
 
 
-def test_kfu_bwd():
+@pytest.mark.parametrize('N,M,Q', [(150, 30, 3), (1000, 100, 10), (4100, 256, 16), (5000, 64, 5)])
+def test_kfu_bwd(N, M, Q):
     L = _lib()
-    say('kfu bwd')
+    say('kfu bwd', N, M, Q)
     rs = np.random.RandomState(5)
-    N, M, Q = 150, 30, 3
     x, z = rs.randn(N, Q), rs.randn(M, Q)
     ls, sf = np.log(rs.uniform(0.7, 1.5, Q)), np.array([0.1])
     K = O.se_kernel(ls, sf, x, z)
