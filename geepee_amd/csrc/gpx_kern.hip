@@ -139,55 +139,58 @@ __global__ __launch_bounds__(64) void kfu_bwd_scalars(int M, int Q, const double
 //   gsf = sum_ab M_ab K_ab, gls_q = sum_ab M_ab K_ab (z_aq - z_bq)^2 / (2 l_q),
 //   gz_aq = -sum_b (M_ab + M_ba) K_ab (z_aq - z_bq) / l_q.
 // One wave per row a (four per workgroup), its lanes over b; part[a] = [gsf_a, gls_a (Q)].
-__global__ __launch_bounds__(256) void dtrace_kernel(int M, int Q, const double *__restrict__ Mm,
+// Z (q-major, so lanes b read consecutive words) and 1/l_q are staged in LDS once per
+// workgroup; QT = Q fixes the per-lane arrays.
+template <int QT>
+__global__ __launch_bounds__(256) void dtrace_kernel(int M, const double *__restrict__ Mm,
                                                      const double *__restrict__ z,
                                                      const double *__restrict__ ls,
                                                      const double *__restrict__ sf,
                                                      double *__restrict__ gz, double *__restrict__ part) {
-  const int lane = threadIdx.x & 63, a = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (a >= M) return;
+  __shared__ double zs[QT][256];
+  __shared__ double ils[QT];
+  const int tid = threadIdx.x, lane = tid & 63, a = blockIdx.x * 4 + (tid >> 6);
+  for (int e = tid; e < M * QT; e += 256) zs[e % QT][e / QT] = z[e];
+  if (tid < QT) ils[tid] = 1.0 / exp(2.0 * ls[tid]);
+  __syncthreads();
+  if (a >= M) return;  // whole waves, after the only barrier
   const double sf2 = exp(2.0 * sf[0]);
-  double il[16], za[16];
+  double il[QT], za[QT], gl[QT], gzq[QT];
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    il[q] = (q < Q) ? 1.0 / exp(2.0 * ls[q]) : 0.0;
-    za[q] = (q < Q) ? z[a * Q + q] : 0.0;
+  for (int q = 0; q < QT; ++q) {
+    il[q] = ils[q];
+    za[q] = zs[q][a];
+    gl[q] = gzq[q] = 0.0;
   }
-  double g0 = 0.0, gl[16], gzq[16];
-#pragma unroll
-  for (int q = 0; q < 16; ++q) gl[q] = gzq[q] = 0.0;
+  double g0 = 0.0;
   for (int b = lane; b < M; b += 64) {
-    double r2 = 0.0;
+    double r2 = 0.0, d[QT];
 #pragma unroll
-    for (int q = 0; q < 16; ++q)
-      if (q < Q) {
-        const double d = za[q] - z[b * Q + q];
-        r2 += d * d * il[q];
-      }
+    for (int q = 0; q < QT; ++q) {
+      d[q] = za[q] - zs[q][b];
+      r2 += d[q] * d[q] * il[q];
+    }
     const double K = sf2 * exp(-0.5 * r2);
     const double mab = Mm[(long)a * M + b], mba = Mm[(long)b * M + a];
     g0 += mab * K;
 #pragma unroll
-    for (int q = 0; q < 16; ++q)
-      if (q < Q) {
-        const double d = za[q] - z[b * Q + q];
-        gl[q] += 0.5 * mab * K * d * d * il[q];
-        gzq[q] += (mab + mba) * K * d;
-      }
+    for (int q = 0; q < QT; ++q) {
+      gl[q] += 0.5 * mab * K * d[q] * d[q] * il[q];
+      gzq[q] += (mab + mba) * K * d[q];
+    }
   }
   g0 = gpx_wave_sum(g0);
 #pragma unroll
-  for (int q = 0; q < 16; ++q)
-    if (q < Q) {
-      gl[q] = gpx_wave_sum(gl[q]);
-      gzq[q] = gpx_wave_sum(gzq[q]);
-    }
-  if (lane == 0) part[(long)a * (Q + 1)] = g0;
+  for (int q = 0; q < QT; ++q) {
+    gl[q] = gpx_wave_sum(gl[q]);
+    gzq[q] = gpx_wave_sum(gzq[q]);
+  }
+  if (lane == 0) part[(long)a * (QT + 1)] = g0;
 #pragma unroll
-  for (int q = 0; q < 16; ++q)
-    if (q < Q && lane == q) {
-      part[(long)a * (Q + 1) + 1 + q] = gl[q];
-      gz[a * Q + q] = -gzq[q] * il[q];
+  for (int q = 0; q < QT; ++q)
+    if (lane == q) {
+      part[(long)a * (QT + 1) + 1 + q] = gl[q];
+      gz[a * QT + q] = -gzq[q] * il[q];
     }
 }
 
@@ -842,10 +845,18 @@ long gpx_dtrace_workspace(int M, int Q) { return (long)M * (Q + 1); }
 int gpx_dtrace_kzz(int M, int Q, const double *Mm, const double *z, const double *ls,
                    const double *sf, double *gsf, double *gls, double *gz, double *work,
                    void *stream) {
-  if (M < 1 || Q < 1 || Q > 16 || !Mm || !z || !ls || !sf || !gsf || !gls || !gz || !work)
+  if (M < 1 || M > 256 || Q < 1 || Q > 16 || !Mm || !z || !ls || !sf || !gsf || !gls || !gz || !work)
     return GPX_EINVAL;
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(dtrace_kernel, dim3(gpx_cdiv(M, 4)), dim3(256), 0, st, M, Q, Mm, z, ls, sf, gz, work);
+  switch (Q) {
+#define GPX_DT(QV)                                                                                        \
+  case QV:                                                                                                \
+    hipLaunchKernelGGL((dtrace_kernel<QV>), dim3(gpx_cdiv(M, 4)), dim3(256), 0, st, M, Mm, z, ls, sf, gz, work); \
+    break;
+    GPX_DT(1) GPX_DT(2) GPX_DT(3) GPX_DT(4) GPX_DT(5) GPX_DT(6) GPX_DT(7) GPX_DT(8)
+    GPX_DT(9) GPX_DT(10) GPX_DT(11) GPX_DT(12) GPX_DT(13) GPX_DT(14) GPX_DT(15) GPX_DT(16)
+#undef GPX_DT
+  }
   GPX_CHECK_LAUNCH();
   hipLaunchKernelGGL(dtrace_final, dim3(Q + 1), dim3(64), 0, st, M, Q, (const double *)work, gsf, gls);
   GPX_CHECK_LAUNCH();

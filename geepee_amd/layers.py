@@ -194,7 +194,8 @@ class SGP_Layer(object):
         (after prepare / compute_cavity, so they can run beside the forward pass):
           'reg' (natural parameters): the posterior chain dSuinv_post = -Su dSu Su with
             dSu = sp/2 (Sigma^-1 + theta2 theta2^T), Kc = sum_d dSuinv_post - (D/2) Kuu,
-            and KiS2 = 2 Ki Suhat - I;
+            KiS2 = 2 Ki Suhat - I, and the cavity's normalising-factor terms
+            Ecav = sc/2 (Sigma-hat^-1 - beta^2 theta2 theta2^T);
           'mm': Mc = sp sum_d Splusmm + sc sum_d Splusmmhat, T1 = -sp/2 Splusmm
             - sc beta/2 Splusmmhat, c2 = sp mu + sc beta muhat, KiS2 = 2 Ki Splusmmhat - I
         (sp = N/alpha - 1, sc = -N/alpha; aep_models.py:245-304, 412-511).  The backward
@@ -214,6 +215,7 @@ class SGP_Layer(object):
             c['Kc'] = lincomb(torch.empty_like(Ki), -0.5 * D, self.Kuu)
             colsum(c['dS_post'], D, M * M, out=c['Kc'].reshape(-1), beta=1.0)
             c['KiS2'] = mm(Ki, self.Suhat, alpha=2.0, E=_eye(M, Ki), gamma=-1.0)
+            c['Ecav'] = _plus_outer(self.Suhatinv, th, th, alpha=-0.5 * sc * beta * beta, gamma=0.5 * sc)
         else:
             c['Mc'] = colsum(self.Splusmm, D, M * M, alpha=sp).reshape(M, M)
             colsum(self.Splusmmhat, D, M * M, out=c['Mc'].reshape(-1), alpha=sc, beta=1.0)
@@ -450,11 +452,10 @@ class SGP_Layer(object):
         dAhat = mm(dm, kfu, tA=True)
         dBhat = mm(dops.rowscale(dv, kfu), kfu, tA=True)
         # cavity: dmu_cav, dSu_cav (+ the normalising factor), then dSuinv_cav
-        dmucav = mm(dAhat, Ki, E=th, gamma=scale_cav * beta)
-        dSucav = mm(Ki, mm(dBhat, Ki), tA=True, E=self.Suhatinv, gamma=0.5 * scale_cav)
-        _add_outer(dSucav, th, th, -0.5 * scale_cav * beta * beta)
-        _add_outer(dSucav, dmucav, th, beta)
         c = self._adjoint('reg', alpha)
+        dmucav = mm(dAhat, Ki, E=th, gamma=scale_cav * beta)
+        dSucav = mm(Ki, mm(dBhat, Ki), tA=True, E=c['Ecav'], gamma=1.0)
+        _add_outer(dSucav, dmucav, th, beta)
         dS_cav = _sandwich(self.Suhat, dSucav, self.Suhat, alpha=-1.0)
         # posterior chain (data independent, from adjoint_constants)
         deta1 = dops.pack_dR(self.theta_1_R, lincomb(dSucav, beta, dS_cav, 1.0, c['dS_post']))

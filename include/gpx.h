@@ -104,7 +104,8 @@ int gpx_kfu_bwd(int N, int M, int Q, const double *dK, const double *K, const do
                 const double *z, const double *ls, const double *sf, double *dsf2,
                 double *dell, double *dZ, double *dx, double *work, void *stream);
 
-/* d tr(Mm Kzz) / d(2 sf, 2 ls, z) for Kzz = SE(z, z) without jitter: gsf, gls (Q), gz (M x Q).
+/* d tr(Mm Kzz) / d(2 sf, 2 ls, z) for Kzz = SE(z, z) without jitter: gsf, gls (Q), gz (M x Q);
+ * M <= 256, Q <= 16.
  * Replaces d_trace_MKzz_dhypers (geepee/kernels.py:447-475). */
 long gpx_dtrace_workspace(int M, int Q);
 int gpx_dtrace_kzz(int M, int Q, const double *Mm, const double *z, const double *ls,

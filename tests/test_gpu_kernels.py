@@ -278,11 +278,11 @@ def test_kfu_bwd(N, M, Q):
         assert nrel(host(got), want) < 1e-12, name
 
 
-def test_dtrace():
+@pytest.mark.parametrize('M,Q', [(45, 4), (7, 1), (100, 5), (100, 10), (256, 16)])
+def test_dtrace(M, Q):
     L = _lib()
-    say('dtrace')
+    say('dtrace', M, Q)
     rs = np.random.RandomState(6)
-    M, Q = 45, 4
     z = rs.randn(M, Q)
     ls, sf = np.log(rs.uniform(0.7, 1.5, Q)), np.array([0.3])
     Mm = rs.randn(M, M)
