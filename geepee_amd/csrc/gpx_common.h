@@ -45,6 +45,30 @@ __device__ __forceinline__ double gpx_wave_sum(double v) {
   return v;
 }
 
+// v + v[partner] for a DPP lane pattern within each 16-lane row (CTRL: 0x140 row_mirror,
+// lane r <- 15 - r; 0x141 row_half_mirror, r <- 7 - r in each half; 0x4E quad lane xor 2;
+// 0xB1 quad lane xor 1).  Every lane of the wave must be active.
+template <int CTRL>
+__device__ __forceinline__ double gpx_dpp_add(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  const int plo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
+  const int phi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+  return v + __hiloint2double(phi, plo);
+}
+
+// Wave64 sum with every step on the vector ALU (no LDS permutes): the two cross-row lane
+// swaps, then within each row mirror, half-mirror, xor 2, xor 1, whose partner sets cover
+// the row's 16 lanes.  Each step adds a lane and its partner, so partners hold the same
+// bits and all 64 lanes end with the same total.  For code where the whole wave is active.
+__device__ __forceinline__ double gpx_wave_sum_dpp(double v) {
+  v = gpx_swap_add<32>(v);
+  v = gpx_swap_add<16>(v);
+  v = gpx_dpp_add<0x140>(v);
+  v = gpx_dpp_add<0x141>(v);
+  v = gpx_dpp_add<0x4E>(v);
+  return gpx_dpp_add<0xB1>(v);
+}
+
 // Block-wide sum for blockDim.x a multiple of 64 (<= 1024).  `red` is LDS scratch of
 // at least 16 doubles.  Every thread gets the total.
 __device__ __forceinline__ double gpx_block_sum(double v, double *red) {

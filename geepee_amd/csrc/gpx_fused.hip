@@ -1200,16 +1200,27 @@ __global__ __launch_bounds__(256) void fused_rows_finish(
 #pragma unroll
   for (int i = 0; i < FRED; ++i) c[i] = 0.0;
   // T = sum of the S slots of Tpart for this lane's columns m = lane + 64 j (j < 4, M <= 256),
-  // the four columns' loads of a slot issued together (each column summed in slot order)
+  // each column summed in slot order; the loads of four slots at clamped (always valid)
+  // indices issue together, and the clamped lanes' sums are never read
   constexpr int MJ = (FM_MAX + 63) / 64;
+  const int nj = (M + 63) >> 6;
   double Tm[MJ];
 #pragma unroll
   for (int j = 0; j < MJ; ++j) Tm[j] = 0.0;
-  for (int s = 0; s < S; ++s) {
-    const double *row = Tpart + ((long)s * N + n) * M;
+  for (int s0 = 0; s0 < S; s0 += 4) {
+    double tv[4][MJ];
 #pragma unroll
-    for (int j = 0; j < MJ; ++j)
-      if (lane + 64 * j < M) Tm[j] += row[lane + 64 * j];
+    for (int u = 0; u < 4; ++u) {
+      const double *row = Tpart + ((long)min(s0 + u, S - 1) * N + n) * M;
+#pragma unroll
+      for (int j = 0; j < MJ; ++j) tv[u][j] = j < nj ? row[min(lane + 64 * j, M - 1)] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (s0 + u < S) {
+#pragma unroll
+        for (int j = 0; j < MJ; ++j) Tm[j] += tv[u][j];
+      }
   }
 #pragma unroll
   for (int j = 0; j < MJ; ++j) {
@@ -1234,7 +1245,7 @@ __global__ __launch_bounds__(256) void fused_rows_finish(
 #pragma unroll
   for (int i = 0; i < FRED; ++i) {
     if (i >= 2 && (i - 2) % QN >= Q) continue;
-    c[i] = gpx_wave_sum(c[i]);
+    c[i] = gpx_wave_sum_dpp(c[i]);
   }
   double out_S1 = 0.0, out_TZ2 = 0.0, out_B1 = 0.0, out_C1 = 0.0;
 #pragma unroll
@@ -1260,7 +1271,7 @@ __global__ __launch_bounds__(256) void fused_rows_finish(
 #pragma unroll
   for (int q = 0; q < QN; ++q) {
     if (q >= Q) continue;
-    const double t = gpx_wave_sum(Vq[q]);
+    const double t = gpx_wave_sum_dpp(Vq[q]);
     if (q == lane) Vsel = t;
   }
   if (lane < Q) {
@@ -1291,9 +1302,10 @@ __global__ __launch_bounds__(256) void fused_rows_finish(
   }
 }
 
-// psi1, m = psi1 Ahat^T and v = sf2 + sum_s part - m^2 for 16 points per block.
+// psi1, m = psi1 Ahat^T and v = sf2 + sum_s part - m^2 for FIN_PTS points per block.
 // Points per block of the forward finish (8: twice the blocks of 16, so the latency-bound
-// psi1 and mean loops have two waves per SIMD at N = 4096).
+// psi1 and mean loops have two waves per SIMD at N = 4096).  The split partials' loads
+// issue four at a time (summed in split order).
 constexpr int FIN_PTS = 8;
 
 __global__ __launch_bounds__(256) void fused_fwd_finish(int N, int M, int Q, int D, const double *__restrict__ mx,
@@ -1343,6 +1355,15 @@ __global__ __launch_bounds__(256) void fused_fwd_finish(int N, int M, int Q, int
   for (int e = tid; e < FIN_PTS * D; e += 256) {
     const int nn = e / D, d = e % D, n = n0 + nn;
     if (n >= N) continue;
+    double s = 0.0;
+    for (int sp0 = 0; sp0 < S; sp0 += 4) {
+      double pv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) pv[u] = part[((long)min(sp0 + u, S - 1) * N + n) * D + d];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (sp0 + u < S) s += pv[u];
+    }
     // four independent partial sums (k mod 4): a 128-long chain of dependent FMAs was
     // this loop's latency
     const double *arow = Ahat + (long)d * M;
@@ -1355,8 +1376,6 @@ __global__ __launch_bounds__(256) void fused_fwd_finish(int N, int M, int Q, int
     }
     for (int u = 0; k < M; ++k, ++u) m4[u] += p1[nn][k] * arow[k];
     const double m = (m4[0] + m4[1]) + (m4[2] + m4[3]);
-    double s = 0.0;
-    for (int sp = 0; sp < S; ++sp) s += part[((long)sp * N + n) * D + d];
     mout[(long)n * D + d] = m;
     vout[(long)n * D + d] = sf2 + s - m * m;
   }
