@@ -35,16 +35,6 @@ __device__ __forceinline__ double gpx_swap_add(double v) {
   }
 }
 
-// Wave64 sum reduction (every lane gets the total): the xor butterfly 32, 16, 8, 4, 2, 1,
-// its first two steps as lane swaps.
-__device__ __forceinline__ double gpx_wave_sum(double v) {
-  v = gpx_swap_add<32>(v);
-  v = gpx_swap_add<16>(v);
-#pragma unroll
-  for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
 // v + v[partner] for a DPP lane pattern within each 16-lane row (CTRL: 0x140 row_mirror,
 // lane r <- 15 - r; 0x141 row_half_mirror, r <- 7 - r in each half; 0x4E quad lane xor 2;
 // 0xB1 quad lane xor 1).  Every lane of the wave must be active.
@@ -56,11 +46,12 @@ __device__ __forceinline__ double gpx_dpp_add(double v) {
   return v + __hiloint2double(phi, plo);
 }
 
-// Wave64 sum with every step on the vector ALU (no LDS permutes): the two cross-row lane
-// swaps, then within each row mirror, half-mirror, xor 2, xor 1, whose partner sets cover
-// the row's 16 lanes.  Each step adds a lane and its partner, so partners hold the same
-// bits and all 64 lanes end with the same total.  For code where the whole wave is active.
-__device__ __forceinline__ double gpx_wave_sum_dpp(double v) {
+// Wave64 sum reduction (every lane gets the total), every step on the vector ALU (no LDS
+// permutes): the two cross-row lane swaps, then within each row mirror, half-mirror,
+// xor 2, xor 1, whose partner sets cover the row's 16 lanes.  Each step adds a lane and
+// its partner, so partners hold the same bits and all 64 lanes end with the same total.
+// Called with the whole wave active (early exits are per wave).
+__device__ __forceinline__ double gpx_wave_sum(double v) {
   v = gpx_swap_add<32>(v);
   v = gpx_swap_add<16>(v);
   v = gpx_dpp_add<0x140>(v);

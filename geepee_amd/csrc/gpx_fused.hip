@@ -1245,7 +1245,7 @@ __global__ __launch_bounds__(256) void fused_rows_finish(
 #pragma unroll
   for (int i = 0; i < FRED; ++i) {
     if (i >= 2 && (i - 2) % QN >= Q) continue;
-    c[i] = gpx_wave_sum_dpp(c[i]);
+    c[i] = gpx_wave_sum(c[i]);
   }
   double out_S1 = 0.0, out_TZ2 = 0.0, out_B1 = 0.0, out_C1 = 0.0;
 #pragma unroll
@@ -1271,7 +1271,7 @@ __global__ __launch_bounds__(256) void fused_rows_finish(
 #pragma unroll
   for (int q = 0; q < QN; ++q) {
     if (q >= Q) continue;
-    const double t = gpx_wave_sum_dpp(Vq[q]);
+    const double t = gpx_wave_sum(Vq[q]);
     if (q == lane) Vsel = t;
   }
   if (lane < Q) {
