@@ -409,17 +409,17 @@ __global__ __launch_bounds__(256) void gauss_tilted_kernel(int B, int D, const d
   }
 }
 
-__global__ void gauss_tilted_finish(int nblk, int B, int D, const double *__restrict__ part,
-                                    const double *__restrict__ sn, double alpha, double scale,
-                                    double *__restrict__ scal) {
-  if (threadIdx.x != 0) return;
+// One wave, lane b holding block b's partials (nblk <= 64), then fixed-order wave sums.
+__global__ __launch_bounds__(64) void gauss_tilted_finish(int nblk, int B, int D, const double *__restrict__ part,
+                                                          const double *__restrict__ sn, double alpha,
+                                                          double scale, double *__restrict__ scal) {
+  const int b = threadIdx.x;
+  double lz = b < nblk ? part[2 * b] : 0.0, sdv = b < nblk ? part[2 * b + 1] : 0.0;
+  lz = gpx_wave_sum(lz);
+  sdv = gpx_wave_sum(sdv);
+  if (b != 0) return;
   const double sn2 = exp(2.0 * sn[0]);
   const double two_pi = 6.283185307179586;
-  double lz = 0.0, sdv = 0.0;
-  for (int b = 0; b < nblk; ++b) {
-    lz += part[2 * b];
-    sdv += part[2 * b + 1];
-  }
   lz += (double)B * D * (0.5 * log(two_pi * sn2 / alpha) - 0.5 * alpha * log(two_pi * sn2));
   scal[0] = lz;
   scal[1] = (sdv * 2.0 * sn2 / alpha + (double)B * D * (1.0 - alpha)) * scale;
@@ -464,16 +464,16 @@ __global__ __launch_bounds__(256) void transition_kernel(int B, int D, const dou
   }
 }
 
-__global__ void transition_finish(int nblk, int B, int D, const double *__restrict__ part,
-                                  const double *__restrict__ sn, double alpha, double scale,
-                                  double *__restrict__ scal) {
-  if (threadIdx.x != 0) return;
+// One wave, lane b holding block b's partials (nblk <= 64), then fixed-order wave sums.
+__global__ __launch_bounds__(64) void transition_finish(int nblk, int B, int D, const double *__restrict__ part,
+                                                        const double *__restrict__ sn, double alpha,
+                                                        double scale, double *__restrict__ scal) {
+  const int b = threadIdx.x;
+  double lz = b < nblk ? part[2 * b] : 0.0, sdv = b < nblk ? part[2 * b + 1] : 0.0;
+  lz = gpx_wave_sum(lz);
+  sdv = gpx_wave_sum(sdv);
+  if (b != 0) return;
   const double sn2 = exp(2.0 * sn[0]);
-  double lz = 0.0, sdv = 0.0;
-  for (int b = 0; b < nblk; ++b) {
-    lz += part[2 * b];
-    sdv += part[2 * b + 1];
-  }
   scal[0] = scale * lz;
   scal[1] = sdv * 2.0 * sn2 / alpha + scale * (double)B * D * (1.0 - alpha);
 }

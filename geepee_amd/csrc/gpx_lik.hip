@@ -53,15 +53,15 @@ __global__ __launch_bounds__(256) void mc_reparam_kernel(int S, long nq, const d
 
 // Block partials [logZ, sum dv] -> scal[0] = lz_scale * sum logZ + lz_add,
 // scal[1] = (sum dv * dv_mul + dv_add) * dv_scale.
-__global__ void lik_finish(int nblk, const double *__restrict__ part, double lz_scale, double lz_add,
-                           const double *__restrict__ sn, double alpha, double dv_add, double dv_scale,
-                           double *__restrict__ scal) {
-  if (threadIdx.x != 0) return;
-  double lz = 0.0, sdv = 0.0;
-  for (int b = 0; b < nblk; ++b) {
-    lz += part[2 * b];
-    sdv += part[2 * b + 1];
-  }
+// One wave, lane b holding block b's partials (nblk <= 64), then fixed-order wave sums.
+__global__ __launch_bounds__(64) void lik_finish(int nblk, const double *__restrict__ part, double lz_scale,
+                                                 double lz_add, const double *__restrict__ sn, double alpha,
+                                                 double dv_add, double dv_scale, double *__restrict__ scal) {
+  const int b = threadIdx.x;
+  double lz = b < nblk ? part[2 * b] : 0.0, sdv = b < nblk ? part[2 * b + 1] : 0.0;
+  lz = gpx_wave_sum(lz);
+  sdv = gpx_wave_sum(sdv);
+  if (b != 0) return;
   scal[0] = lz_scale * lz + lz_add;
   if (sn) scal[1] = (sdv * 2.0 * exp(2.0 * sn[0]) / alpha + dv_add) * dv_scale;
 }

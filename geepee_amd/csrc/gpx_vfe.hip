@@ -42,14 +42,14 @@ __device__ __forceinline__ void partials2(double a, double b, double *part) {
 }
 
 // scal[0] = s0 * sum(part even) + a0, scal[1] = s1 * sum(part odd) + a1 (when want1)
-__global__ void finish2(int nblk, const double *__restrict__ part, double s0, double a0, double s1, double a1,
-                        int want1, double *__restrict__ scal) {
-  if (threadIdx.x != 0) return;
-  double x = 0.0, y = 0.0;
-  for (int b = 0; b < nblk; ++b) {
-    x += part[2 * b];
-    y += part[2 * b + 1];
-  }
+// One wave, lane b holding block b's partials (nblk <= 64), then fixed-order wave sums.
+__global__ __launch_bounds__(64) void finish2(int nblk, const double *__restrict__ part, double s0, double a0,
+                                              double s1, double a1, int want1, double *__restrict__ scal) {
+  const int b = threadIdx.x;
+  double x = b < nblk ? part[2 * b] : 0.0, y = b < nblk ? part[2 * b + 1] : 0.0;
+  x = gpx_wave_sum(x);
+  y = gpx_wave_sum(y);
+  if (b != 0) return;
   scal[0] = s0 * x + a0;
   if (want1) scal[1] = s1 * y + a1;
 }

@@ -97,14 +97,14 @@ __global__ __launch_bounds__(256) void lvm_grad_kernel(int N, int B, int Q, cons
 }
 
 // scal[0] from the per-block partials, summed in block order.
-__global__ void lvm_grad_finish(int nblk, int N, int Q, double pm, double pv, double sc_cav, double sc_post,
-                                double *__restrict__ scal) {
-  if (threadIdx.x != 0) return;
-  double pc = 0.0, pp = 0.0;
-  for (int b = 0; b < nblk; ++b) {
-    pc += scal[2 + 2 * b];
-    pp += scal[3 + 2 * b];
-  }
+// One wave, lane b holding block b's partials (nblk <= 64), then fixed-order wave sums.
+__global__ __launch_bounds__(64) void lvm_grad_finish(int nblk, int N, int Q, double pm, double pv, double sc_cav,
+                                                      double sc_post, double *__restrict__ scal) {
+  const int b = threadIdx.x;
+  double pc = b < nblk ? scal[2 + 2 * b] : 0.0, pp = b < nblk ? scal[3 + 2 * b] : 0.0;
+  pc = gpx_wave_sum(pc);
+  pp = gpx_wave_sum(pp);
+  if (b != 0) return;
   const double prior = 0.5 * (pm * pm / pv + log(pv)) * (double)N * Q;
   scal[0] = prior + sc_cav * pc + sc_post * pp;
 }
@@ -186,13 +186,14 @@ __global__ __launch_bounds__(256) void ssm_grad_kernel(
   }
 }
 
-__global__ void ssm_grad_finish(int nblk, int Q, double px1, double px2, double *__restrict__ scal) {
-  if (threadIdx.x != 0) return;
-  double pp = 0.0, pc = 0.0;
-  for (int b = 0; b < nblk; ++b) {
-    pp += scal[2 + 2 * b];
-    pc += scal[3 + 2 * b];
-  }
+// One wave, lane b holding block b's partials (nblk <= 64), then fixed-order wave sums.
+__global__ __launch_bounds__(64) void ssm_grad_finish(int nblk, int Q, double px1, double px2,
+                                                      double *__restrict__ scal) {
+  const int b = threadIdx.x;
+  double pp = b < nblk ? scal[2 + 2 * b] : 0.0, pc = b < nblk ? scal[3 + 2 * b] : 0.0;
+  pp = gpx_wave_sum(pp);
+  pc = gpx_wave_sum(pc);
+  if (b != 0) return;
   const double m0 = px1 / px2, v0 = 1.0 / px2;
   scal[0] = 0.5 * Q * (m0 * m0 / v0 + log(v0)) + pp + pc;
 }
@@ -251,11 +252,11 @@ __global__ __launch_bounds__(256) void hidden_grad_kernel(long n, const double *
   if (threadIdx.x == 0) scal[2 + blockIdx.x] = phi;
 }
 
-__global__ void hidden_grad_finish(int nblk, double *__restrict__ scal) {
-  if (threadIdx.x != 0) return;
-  double t = 0.0;
-  for (int b = 0; b < nblk; ++b) t += scal[2 + b];
-  scal[0] = t;
+// One wave, lane b holding block b's partial (nblk <= 64), then a fixed-order wave sum.
+__global__ __launch_bounds__(64) void hidden_grad_finish(int nblk, double *__restrict__ scal) {
+  const int b = threadIdx.x;
+  const double t = gpx_wave_sum(b < nblk ? scal[2 + b] : 0.0);
+  if (b == 0) scal[0] = t;
 }
 
 }  // namespace
