@@ -273,20 +273,19 @@ __global__ __launch_bounds__(256) void gemm_fullk_kernel(
   }
 }
 
-// k == 1: an outer product.  One thread per output (a block per row segment) with the
-// product and epilogue of the MFMA path, which for k = 1 adds exact zeros to round(a b),
-// so the results are the same bits at a fraction of the latency.
+// k == 1: an outer product.  One thread per output over the flattened (batch, row,
+// column) index, with the product and epilogue of the MFMA path, which for k = 1 adds
+// exact zeros to round(a b), so the results are the same bits at a fraction of the latency.
 __global__ __launch_bounds__(256) void rank1_kernel(int m, int n, int tA, int tB, double alpha,
                                                     const double *__restrict__ A, int lda, long sA,
                                                     const double *__restrict__ B, int ldb, long sB, GemmEpi ep,
                                                     double *__restrict__ C, int ldc, long sC, int batch) {
-  // grid: x over the rows (and columns in chunks of 256), y over the batch
-  const int b = blockIdx.y, i = blockIdx.x % m;
+  const long mn = (long)m * n, e = blockIdx.x * 256L + threadIdx.x;
+  if (e >= mn * batch) return;
+  const int b = (int)(e / mn), r = (int)(e - b * mn), i = r / n, j = r - i * n;
   const double a = tA ? A[b * sA + i] : A[b * sA + (long)i * lda];
-  for (int j = (blockIdx.x / m) * 256 + threadIdx.x; j < n; j += 256 * ((gridDim.x + m - 1) / m)) {
-    const double bb = tB ? B[b * sB + (long)j * ldb] : B[b * sB + j];
-    epi_store(C + b * sC + (long)i * ldc + j, alpha * (a * bb), ep, b, i, j);
-  }
+  const double bb = tB ? B[b * sB + (long)j * ldb] : B[b * sB + j];
+  epi_store(C + b * sC + (long)i * ldc + j, alpha * (a * bb), ep, b, i, j);
 }
 
 // n == 1: matrix-vector products.  One wave per output row: lanes stride over k, then a
@@ -337,9 +336,8 @@ int gpx_gemm_batched_ex(int transA, int transB, int m, int n, int k, double alph
   }
   // outer products and matrix-vector products (automatic tile choice only)
   if (tile == 0 && splits == 1 && k == 1) {
-    if (batch > 65535) return GPX_EINVAL;
-    hipLaunchKernelGGL(rank1_kernel, dim3(m * gpx_cdiv(n, 256), batch), dim3(256), 0, st, m, n, transA, transB,
-                       alpha, A, lda, strideA, B, ldb, strideB, ep, C, ldc, strideC, batch);
+    hipLaunchKernelGGL(rank1_kernel, dim3(gpx_cdiv((long)batch * m * n, 256)), dim3(256), 0, st, m, n, transA,
+                       transB, alpha, A, lda, strideA, B, ldb, strideB, ep, C, ldc, strideC, batch);
     GPX_CHECK_LAUNCH();
     return GPX_OK;
   }

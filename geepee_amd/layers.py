@@ -196,7 +196,7 @@ class SGP_Layer(object):
             dSu = sp/2 (Sigma^-1 + theta2 theta2^T), Kc = sum_d dSuinv_post - (D/2) Kuu,
             KiS2 = 2 Ki Suhat - I, and the cavity's normalising-factor terms
             Ecav = sc/2 (Sigma-hat^-1 - beta^2 theta2 theta2^T);
-          'mm': Mc = sp sum_d Splusmm + sc sum_d Splusmmhat, T1 = -sp/2 Splusmm
+          'mm': Mc = sp sum_d Splusmm + sc sum_d Splusmmhat, McKi = Mc Ki, T1 = -sp/2 Splusmm
             - sc beta/2 Splusmmhat, c2 = sp mu + sc beta muhat, KiS2 = 2 Ki Splusmmhat - I
         (sp = N/alpha - 1, sc = -N/alpha; aep_models.py:245-304, 412-511).  The backward
         passes use them when present."""
@@ -219,6 +219,7 @@ class SGP_Layer(object):
         else:
             c['Mc'] = colsum(self.Splusmm, D, M * M, alpha=sp).reshape(M, M)
             colsum(self.Splusmmhat, D, M * M, out=c['Mc'].reshape(-1), alpha=sc, beta=1.0)
+            c['McKi'] = mm(c['Mc'], Ki)
             c['T1'] = lincomb(torch.empty_like(self.Splusmm), -0.5 * sp, self.Splusmm, -0.5 * sc * beta,
                               self.Splusmmhat)
             c['c2'] = lincomb(torch.empty_like(self.mu), sp, self.mu, sc * beta, self.muhat)
@@ -588,10 +589,10 @@ class SGP_Layer(object):
         mm(dAhat, self.muhat, dKi, tA=True, beta=1.0)
         # 2 sum_d KiS_d^T dBhat_d - sum_d dBhat_d as one product with KiS2 = 2 KiS - I
         mm(c['KiS2'].reshape(D * M, M), dBhat.reshape(D * M, M), dKi, tA=True, beta=1.0)
-        Minner = lincomb(torch.empty_like(dKi), 1.0, c['Mc'], -2.0, dKi)
         dtheta1 = lincomb(torch.empty_like(dvcavinv), beta, dvcavinv, 1.0, c['T1'])
         deta1 = dops.pack_dR(self.theta_1_R, dtheta1)
-        M_all = _sandwich(Ki, Minner, Ki, alpha=0.5, E=Ki, gamma=0.5 * D)
+        # M_all = Ki (Mc - 2 dKi) Ki / 2 + (D/2) Ki, with Mc Ki from the adjoint constants
+        M_all = mm(Ki, mm(dKi, Ki, alpha=-2.0, E=c['McKi'], gamma=1.0), alpha=0.5, E=Ki, gamma=0.5 * D)
         gsf, gls, gz = dops.dtrace(M_all, self.zu, self.ls, self.sf)
         dls, dsf = dops.hyper(self.ls, self.sf, dell, gls, dsf2, sumdv, gsf)
         dzu = lincomb(torch.empty_like(dzu), 1.0, dzu, 1.0, gz)
