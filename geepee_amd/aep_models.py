@@ -514,6 +514,9 @@ class SDGPR(Base_Model):
         phis, ready = [], []
         for i, (layer, st) in enumerate(zip(self.sgp_layers, self._streams)):
             st.wait_stream(cur)
+            j = self._setup_after(i)
+            if j is not None:
+                st.wait_event(ready[j])
             with torch.cuda.stream(st):
                 layer.finish_prepare(*precs[i], alpha=alpha)
                 phis.append(layer.compute_phi(alpha))
@@ -525,6 +528,14 @@ class SDGPR(Base_Model):
                 if not self._defer_adjoint(i):
                     layer.adjoint_constants('reg' if i == 0 else 'mm', alpha)
         return phis, ready
+
+    def _setup_after(self, i):
+        """Index of the layer whose q(u) set-up layer i's set-up waits for (None: it starts
+        right after the batched inverses).  The first layer's set-up runs alone and its
+        forward starts sooner; the others follow beside that forward (10-5-5-1 deep GP at
+        M = 100, same-box replays: 0.759-0.764 ms against 0.776-0.798 ms with all three set-ups
+        together, 0.83 ms with each waiting for the previous one)."""
+        return 0 if i > 0 else None
 
     def _defer_adjoint(self, i):
         """Whether layer i's adjoint constants are issued on its stream after the forwards
