@@ -449,12 +449,7 @@ class SDGPR(Base_Model):
                 ms.append(m); vs.append(v); p1.append(a); p2.append(b)
         last_mm = self.L > 1
         if ready is not None:
-            for i in range(self.L - 1, -1, -1):
-                if self._defer_adjoint(i):
-                    st = self._streams[i]
-                    st.wait_stream(torch.cuda.current_stream())
-                    with torch.cuda.stream(st):
-                        self.sgp_layers[i].adjoint_constants('reg' if i == 0 else 'mm', alpha)
+            self._issue_deferred_adjoint(alpha)
         dmi, dvi, dma, scal = self.lik_layer.compute_log_Z(ms[-1], vs[-1], yb, alpha, scale_logZ,
                                                            want_dmall=last_mm)
         grads = {}
@@ -536,6 +531,16 @@ class SDGPR(Base_Model):
         M = 100, same-box replays: 0.759-0.764 ms against 0.776-0.798 ms with all three set-ups
         together, 0.83 ms with each waiting for the previous one)."""
         return 0 if i > 0 else None
+
+    def _issue_deferred_adjoint(self, alpha):
+        """The deferred layers' adjoint constants, each on its layer's stream after the work
+        issued so far on the current one."""
+        for i in range(self.L - 1, -1, -1):
+            if self._defer_adjoint(i):
+                st = self._streams[i]
+                st.wait_stream(torch.cuda.current_stream())
+                with torch.cuda.stream(st):
+                    self.sgp_layers[i].adjoint_constants('reg' if i == 0 else 'mm', alpha)
 
     def _defer_adjoint(self, i):
         """Whether layer i's adjoint constants are issued on its stream after the forwards
